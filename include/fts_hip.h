@@ -1,0 +1,152 @@
+/* C ABI of libfts_hip.so: the MI355X (gfx950) chain-propagator and concentration path.
+ *
+ * Plain pointers, sizes and opaque handles; no C++ or torch types. Every entry
+ * point returns FTS_OK (0) or FTS_ERROR (nonzero) and leaves a message for
+ * fts_last_error() (thread-local). Device pointers are HIP device addresses on the
+ * context's device; "on_device" flags say whether a double* argument is a device
+ * or a host address. All work is enqueued on the context's stream; calls that
+ * return host values synchronise that stream.
+ *
+ * Each entry point names the reference interface it stands behind.
+ */
+#ifndef FTS_HIP_H
+#define FTS_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct fts_context fts_context;
+typedef struct fts_grid fts_grid;
+typedef struct fts_solver fts_solver;
+typedef struct fts_am fts_am;
+
+enum { FTS_OK = 0, FTS_ERROR = 1 };
+enum { FTS_CONTINUOUS = 0, FTS_DISCRETE = 1 };
+enum { FTS_FP64 = 0, FTS_FP32 = 1 };
+/* kind of the initial slice of a propagator (PropagatorCode.cpp:117-247 key forms) */
+enum { FTS_LEAF = 0, FTS_LEAF_QINIT = 1, FTS_JUNCTION = 2, FTS_AGGREGATED = 3 };
+/* sources of fts_solver_sum_concentrations */
+enum { FTS_SRC_BLOCK = 0, FTS_SRC_SOLVENT = 1 };
+
+/* Last error message of the calling thread ("" if none). */
+const char* fts_last_error(void);
+/* ABI version, bumped on incompatible changes. */
+int fts_abi_version(void);
+
+/* ---- context: one device and one stream (replaces CudaCommon's singleton,
+ *      CudaCommon.cu:15-113) */
+int fts_context_create(int device, fts_context** out);
+int fts_context_destroy(fts_context* ctx);
+int fts_context_synchronize(fts_context* ctx);
+/* hipStream_t of the context, returned as void* */
+int fts_context_stream(fts_context* ctx, void** stream);
+/* device description ("name, gcnArch, CUs, memory") into buf (AbstractFactory::display_info) */
+int fts_context_info(fts_context* ctx, char* buf, int buflen);
+
+/* ---- grid: ComputationBox (ComputationBox.h:22-76, CudaComputationBox.cu:17-118)
+ * dim 1..3, nx and lx of length dim, mask NULL or nx-product values of 0/1. */
+int fts_grid_create(fts_context* ctx, int dim, const int* nx, const double* lx, const double* mask, fts_grid** out);
+int fts_grid_destroy(fts_grid* g);
+/* ComputationBox::set_lx (ComputationBox.cpp:229-262) */
+int fts_grid_set_lx(fts_grid* g, const double* lx);
+int fts_grid_volume(fts_grid* g, double* out);
+/* ComputationBox::integral / inner_product (ComputationBox.cpp:265-279) */
+int fts_grid_integral(fts_grid* g, const double* a, int on_device, double* out);
+int fts_grid_inner_product(fts_grid* g, const double* a, const double* b, int on_device, double* out);
+/* Batched real 3D FFT on device buffers, the transform of MklFFT3D::forward /
+ * backward (MklFFT3D.cpp:55-79): forward unnormalised real -> half complex
+ * [nx][ny][nz/2+1] (interleaved re,im), backward half complex -> real divided by
+ * the grid size. precision FTS_FP64 (double) or FTS_FP32 (float). In-place is not
+ * allowed. batch fields are contiguous. */
+int fts_grid_fft_forward(fts_grid* g, int precision, const void* d_in, void* d_out, int batch);
+int fts_grid_fft_backward(fts_grid* g, int precision, const void* d_in, void* d_out, int batch);
+
+/* ---- propagator computation plan: the output of PropagatorAnalyzer
+ * (PropagatorAnalyzer.cpp:142-269) and Scheduler (Scheduler.cpp:11-190) as flat
+ * arrays. Propagators are indexed in ComparePropagatorKey order, blocks in
+ * std::map<(polymer, left key, right key)> order, monomers in name order. */
+typedef struct {
+    int model;                   /* FTS_CONTINUOUS or FTS_DISCRETE */
+    double ds;
+    int n_monomers;
+    const double* segment_length; /* statistical segment length a_m; bond b^2 = a_m^2 */
+    int n_props;
+    const int* prop_monomer;
+    const int* prop_max_n;       /* max_n_segment */
+    const int* prop_kind;        /* FTS_LEAF ... FTS_AGGREGATED */
+    const int* prop_qinit;       /* q_init index for FTS_LEAF_QINIT, else -1 */
+    const int* prop_dep_off;     /* into dep_* */
+    const int* prop_n_deps;
+    const int* dep_prop;
+    const int* dep_n;
+    const int* dep_rep;
+    const int* prop_je_off;      /* into je (discrete junction_ends) */
+    const int* prop_n_je;
+    const int* je;
+    int n_blocks;
+    const int* blk_polymer;
+    const int* blk_left;
+    const int* blk_right;
+    const int* blk_n_left;
+    const int* blk_n_right;
+    const int* blk_rep;
+    const int* blk_monomer;
+    const int* blk_n_vu;
+    int n_polymers;
+    const double* pol_vf;
+    const double* pol_alpha;
+    int n_solvents;
+    const double* sol_vf;
+    const int* sol_monomer;
+    int n_qinit;
+    int n_slices;                /* schedule: slice s holds jobs slice_off[s] .. slice_off[s+1]-1 */
+    const int* slice_off;
+    const int* job_prop;
+    const int* job_from;
+    const int* job_to;
+} fts_plan;
+
+/* CpuComputationContinuous / CpuComputationDiscrete constructors
+ * (CpuComputationContinuous.cpp:9-105, CpuComputationDiscrete.cpp:8-137);
+ * precision FTS_FP64 or FTS_FP32 (propagators and FFTs; Q, concentrations and
+ * stress accumulate in FP64 either way). The plan arrays are copied. */
+int fts_solver_create(fts_grid* g, const fts_plan* plan, int precision, fts_solver** out);
+int fts_solver_destroy(fts_solver* s);
+/* PropagatorComputation::update_laplacian_operator after fts_grid_set_lx */
+int fts_solver_update_laplacian(fts_solver* s);
+/* PropagatorComputation::compute_propagators (CpuComputationContinuous.cpp:148-369,
+ * CpuComputationDiscrete.cpp:192-582). w: n_monomers field pointers, q_init:
+ * n_qinit pointers (or NULL when n_qinit == 0), each of grid size, FP64. */
+int fts_solver_compute_propagators(fts_solver* s, const double* const* w, const double* const* q_init,
+                                   int on_device);
+/* compute_concentrations (CpuComputationContinuous.cpp:371-443, CpuComputationDiscrete.cpp:584-663) */
+int fts_solver_compute_concentrations(fts_solver* s);
+/* compute_stress (CpuComputationContinuous.cpp:643-726, CpuComputationDiscrete.cpp:857-996) */
+int fts_solver_compute_stress(fts_solver* s);
+/* get_total_partition / get_solvent_partition */
+int fts_solver_get_partition(fts_solver* s, int polymer, double* out);
+int fts_solver_get_solvent_partition(fts_solver* s, int solvent, double* out);
+/* dq_dl[polymer][d] for d < dim (before the phi/alpha/Q normalisation of
+ * PropagatorComputation::get_stress, PropagatorComputation.cpp:35-52) */
+int fts_solver_get_dq_dl(fts_solver* s, int polymer, double* out);
+/* out = sum_t scale[t] * phi(src_kind[t], src_index[t]) over n_terms sources:
+ * get_total_concentration(_gce), get_block_concentration, get_solvent_concentration */
+int fts_solver_sum_concentrations(fts_solver* s, int n_terms, const int* src_kind, const int* src_index,
+                                  const double* scale, double* out, int on_device);
+/* get_chain_propagator: q[prop][n] (half = 1: discrete half-bond slice), FP64 out */
+int fts_solver_get_propagator(fts_solver* s, int prop, int n, int half, double* out, int on_device);
+
+/* ---- Anderson mixing (AndersonMixing.cpp:3-47, CpuAndersonMixing.cpp:76-161,
+ *      CudaAndersonMixing.cu:9-312); history kept on the device, FP64 Gram solve */
+int fts_am_create(fts_context* ctx, int n_var, int max_hist, double start_error, double mix_min, double mix_init,
+                  fts_am** out);
+int fts_am_destroy(fts_am* am);
+int fts_am_reset(fts_am* am);
+int fts_am_calculate_new_fields(fts_am* am, double* w_new, const double* w_current, const double* w_deriv,
+                                double old_error_level, double error_level, int on_device);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,311 @@
+// extern "C" entry points of include/fts_hip.h: argument checks, exception to
+// status-code conversion, thread-local error text.
+#include <string>
+
+#include "fts_internal.hpp"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+template <typename F>
+int guarded(F&& f) {
+    try {
+        f();
+        g_last_error.clear();
+        return FTS_OK;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+    } catch (...) {
+        g_last_error = "unknown error";
+    }
+    return FTS_ERROR;
+}
+
+void need(const void* p, const char* what) {
+    if (!p) throw fts::Error(std::string(what) + " is null");
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* fts_last_error(void) { return g_last_error.c_str(); }
+
+int fts_abi_version(void) { return 1; }
+
+int fts_context_create(int device, fts_context** out) {
+    return guarded([&] {
+        need(out, "out");
+        int n = 0;
+        FTS_HIP_CHECK(hipGetDeviceCount(&n));
+        FTS_REQUIRE(device >= 0 && device < n, "device index out of range");
+        FTS_HIP_CHECK(hipSetDevice(device));
+        auto* c = new fts_context;
+        c->device = device;
+        if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+            delete c;
+            throw fts::Error("hipStreamCreate failed");
+        }
+        *out = c;
+    });
+}
+
+int fts_context_destroy(fts_context* ctx) {
+    return guarded([&] {
+        if (!ctx) return;
+        (void)hipStreamSynchronize(ctx->stream);
+        (void)hipStreamDestroy(ctx->stream);
+        delete ctx;
+    });
+}
+
+int fts_context_synchronize(fts_context* ctx) {
+    return guarded([&] {
+        need(ctx, "ctx");
+        FTS_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int fts_context_stream(fts_context* ctx, void** stream) {
+    return guarded([&] {
+        need(ctx, "ctx");
+        need(stream, "stream");
+        *stream = (void*)ctx->stream;
+    });
+}
+
+int fts_context_info(fts_context* ctx, char* buf, int buflen) {
+    return guarded([&] {
+        need(ctx, "ctx");
+        need(buf, "buf");
+        FTS_REQUIRE(buflen > 0, "buflen must be positive");
+        hipDeviceProp_t p;
+        FTS_HIP_CHECK(hipGetDeviceProperties(&p, ctx->device));
+        snprintf(buf, buflen, "%s, %s, %d CUs, %.1f GiB", p.name, p.gcnArchName, p.multiProcessorCount,
+                 p.totalGlobalMem / 1073741824.0);
+    });
+}
+
+int fts_grid_create(fts_context* ctx, int dim, const int* nx, const double* lx, const double* mask, fts_grid** out) {
+    return guarded([&] {
+        need(ctx, "ctx");
+        need(nx, "nx");
+        need(lx, "lx");
+        need(out, "out");
+        FTS_HIP_CHECK(hipSetDevice(ctx->device));
+        auto* g = new fts_grid;
+        try {
+            g->g.reset(new fts::Grid(ctx, dim, nx, lx, mask));
+        } catch (...) {
+            delete g;
+            throw;
+        }
+        *out = g;
+    });
+}
+
+int fts_grid_destroy(fts_grid* g) {
+    return guarded([&] { delete g; });
+}
+
+int fts_grid_set_lx(fts_grid* g, const double* lx) {
+    return guarded([&] {
+        need(g, "grid");
+        need(lx, "lx");
+        g->g->set_lx(lx);
+    });
+}
+
+int fts_grid_volume(fts_grid* g, double* out) {
+    return guarded([&] {
+        need(g, "grid");
+        need(out, "out");
+        *out = g->g->volume;
+    });
+}
+
+int fts_grid_integral(fts_grid* g, const double* a, int on_device, double* out) {
+    return guarded([&] {
+        need(g, "grid");
+        need(a, "a");
+        need(out, "out");
+        *out = g->g->integral(a, on_device != 0);
+    });
+}
+
+int fts_grid_inner_product(fts_grid* g, const double* a, const double* b, int on_device, double* out) {
+    return guarded([&] {
+        need(g, "grid");
+        need(a, "a");
+        need(b, "b");
+        need(out, "out");
+        *out = g->g->inner_product(a, b, on_device != 0);
+    });
+}
+
+int fts_grid_fft_forward(fts_grid* g, int precision, const void* d_in, void* d_out, int batch) {
+    return guarded([&] {
+        need(g, "grid");
+        need(d_in, "in");
+        need(d_out, "out");
+        FTS_REQUIRE(batch >= 1, "batch must be positive");
+        if (precision == FTS_FP64)
+            g->g->fft_forward<double>(static_cast<const double*>(d_in), static_cast<double2*>(d_out), batch);
+        else if (precision == FTS_FP32)
+            g->g->fft_forward<float>(static_cast<const float*>(d_in), static_cast<float2*>(d_out), batch);
+        else
+            throw fts::Error("precision must be FTS_FP64 or FTS_FP32");
+        FTS_HIP_CHECK(hipStreamSynchronize(g->g->stream()));
+    });
+}
+
+int fts_grid_fft_backward(fts_grid* g, int precision, const void* d_in, void* d_out, int batch) {
+    return guarded([&] {
+        need(g, "grid");
+        need(d_in, "in");
+        need(d_out, "out");
+        FTS_REQUIRE(batch >= 1, "batch must be positive");
+        if (precision == FTS_FP64)
+            g->g->fft_backward<double>(static_cast<const double2*>(d_in), static_cast<double*>(d_out), batch);
+        else if (precision == FTS_FP32)
+            g->g->fft_backward<float>(static_cast<const float2*>(d_in), static_cast<float*>(d_out), batch);
+        else
+            throw fts::Error("precision must be FTS_FP64 or FTS_FP32");
+    });
+}
+
+int fts_solver_create(fts_grid* g, const fts_plan* plan, int precision, fts_solver** out) {
+    return guarded([&] {
+        need(g, "grid");
+        need(plan, "plan");
+        need(out, "out");
+        auto* s = new fts_solver;
+        try {
+            s->s = fts::make_solver(g->g.get(), plan, precision);
+        } catch (...) {
+            delete s;
+            throw;
+        }
+        *out = s;
+    });
+}
+
+int fts_solver_destroy(fts_solver* s) {
+    return guarded([&] { delete s; });
+}
+
+int fts_solver_update_laplacian(fts_solver* s) {
+    return guarded([&] {
+        need(s, "solver");
+        s->s->update_laplacian();
+    });
+}
+
+int fts_solver_compute_propagators(fts_solver* s, const double* const* w, const double* const* q_init, int on_device) {
+    return guarded([&] {
+        need(s, "solver");
+        need(w, "w");
+        s->s->compute_propagators(w, q_init, on_device != 0);
+    });
+}
+
+int fts_solver_compute_concentrations(fts_solver* s) {
+    return guarded([&] {
+        need(s, "solver");
+        s->s->compute_concentrations();
+    });
+}
+
+int fts_solver_compute_stress(fts_solver* s) {
+    return guarded([&] {
+        need(s, "solver");
+        s->s->compute_stress();
+    });
+}
+
+int fts_solver_get_partition(fts_solver* s, int polymer, double* out) {
+    return guarded([&] {
+        need(s, "solver");
+        need(out, "out");
+        *out = s->s->partition(polymer);
+    });
+}
+
+int fts_solver_get_solvent_partition(fts_solver* s, int solvent, double* out) {
+    return guarded([&] {
+        need(s, "solver");
+        need(out, "out");
+        *out = s->s->solvent_partition(solvent);
+    });
+}
+
+int fts_solver_get_dq_dl(fts_solver* s, int polymer, double* out) {
+    return guarded([&] {
+        need(s, "solver");
+        need(out, "out");
+        s->s->dq_dl(polymer, out);
+    });
+}
+
+int fts_solver_sum_concentrations(fts_solver* s, int n_terms, const int* src_kind, const int* src_index,
+                                  const double* scale, double* out, int on_device) {
+    return guarded([&] {
+        need(s, "solver");
+        need(out, "out");
+        FTS_REQUIRE(n_terms >= 0, "n_terms must be non-negative");
+        if (n_terms) {
+            need(src_kind, "src_kind");
+            need(src_index, "src_index");
+        }
+        s->s->sum_concentrations(n_terms, src_kind, src_index, scale, out, on_device != 0);
+    });
+}
+
+int fts_solver_get_propagator(fts_solver* s, int prop, int n, int half, double* out, int on_device) {
+    return guarded([&] {
+        need(s, "solver");
+        need(out, "out");
+        s->s->propagator(prop, n, half, out, on_device != 0);
+    });
+}
+
+int fts_am_create(fts_context* ctx, int n_var, int max_hist, double start_error, double mix_min, double mix_init,
+                  fts_am** out) {
+    return guarded([&] {
+        need(ctx, "ctx");
+        need(out, "out");
+        auto* a = new fts_am;
+        try {
+            a->a.reset(new fts::AndersonMixingDevice(ctx, n_var, max_hist, start_error, mix_min, mix_init));
+        } catch (...) {
+            delete a;
+            throw;
+        }
+        *out = a;
+    });
+}
+
+int fts_am_destroy(fts_am* am) {
+    return guarded([&] { delete am; });
+}
+
+int fts_am_reset(fts_am* am) {
+    return guarded([&] {
+        need(am, "am");
+        am->a->reset();
+    });
+}
+
+int fts_am_calculate_new_fields(fts_am* am, double* w_new, const double* w_current, const double* w_deriv,
+                                double old_error_level, double error_level, int on_device) {
+    return guarded([&] {
+        need(am, "am");
+        need(w_new, "w_new");
+        need(w_current, "w_current");
+        need(w_deriv, "w_deriv");
+        am->a->calculate_new_fields(w_new, w_current, w_deriv, old_error_level, error_level, on_device != 0);
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,376 @@
+// Grid (ComputationBox + FFT plan), separable Boltzmann tables and the
+// deterministic reductions (replacing cub::DeviceReduce::Sum, CudaComputationBox.cu:56-118).
+#include <cmath>
+
+#include "fts_internal.hpp"
+#include "fts_reduce.hpp"
+
+namespace fts {
+
+namespace {
+
+// ~8 elements per thread, whole waves, at most 1024 threads
+int choose_threads(int work) {
+    int bd = ceil_div(ceil_div(work, 8), 64) * 64;
+    return std::max(64, std::min(1024, bd));
+}
+
+template <typename T>
+void upload_twiddles(int L, DeviceBuffer& buf) {
+    using T2 = typename vec2<T>::type;
+    std::vector<T2> h(L);
+    for (int m = 0; m < L; m++) {
+        const double a = 2.0 * M_PI * (double)m / (double)L;
+        h[m] = T2{(T)std::cos(a), (T)-std::sin(a)};
+    }
+    buf.allocate(sizeof(T2) * L);
+    FTS_HIP_CHECK(hipMemcpy(buf.get(), h.data(), sizeof(T2) * L, hipMemcpyHostToDevice));
+}
+
+constexpr size_t LDS_LIMIT = 160 * 1024;
+
+template <typename K>
+void allow_lds(K kernel) {
+    FTS_HIP_CHECK(hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)LDS_LIMIT));
+}
+
+template <typename T, int MAXR, bool BIG>
+void allow_lds_slab() {
+    allow_lds(&slab_kernel<T, SLAB_FWD, MAXR, BIG>);
+    allow_lds(&slab_kernel<T, SLAB_INV1, MAXR, BIG>);
+    allow_lds(&slab_kernel<T, SLAB_INV2, MAXR, BIG>);
+}
+
+template <typename T>
+void set_lds_attributes() {
+    static bool done = false;
+    if (done) return;
+    allow_lds_slab<T, 8, false>();
+    allow_lds_slab<T, 16, false>();
+    allow_lds_slab<T, 8, true>();
+    allow_lds_slab<T, 16, true>();
+    allow_lds(&xpass_kernel<T, false>);
+    allow_lds(&xpass_kernel<T, true>);
+    done = true;
+}
+
+template <typename T, int MODE, int MAXR, bool BIG>
+void launch_slab_k(const Grid& g, const SlabBatch<T>& b, dim3 grid, dim3 block, size_t lds) {
+    hipLaunchKernelGGL((slab_kernel<T, MODE, MAXR, BIG>), grid, block, lds, g.stream(), g.sg, b, g.tw<T>(1),
+                       g.tw<T>(2));
+}
+
+template <typename T, int MODE>
+void launch_slab_mode(const Grid& g, const SlabBatch<T>& b, dim3 grid, dim3 block, size_t lds) {
+    if (g.slab_maxr == 8) {
+        if (g.big) launch_slab_k<T, MODE, 8, true>(g, b, grid, block, lds);
+        else launch_slab_k<T, MODE, 8, false>(g, b, grid, block, lds);
+    } else {
+        if (g.big) launch_slab_k<T, MODE, 16, true>(g, b, grid, block, lds);
+        else launch_slab_k<T, MODE, 16, false>(g, b, grid, block, lds);
+    }
+}
+
+}  // namespace
+
+Grid::Grid(fts_context* c, int d, const int* n, const double* l, const double* mask) : ctx(c), dim(d) {
+    FTS_REQUIRE(dim >= 1 && dim <= 3, "dim must be 1, 2 or 3");
+    M = 1;
+    for (int i = 0; i < dim; i++) {
+        FTS_REQUIRE(n[i] > 0, "nx must be positive");
+        FTS_REQUIRE(l[i] > 0, "lx must be positive");
+        nx[i] = n[i];
+        lx[i] = l[i];
+        dx[i] = l[i] / n[i];
+        M *= n[i];
+    }
+    for (int i = dim; i < 3; i++) {
+        nx[i] = 1;
+        lx[i] = 1.0;
+        dx[i] = 1.0;
+    }
+    for (int a = 0; a < 3; a++) {
+        const int src = a - (3 - dim);
+        tnx[a] = src >= 0 ? nx[src] : 1;
+        tdx[a] = src >= 0 ? dx[src] : 1.0;
+    }
+    nzc = tnx[2] / 2 + 1;
+    Mc = tnx[0] * tnx[1] * nzc;
+    // ComputationBox.cpp:47-98: dv = prod dx (times mask), volume = prod lx
+    dV = 1.0;
+    for (int i = 0; i < dim; i++) dV *= dx[i];
+    volume = 1.0;
+    for (int i = 0; i < dim; i++) volume *= lx[i];
+    if (mask) {
+        mask_h.resize(M);
+        std::vector<float> m32(M);
+        for (int i = 0; i < M; i++) {
+            if (std::abs(mask[i]) < 1e-7) mask_h[i] = 0.0;
+            else if (std::abs(mask[i] - 1.0) < 1e-7) mask_h[i] = 1.0;
+            else throw Error("mask[" + std::to_string(i) + "] must be 0.0 or 1.0");
+            m32[i] = (float)mask_h[i];
+        }
+        mask64.allocate(sizeof(double) * M);
+        mask32.allocate(sizeof(float) * M);
+        FTS_HIP_CHECK(hipMemcpy(mask64.get(), mask_h.data(), sizeof(double) * M, hipMemcpyHostToDevice));
+        FTS_HIP_CHECK(hipMemcpy(mask32.get(), m32.data(), sizeof(float) * M, hipMemcpyHostToDevice));
+    }
+
+    ax = plan_axis(tnx[0]);
+    ay = plan_axis(tnx[1]);
+    az = plan_axis(tnx[2]);
+    for (int a = 0; a < 3; a++) {
+        upload_twiddles<double>(tnx[a], tw64[a]);
+        upload_twiddles<float>(tnx[a], tw32[a]);
+    }
+
+    sg.ny = tnx[1];
+    sg.nz = tnx[2];
+    sg.nzc = nzc;
+    sg.P = nzc;
+    sg.rows = 2 * ((tnx[1] + 1) / 2);
+    sg.ay = ay;
+    sg.az = az;
+    sg.inv_m = 1.0 / (double)M;
+    slab_threads = choose_threads(sg.ny * sg.nz);
+    const int per_thread = ceil_div(sg.ny * sg.nz, slab_threads);
+    FTS_REQUIRE(per_thread <= 16, "y-z slab too large (more than 16384 points)");
+    // the 8-value variant is compiled for at most 512 threads (more registers per thread)
+    slab_maxr = (per_thread <= 8 && slab_threads <= 512) ? 8 : 16;
+    sg.ay.group = group_lines(sg.ay, slab_threads);
+    sg.az.group = group_lines(sg.az, slab_threads);
+    FTS_REQUIRE(slab_lds<double>() <= LDS_LIMIT, "y-z slab does not fit in LDS");
+    big = plan_is_big(ax) || plan_is_big(ay) || plan_is_big(az);
+
+    xg.nx = tnx[0];
+    xg.ny = tnx[1];
+    xg.nzc = nzc;
+    int C = nzc, tiles = 1;
+    while (tnx[1] * tiles < 128 && C > 8) {
+        C = ceil_div(C, 2);
+        tiles = ceil_div(nzc, C);
+    }
+    while ((size_t)(tnx[0] * C + tnx[0]) * sizeof(double2) > 64 * 1024 && C > 1) {
+        C = ceil_div(C, 2);
+        tiles = ceil_div(nzc, C);
+    }
+    xg.C = C;
+    xg.n_tiles = tiles;
+    xg.P = C;
+    xg.ax = ax;
+    x_threads = choose_threads(tnx[0] * C);
+    xg.ax.group = group_lines(xg.ax, x_threads);
+
+    set_lds_attributes<double>();
+    set_lds_attributes<float>();
+
+    partials.allocate(sizeof(double) * MAX_BATCH * RED_BLOCKS * 3);
+    scalars.allocate(sizeof(double) * MAX_BATCH);
+    scratch64.allocate(sizeof(double) * 2 * (size_t)M);
+}
+
+// ComputationBox.cpp:229-262
+void Grid::set_lx(const double* l) {
+    for (int i = 0; i < dim; i++) FTS_REQUIRE(l[i] > 0, "new lx must be positive");
+    for (int i = 0; i < dim; i++) {
+        lx[i] = l[i];
+        dx[i] = l[i] / nx[i];
+    }
+    for (int a = 0; a < 3; a++) {
+        const int src = a - (3 - dim);
+        tdx[a] = src >= 0 ? dx[src] : 1.0;
+    }
+    dV = 1.0;
+    for (int i = 0; i < dim; i++) dV *= dx[i];
+    volume = 0.0;
+    for (int i = 0; i < M; i++) volume += mask_h.empty() ? dV : dV * mask_h[i];
+}
+
+template <>
+const double* Grid::mask<double>() const {
+    return mask64.as<double>();
+}
+template <>
+const float* Grid::mask<float>() const {
+    return mask32.as<float>();
+}
+template <>
+const double2* Grid::tw<double>(int a) const {
+    return tw64[a].as<double2>();
+}
+template <>
+const float2* Grid::tw<float>(int a) const {
+    return tw32[a].as<float2>();
+}
+
+template <typename T>
+void Grid::launch_slab(int mode, const SlabBatch<T>& b) const {
+    if (b.n == 0) return;
+    const dim3 grid(tnx[0], b.n), block(slab_threads);
+    const size_t lds = slab_lds<T>();
+    if (mode == SLAB_FWD) launch_slab_mode<T, SLAB_FWD>(*this, b, grid, block, lds);
+    else if (mode == SLAB_INV1) launch_slab_mode<T, SLAB_INV1>(*this, b, grid, block, lds);
+    else launch_slab_mode<T, SLAB_INV2>(*this, b, grid, block, lds);
+    FTS_HIP_CHECK(hipGetLastError());
+}
+
+template <typename T>
+void Grid::launch_xpass(const XBatch<T>& b, const T* bx, const T* by, const T* bz) const {
+    if (b.n == 0) return;
+    const dim3 grid(tnx[1] * xg.n_tiles, b.n), block(x_threads);
+    if (big)
+        hipLaunchKernelGGL((xpass_kernel<T, true>), grid, block, x_lds<T>(), stream(), xg, b, tw<T>(0), bx, by, bz);
+    else
+        hipLaunchKernelGGL((xpass_kernel<T, false>), grid, block, x_lds<T>(), stream(), xg, b, tw<T>(0), bx, by, bz);
+    FTS_HIP_CHECK(hipGetLastError());
+}
+
+template <typename T>
+void Grid::launch_dots(const DotBatch<T>& b, double* out) const {
+    if (b.n == 0) return;
+    hipLaunchKernelGGL((dot_stage1<T>), dim3(RED_BLOCKS, b.n), dim3(RED_THREADS), 0, stream(), b, M, dV,
+                       mask_h.empty() ? nullptr : mask64.as<double>(), partials.as<double>());
+    FTS_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL((dot_stage2<T>), dim3(b.n), dim3(RED_BLOCKS), 0, stream(), b, partials.as<double>(), out);
+    FTS_HIP_CHECK(hipGetLastError());
+}
+
+double Grid::inner_product(const double* a, const double* b, bool on_device) {
+    const double* da = a;
+    const double* db = b;
+    if (!on_device) {
+        FTS_HIP_CHECK(hipMemcpyAsync(scratch64.as<double>(), a, sizeof(double) * M, hipMemcpyHostToDevice, stream()));
+        FTS_HIP_CHECK(hipMemcpyAsync(scratch64.as<double>() + M, b, sizeof(double) * M, hipMemcpyHostToDevice,
+                                     stream()));
+        da = scratch64.as<double>();
+        db = scratch64.as<double>() + M;
+    }
+    DotBatch<double> batch{};
+    batch.n = 1;
+    batch.it[0] = DotItem<double>{da, db, nullptr, 0.0, 1.0};
+    launch_dots(batch, scalars.as<double>());
+    double r;
+    FTS_HIP_CHECK(hipMemcpyAsync(&r, scalars.as<double>(), sizeof(double), hipMemcpyDeviceToHost, stream()));
+    FTS_HIP_CHECK(hipStreamSynchronize(stream()));
+    return r;
+}
+
+double Grid::integral(const double* a, bool on_device) {
+    const double* da = a;
+    if (!on_device) {
+        FTS_HIP_CHECK(hipMemcpyAsync(scratch64.as<double>(), a, sizeof(double) * M, hipMemcpyHostToDevice, stream()));
+        da = scratch64.as<double>();
+    }
+    DotBatch<double> batch{};
+    batch.n = 1;
+    batch.it[0] = DotItem<double>{da, nullptr, nullptr, 0.0, 1.0};
+    launch_dots(batch, scalars.as<double>());
+    double r;
+    FTS_HIP_CHECK(hipMemcpyAsync(&r, scalars.as<double>(), sizeof(double), hipMemcpyDeviceToHost, stream()));
+    FTS_HIP_CHECK(hipStreamSynchronize(stream()));
+    return r;
+}
+
+template <typename T>
+void Grid::fft_forward(const T* in, typename vec2<T>::type* out, int batch) {
+    for (int b0 = 0; b0 < batch; b0 += MAX_BATCH) {
+        const int nb = std::min(MAX_BATCH, batch - b0);
+        SlabBatch<T> sb{};
+        XBatch<T> xb{};
+        sb.n = xb.n = nb;
+        for (int b = 0; b < nb; b++) {
+            SlabItem<T>& it = sb.it[b];
+            it.qin = in + (size_t)(b0 + b) * M;
+            it.c0 = out + (size_t)(b0 + b) * Mc;
+            it.n_out = 1;
+            xb.it[b] = XItem<T>{out + (size_t)(b0 + b) * Mc, -1};
+        }
+        launch_slab<T>(SLAB_FWD, sb);
+        launch_xpass<T>(xb, nullptr, nullptr, nullptr);
+    }
+}
+
+template <typename T>
+void Grid::fft_backward(const typename vec2<T>::type* in, T* out, int batch) {
+    // the pencil pass transforms in place, so the input is first copied to out's
+    // companion workspace: use a temporary spectrum buffer
+    using T2 = typename vec2<T>::type;
+    DeviceBuffer tmp(sizeof(T2) * (size_t)Mc * std::min(batch, MAX_BATCH));
+    for (int b0 = 0; b0 < batch; b0 += MAX_BATCH) {
+        const int nb = std::min(MAX_BATCH, batch - b0);
+        FTS_HIP_CHECK(hipMemcpyAsync(tmp.get(), in + (size_t)b0 * Mc, sizeof(T2) * (size_t)Mc * nb,
+                                     hipMemcpyDeviceToDevice, stream()));
+        SlabBatch<T> sb{};
+        XBatch<T> xb{};
+        sb.n = xb.n = nb;
+        for (int b = 0; b < nb; b++) {
+            xb.it[b] = XItem<T>{tmp.as<T2>() + (size_t)b * Mc, -2};
+            SlabItem<T>& it = sb.it[b];
+            it.c0 = tmp.as<T2>() + (size_t)b * Mc;
+            it.qout = out + (size_t)(b0 + b) * M;
+        }
+        launch_xpass<T>(xb, nullptr, nullptr, nullptr);
+        launch_slab<T>(SLAB_INV1, sb);
+    }
+    FTS_HIP_CHECK(hipStreamSynchronize(stream()));
+}
+
+template void Grid::launch_slab<double>(int, const SlabBatch<double>&) const;
+template void Grid::launch_slab<float>(int, const SlabBatch<float>&) const;
+template void Grid::launch_xpass<double>(const XBatch<double>&, const double*, const double*, const double*) const;
+template void Grid::launch_xpass<float>(const XBatch<float>&, const float*, const float*, const float*) const;
+template void Grid::launch_dots<double>(const DotBatch<double>&, double*) const;
+template void Grid::launch_dots<float>(const DotBatch<float>&, double*) const;
+template void Grid::fft_forward<double>(const double*, double2*, int);
+template void Grid::fft_forward<float>(const float*, float2*, int);
+template void Grid::fft_backward<double>(const double2*, double*, int);
+template void Grid::fft_backward<float>(const float2*, float*, int);
+
+// ------------------------------------------------------------------ bond tables
+void BondTables::build(const Grid& g, const std::vector<double>& bsq, double ds) {
+    const double PI = 3.14159265358979323846;
+    const int nt = 2 * (int)bsq.size();
+    std::vector<double> hx((size_t)nt * g.tnx[0]), hy((size_t)nt * g.tnx[1]), hz((size_t)nt * g.nzc);
+    for (size_t m = 0; m < bsq.size(); m++) {
+        for (int h = 0; h < 2; h++) {
+            const int t = 2 * (int)m + h;
+            const double b2 = h == 0 ? bsq[m] : bsq[m] / 2;
+            double xf[3];
+            for (int d = 0; d < 3; d++) xf[d] = -b2 * std::pow(2 * PI / (g.tnx[d] * g.tdx[d]), 2) * ds / 6.0;
+            for (int i = 0; i < g.tnx[0]; i++) {
+                const int it = i > g.tnx[0] / 2 ? g.tnx[0] - i : i;
+                hx[(size_t)t * g.tnx[0] + i] = std::exp(it * it * xf[0]);
+            }
+            for (int j = 0; j < g.tnx[1]; j++) {
+                const int jt = j > g.tnx[1] / 2 ? g.tnx[1] - j : j;
+                hy[(size_t)t * g.tnx[1] + j] = std::exp(jt * jt * xf[1]);
+            }
+            for (int k = 0; k < g.nzc; k++) hz[(size_t)t * g.nzc + k] = std::exp(k * k * xf[2]);
+        }
+    }
+    auto up = [](DeviceBuffer& b64, DeviceBuffer& b32, const std::vector<double>& h) {
+        std::vector<float> f(h.begin(), h.end());
+        if (b64.bytes() != sizeof(double) * h.size()) b64.allocate(sizeof(double) * h.size());
+        if (b32.bytes() != sizeof(float) * h.size()) b32.allocate(sizeof(float) * h.size());
+        FTS_HIP_CHECK(hipMemcpy(b64.get(), h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice));
+        FTS_HIP_CHECK(hipMemcpy(b32.get(), f.data(), sizeof(float) * h.size(), hipMemcpyHostToDevice));
+    };
+    up(bx64, bx32, hx);
+    up(by64, by32, hy);
+    up(bz64, bz32, hz);
+}
+
+template <>
+const double* BondTables::x<double>() const { return bx64.as<double>(); }
+template <>
+const double* BondTables::y<double>() const { return by64.as<double>(); }
+template <>
+const double* BondTables::z<double>() const { return bz64.as<double>(); }
+template <>
+const float* BondTables::x<float>() const { return bx32.as<float>(); }
+template <>
+const float* BondTables::y<float>() const { return by32.as<float>(); }
+template <>
+const float* BondTables::z<float>() const { return bz32.as<float>(); }
+
+}  // namespace fts

@@ -1,0 +1,156 @@
+// Internal classes behind the C ABI of include/fts_hip.h.
+#pragma once
+
+#include <array>
+#include <memory>
+#include <vector>
+
+#include "../../../include/fts_hip.h"
+#include "fts_common.hpp"
+#include "fts_fft.hpp"
+
+struct fts_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+};
+
+namespace fts {
+
+constexpr int RED_BLOCKS = 256;   // first-stage workgroups of the deterministic reductions
+constexpr int RED_THREADS = 256;
+
+// Batched weighted inner product: out[item] = scale * sum_i dv_i * a_i * b_i * exp(wcoef * w_i)
+// (a or b null: 1; w null: 1), dv_i = dV * mask_i.
+template <typename T>
+struct DotItem {
+    const T* a;
+    const T* b;
+    const T* w;
+    T wcoef;
+    double scale;
+};
+
+template <typename T>
+struct DotBatch {
+    int n;
+    DotItem<T> it[MAX_BATCH];
+};
+
+class Grid {
+public:
+    Grid(fts_context* ctx, int dim, const int* nx, const double* lx, const double* mask);
+    void set_lx(const double* lx);
+
+    fts_context* ctx;
+    int dim;
+    int nx[3];
+    double lx[3];
+    double dx[3];
+    int tnx[3];     // 3D-mapped sizes (Pseudo.cpp:31-45)
+    double tdx[3];
+    int M, nzc, Mc;
+    double volume;
+    double dV;
+    std::vector<double> mask_h;
+    DeviceBuffer mask64, mask32;
+    AxisPlan ax, ay, az;
+    DeviceBuffer tw64[3], tw32[3];
+    SlabGeom sg;
+    XGeom xg;
+    int slab_threads, x_threads;
+    int slab_maxr;   // real values per thread in the slab kernel (8 or 16)
+    bool big;        // some axis needs a radix outside {2, 3, 4, 5, 8}
+    DeviceBuffer partials;   // reduction scratch: MAX_BATCH * RED_BLOCKS doubles
+    DeviceBuffer scalars;    // reduction results: MAX_BATCH doubles
+    DeviceBuffer scratch64;  // general FP64 scratch of 2 fields
+
+    hipStream_t stream() const { return ctx->stream; }
+    template <typename T>
+    const T* mask() const;
+    template <typename T>
+    const typename vec2<T>::type* tw(int axis) const;
+    template <typename T>
+    size_t slab_lds() const {
+        return (size_t)(sg.rows * sg.P + sg.ny + sg.nz) * sizeof(typename vec2<T>::type);
+    }
+    template <typename T>
+    size_t x_lds() const {
+        return (size_t)(xg.nx * xg.P + xg.nx) * sizeof(typename vec2<T>::type);
+    }
+    template <typename T>
+    void launch_slab(int mode, const SlabBatch<T>& b) const;
+    template <typename T>
+    void launch_xpass(const XBatch<T>& b, const T* bx, const T* by, const T* bz) const;
+    // first stage into partials, second stage into out (device, b.n doubles)
+    template <typename T>
+    void launch_dots(const DotBatch<T>& b, double* out) const;
+    double inner_product(const double* a, const double* b, bool on_device);
+    double integral(const double* a, bool on_device);
+    template <typename T>
+    void fft_forward(const T* in, typename vec2<T>::type* out, int batch);
+    template <typename T>
+    void fft_backward(const typename vec2<T>::type* in, T* out, int batch);
+};
+
+// Boltzmann factors of one (monomer, bond kind), separable along the three axes.
+// Row t of bx/by/bz holds exp(-b^2 ds/6 (2 pi k_d / L_d)^2) along axis d
+// (Pseudo.cpp:17-82 split into its three factors); t = 2m: full bond of monomer m,
+// t = 2m + 1: half bond (b^2/2).
+struct BondTables {
+    DeviceBuffer bx64, by64, bz64, bx32, by32, bz32;
+    void build(const Grid& g, const std::vector<double>& bsq, double ds);
+    template <typename T>
+    const T* x() const;
+    template <typename T>
+    const T* y() const;
+    template <typename T>
+    const T* z() const;
+};
+
+class SolverBase {
+public:
+    virtual ~SolverBase() = default;
+    virtual void update_laplacian() = 0;
+    virtual void compute_propagators(const double* const* w, const double* const* q_init, bool on_device) = 0;
+    virtual void compute_concentrations() = 0;
+    virtual void compute_stress() = 0;
+    virtual double partition(int p) = 0;
+    virtual double solvent_partition(int s) = 0;
+    virtual void dq_dl(int p, double* out) = 0;
+    virtual void sum_concentrations(int n, const int* kind, const int* idx, const double* scale, double* out,
+                                    bool on_device) = 0;
+    virtual void propagator(int prop, int n, int half, double* out, bool on_device) = 0;
+};
+
+std::unique_ptr<SolverBase> make_solver(Grid* g, const fts_plan* plan, int precision);
+
+class AndersonMixingDevice {
+public:
+    AndersonMixingDevice(fts_context* ctx, int n_var, int max_hist, double start_error, double mix_min,
+                         double mix_init);
+    void reset();
+    void calculate_new_fields(double* w_new, const double* w, const double* d, double old_err, double err,
+                              bool on_device);
+
+private:
+    fts_context* ctx_;
+    int n_var_, max_hist_, n_anderson_;
+    double start_error_, mix_min_, mix_, mix_init_;
+    // device rings of max_hist+1 rows; host ring of dot products (CircularBuffer.cpp semantics)
+    DeviceBuffer w_hist_, d_hist_, in_w_, in_d_, out_, partials_, dots_d_;
+    int ring_start_ = 0, ring_n_ = 0;
+    std::vector<double> dots_;   // (max_hist+1)^2 ring rows
+    int dots_start_ = 0, dots_n_ = 0;
+};
+
+}  // namespace fts
+
+struct fts_grid {
+    std::unique_ptr<fts::Grid> g;
+};
+struct fts_solver {
+    std::unique_ptr<fts::SolverBase> s;
+};
+struct fts_am {
+    std::unique_ptr<fts::AndersonMixingDevice> a;
+};

@@ -1,0 +1,158 @@
+// The "hip" platform behind the reference's factory interface
+// (AbstractFactory.h:21-62, PlatformSelector.h:12-18, ComputationBox.h:22-76,
+// PropagatorComputation.h:21-75, AndersonMixing.h:13-29). Every numerical call
+// goes through the C ABI of libfts_hip.so (include/fts_hip.h).
+#pragma once
+
+#include <map>
+#include <string>
+#include <vector>
+
+#include "fts_hip.h"
+#include "fts_model.h"
+
+namespace lfts {
+
+// One device context per process (device from LFTS_HIP_DEVICE, default 0).
+fts_context* hip_context();
+void check_fts(int rc);
+
+class ComputationBox {
+public:
+    ComputationBox(std::vector<int> nx, std::vector<double> lx, std::vector<std::string> bc, const double* mask);
+    ~ComputationBox();
+    ComputationBox(const ComputationBox&) = delete;
+    ComputationBox& operator=(const ComputationBox&) = delete;
+
+    int get_dim() const { return (int)nx_.size(); }
+    std::vector<int> get_nx() const { return nx_; }
+    int get_nx(int i) const;
+    std::vector<double> get_lx() const { return lx_; }
+    double get_lx(int i) const;
+    std::vector<double> get_dx() const { return dx_; }
+    double get_dx(int i) const;
+    double get_dv(int i) const;
+    int get_n_grid() const { return n_grid_; }
+    double get_volume() const { return volume_; }
+    const double* get_mask() const { return mask_.empty() ? nullptr : mask_.data(); }
+    bool is_periodic() const;
+    void set_lx(std::vector<double> new_lx);
+    double integral(const double* g) const;
+    double inner_product(const double* g, const double* h) const;
+    double multi_inner_product(int n_comp, const double* g, const double* h) const;
+    void zero_mean(double* g) const;
+    fts_grid* handle() const { return grid_; }
+
+private:
+    std::vector<int> nx_;
+    std::vector<double> lx_, dx_;
+    std::vector<std::string> bc_;
+    std::vector<double> mask_;
+    int n_grid_;
+    double volume_;
+    fts_grid* grid_ = nullptr;
+};
+
+class PropagatorComputation {
+public:
+    PropagatorComputation(ComputationBox* cb, Molecules* molecules, PropagatorAnalyzer* analyzer, int precision,
+                          int n_streams);
+    ~PropagatorComputation();
+    PropagatorComputation(const PropagatorComputation&) = delete;
+    PropagatorComputation& operator=(const PropagatorComputation&) = delete;
+
+    int get_n_grid() const { return cb_->get_n_grid(); }
+    int get_n_blocks(int polymer) { return molecules_->get_polymer(polymer).get_n_blocks(); }
+    void update_laplacian_operator();
+    void compute_propagators(const std::map<std::string, const double*>& w_input,
+                             const std::map<std::string, const double*>& q_init, bool on_device = false);
+    void compute_concentrations();
+    void compute_statistics(const std::map<std::string, const double*>& w_input,
+                            const std::map<std::string, const double*>& q_init) {
+        compute_propagators(w_input, q_init);
+        compute_concentrations();
+    }
+    void compute_stress();
+    double get_total_partition(int polymer);
+    void get_chain_propagator(double* q_out, int polymer, int v, int u, int n);
+    void get_total_concentration(const std::string& monomer_type, double* phi, bool on_device = false);
+    void get_total_concentration(int polymer, const std::string& monomer_type, double* phi, bool on_device = false);
+    void get_total_concentration_gce(double fugacity, int polymer, const std::string& monomer_type, double* phi);
+    void get_block_concentration(int polymer, double* phi);
+    double get_solvent_partition(int s);
+    void get_solvent_concentration(int s, double* phi);
+    std::vector<double> get_stress();
+    std::vector<double> get_stress_gce(const std::vector<double>& fugacities);
+    bool check_total_partition();
+
+    const std::vector<std::string>& monomer_types() const { return monomers_; }
+    const std::vector<std::vector<Scheduler::Job>>& schedule() const { return schedule_; }
+    int precision() const { return precision_; }
+    int n_streams() const { return n_streams_; }
+
+private:
+    std::vector<double> dq_dl(int p);
+    void sum_blocks(int polymer, const std::string& monomer, bool with_solvents, double scale, double* out,
+                    bool on_device);
+
+    ComputationBox* cb_;
+    Molecules* molecules_;
+    PropagatorAnalyzer* analyzer_;
+    int precision_, n_streams_;
+    std::vector<std::string> monomers_;
+    std::map<std::string, int> prop_index_;
+    std::vector<std::string> qinit_names_;
+    std::vector<BlockKey> block_keys_;
+    std::vector<std::vector<Scheduler::Job>> schedule_;
+    std::map<std::string, std::vector<double>> last_w_;   // host fields of the last call (check_total_partition)
+    fts_solver* solver_ = nullptr;
+};
+
+class AndersonMixing {
+public:
+    AndersonMixing(int n_var, int max_hist, double start_error, double mix_min, double mix_init);
+    ~AndersonMixing();
+    AndersonMixing(const AndersonMixing&) = delete;
+    AndersonMixing& operator=(const AndersonMixing&) = delete;
+    int get_n_var() const { return n_var_; }
+    void reset_count();
+    void calculate_new_fields(double* w_new, const double* w_current, const double* w_deriv, double old_error_level,
+                              double error_level, bool on_device = false);
+
+private:
+    int n_var_;
+    fts_am* am_ = nullptr;
+};
+
+class AbstractFactory {
+public:
+    AbstractFactory(std::string name, int precision, bool reduce_memory_usage)
+        : name_(std::move(name)), precision_(precision), reduce_memory_usage_(reduce_memory_usage) {}
+    ComputationBox* create_computation_box(std::vector<int> nx, std::vector<double> lx, std::vector<std::string> bc,
+                                           const double* mask);
+    Molecules* create_molecules_information(const std::string& chain_model, double ds,
+                                            const std::map<std::string, double>& bond_lengths);
+    PropagatorAnalyzer* create_propagator_analyzer(Molecules* molecules, bool aggregate);
+    PropagatorComputation* create_pseudospectral_solver(ComputationBox* cb, Molecules* molecules,
+                                                        PropagatorAnalyzer* analyzer);
+    PropagatorComputation* create_realspace_solver(ComputationBox* cb, Molecules* molecules,
+                                                   PropagatorAnalyzer* analyzer);
+    AndersonMixing* create_anderson_mixing(int n_var, int max_hist, double start_error, double mix_min,
+                                           double mix_init);
+    std::string get_model_name() const { return model_name_; }
+    void display_info() const;
+
+private:
+    std::string name_;
+    int precision_;
+    bool reduce_memory_usage_;
+    std::string model_name_;
+};
+
+struct PlatformSelector {
+    static std::vector<std::string> avail_platforms();
+    static AbstractFactory* create_factory(const std::string& platform);
+    static AbstractFactory* create_factory(const std::string& platform, bool reduce_memory_usage);
+};
+
+}  // namespace lfts

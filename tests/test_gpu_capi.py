@@ -1,0 +1,211 @@
+"""Parity of the HIP path with the oracle and the reference's known answers,
+called through the C ABI of libfts_hip.so (include/fts_hip.h).
+
+FP64 bar: <= 1e-10 relative for Q, phi and stress against the FP64 oracle.
+FP32 (propagators/FFTs in float, Q/phi/stress accumulated in FP64): <= 2e-5
+relative for Q and <= 2e-5 of max|phi| for phi.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.conftest import golden
+
+pytestmark = pytest.mark.gpu
+
+KAT_NAMES = ["linear_continuous", "linear_discrete", "branched_continuous", "branched_discrete",
+             "mixture_continuous", "mixture_discrete"]
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+@pytest.fixture(scope="module")
+def capi():
+    from langevin_fts_amd import capi as c
+    return c
+
+
+@pytest.fixture(scope="module")
+def ctx(capi):
+    return capi.Context(0)
+
+
+def rel(a, b):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+@pytest.mark.parametrize("shape", [(5, 4, 3), (32, 32, 32), (64, 64, 64), (48, 40, 36), (7, 11, 13),
+                                   (23, 27, 25), (12, 10), (16,), (96, 96, 96)])
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_fft_matches_numpy(ctx, capi, shape, prec):
+    say(f"fft {shape} {prec}")
+    rng = np.random.default_rng(11)
+    a = rng.standard_normal(shape)
+    g = capi.Grid(ctx, list(shape), [1.0] * len(shape))
+    dt = np.float64 if prec == "fp64" else np.float32
+    p = capi.FTS_FP64 if prec == "fp64" else capi.FTS_FP32
+    batch = 3
+    x = capi.DeviceArray.from_host(np.stack([a, 2 * a, -a]).astype(dt))
+    spec = capi.DeviceArray(batch * g.Mc * 2 * np.dtype(dt).itemsize)
+    back = capi.DeviceArray(batch * g.M * np.dtype(dt).itemsize)
+    g.fft_forward(x.ptr, spec.ptr, batch, p)
+    s = spec.download((batch, g.Mc, 2), dt).astype(np.float64)
+    s = s[..., 0] + 1j * s[..., 1]
+    ref = np.fft.rfftn(a).reshape(-1)
+    tol = 1e-12 if prec == "fp64" else 3e-6
+    for b, f in enumerate([1, 2, -1]):
+        assert rel(s[b], f * ref) < tol
+    g.fft_backward(spec.ptr, back.ptr, batch, p)
+    r = back.download((batch, g.M), dt).astype(np.float64)
+    for b, f in enumerate([1, 2, -1]):
+        assert rel(r[b], f * a.reshape(-1)) < tol
+
+
+@pytest.mark.parametrize("name", KAT_NAMES)
+@pytest.mark.parametrize("agg", [0, 1])
+def test_reference_kats_on_gpu(ctx, capi, kats, plans, name, agg):
+    """TestPseudo{Linear,Branched,BranchedMixture}{Continuous,Discrete}3D.cpp literals."""
+    say(f"kat {name} agg={agg}")
+    k = kats[name]
+    plan = plans[f"{name.rsplit('_', 1)[0]}|{k['model']}|{agg}"]
+    g = capi.Grid(ctx, k["nx"], k["lx"])
+    P = capi.Plan(plan, k["ds"], k["bonds"], plan["schedules"]["4"])
+    s = capi.Solver(g, P, capi.FTS_FP64)
+    s.compute_statistics(k["w"])
+    V = float(np.prod(k["lx"]))
+    for p, qv in enumerate(k["QV"]):
+        Q = s.get_total_partition(p)
+        err = abs(Q - qv / V) / (abs(Q) if k.get("QV_relative") else 1.0)
+        assert err < 1e-10, (p, Q, qv / V)
+    for m, ref in k["phi"].items():
+        assert np.max(np.abs(s.get_total_concentration(m) - np.array(ref))) < 1e-10
+    if agg == 0:
+        for cp in k["chain_propagators"]:
+            n = [b["n_segment"] for b in plan["polymers"][cp["p"]]["blocks"]
+                 if {b["v"], b["u"]} == {cp["v"], cp["u"]}][0]
+            q = s.get_chain_propagator(cp["p"], cp["v"], cp["u"], n)
+            assert np.max(np.abs(q - np.array(cp["ref"]))) < 1e-10
+    if k["stress"]:
+        s.compute_stress()
+        assert np.max(np.abs(s.get_stress() - np.array(k["stress"]))) < 1e-8
+
+
+CASES = [
+    ("branched", "continuous", 0), ("branched", "discrete", 1),
+    ("mixture", "continuous", 1), ("mixture", "discrete", 0),
+    ("side_chains", "continuous", 1), ("side_chains", "discrete", 1),
+    ("dendritic", "discrete", 1), ("aggregation3d", "continuous", 1),
+    ("star_abc_homo_a", "continuous", 0), ("with_solvent", "discrete", 0), ("with_solvent", "continuous", 0),
+    ("grafted", "continuous", 0), ("grafted", "discrete", 0),
+    ("diblock_f0.5_ds1_90", "discrete", 0),
+]
+
+
+def _pair(ctx, capi, name, model, agg, nx, lx, precision, streams="4", seed=5, mask=None):
+    plan = golden("plans")[f"{name}|{model}|{agg}"]
+    inp = plan["input"]
+    M = int(np.prod(nx))
+    rng = np.random.default_rng(seed)
+    w = {m: 0.6 * rng.standard_normal(M) for m in inp["bonds"]}
+    q_init = {"G": 0.5 + rng.random(M)} if name == "grafted" else None
+    og = O.Grid(nx, lx, mask)
+    ref = O.Solver(og, plan, inp["ds"], inp["bonds"], inp["solvents"])
+    ref.compute_statistics(w, q_init)
+    g = capi.Grid(ctx, nx, lx, mask)
+    P = capi.Plan(plan, inp["ds"], inp["bonds"], plan["schedules"][streams], inp["solvents"])
+    s = capi.Solver(g, P, precision)
+    s.compute_statistics(w, q_init)
+    return plan, ref, s
+
+
+@pytest.mark.parametrize("name,model,agg", CASES)
+def test_fp64_matches_oracle(ctx, capi, name, model, agg):
+    nx, lx = [12, 10, 9], [3.1, 2.7, 2.2]
+    say(f"fp64 parity {name} {model} agg={agg} {nx}")
+    plan, ref, s = _pair(ctx, capi, name, model, agg, nx, lx, capi.FTS_FP64)
+    for p in range(len(plan["polymers"])):
+        assert abs(s.get_total_partition(p) / ref.get_total_partition(p) - 1) < 1e-10
+    for i in range(len(plan["input"]["solvents"])):
+        assert abs(s.get_solvent_partition(i) / ref.get_solvent_partition(i) - 1) < 1e-10
+    for m in plan["input"]["bonds"]:
+        assert rel(s.get_total_concentration(m), ref.get_total_concentration(m)) < 1e-10
+    s.compute_stress()
+    ref.compute_stress()
+    assert rel(s.get_stress(), ref.get_stress()) < 1e-10
+
+
+@pytest.mark.parametrize("name,model,agg", [("branched", "continuous", 1), ("mixture", "discrete", 1),
+                                            ("diblock_f0.5_ds1_90", "discrete", 0)])
+def test_fp32_within_stated_error(ctx, capi, name, model, agg):
+    nx, lx = [16, 12, 10], [3.1, 2.7, 2.2]
+    say(f"fp32 {name} {model} agg={agg}")
+    plan, ref, s = _pair(ctx, capi, name, model, agg, nx, lx, capi.FTS_FP32)
+    for p in range(len(plan["polymers"])):
+        assert abs(s.get_total_partition(p) / ref.get_total_partition(p) - 1) < 2e-5
+    for m in plan["input"]["bonds"]:
+        assert rel(s.get_total_concentration(m), ref.get_total_concentration(m)) < 2e-5
+
+
+@pytest.mark.parametrize("model", ["continuous", "discrete"])
+def test_schedule_width_does_not_change_results(ctx, capi, model):
+    """Batching follows the schedule; every stream count gives bitwise the same fields."""
+    nx, lx = [10, 8, 6], [2.9, 2.4, 2.0]
+    say(f"schedules {model}")
+    outs = []
+    for st in ["1", "2", "4", "8"]:
+        plan, ref, s = _pair(ctx, capi, "side_chains", model, 1, nx, lx, capi.FTS_FP64, streams=st)
+        outs.append(np.concatenate([s.get_total_concentration("A"), s.get_total_concentration("B")]))
+    for o in outs[1:]:
+        assert np.array_equal(o, outs[0])
+
+
+def test_mask_matches_oracle(ctx, capi):
+    nx, lx = [12, 10, 8], [3.0, 2.5, 2.0]
+    say("mask")
+    rng = np.random.default_rng(3)
+    mask = (rng.random(int(np.prod(nx))) > 0.15).astype(np.float64)
+    for model in ["continuous", "discrete"]:
+        plan, ref, s = _pair(ctx, capi, "branched", model, 0, nx, lx, capi.FTS_FP64, mask=mask)
+        assert abs(s.get_total_partition(0) / ref.get_total_partition(0) - 1) < 1e-10
+        assert rel(s.get_total_concentration("A"), ref.get_total_concentration("A")) < 1e-10
+
+
+def test_box_reductions(ctx, capi):
+    nx, lx = [9, 8, 7], [2.0, 3.0, 4.0]
+    rng = np.random.default_rng(1)
+    a, b = rng.standard_normal((2, int(np.prod(nx))))
+    g = capi.Grid(ctx, nx, lx)
+    og = O.Grid(nx, lx)
+    assert abs(g.integral(a) - og.integral(a)) < 1e-12 * np.sum(np.abs(a))
+    assert abs(g.inner_product(a, b) - og.inner_product(a, b)) < 1e-12 * np.sum(np.abs(a * b))
+    g.set_lx([2.5, 3.0, 4.5])
+    og.set_lx([2.5, 3.0, 4.5])
+    assert abs(g.volume - og.volume) < 1e-12 * og.volume
+
+
+def am_map(w):
+    i = np.arange(w.size)
+    f = 0.5 * np.tanh(w) + 0.3 * np.cos(0.1 * i) + 0.2 * np.roll(w, -1)
+    d = f - w
+    return d, np.sqrt(np.sum(d * d) / (1.0 + np.sum(w * w)))
+
+
+@pytest.mark.parametrize("name", ["hist20", "hist5_late"])
+def test_anderson_mixing_matches_reference(ctx, capi, name):
+    """Trajectory of the reference's own CpuAndersonMixing (dumped by oracle/_ref)."""
+    say(f"am {name}")
+    t = golden("am")[name]
+    n_var, hist, start, mmin, minit, n_iter = t["params"]
+    am = capi.AndersonMixing(ctx, n_var, hist, start, mmin, minit)
+    w = np.array(t["w0"])
+    err = 1e20
+    for step in t["steps"]:
+        d, e = am_map(w)
+        old, err = err, e
+        wn = am.calculate_new_fields(w, d, old, err)
+        np.testing.assert_allclose(wn, step["w_new"], rtol=0, atol=1e-10)
+        w = np.array(step["w_new"])
