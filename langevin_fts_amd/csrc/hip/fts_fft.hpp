@@ -424,6 +424,10 @@ __device__ __forceinline__ void load_spectrum(T2* S, const SlabGeom& g, const T2
         const int j = f / g.nzc, k = f - j * g.nzc;
         S[j * g.P + k] = c[f];
     }
+    // odd ny: the partner row of the last row is merged into the same complex
+    // z-row by the inverse; it must hold zeros, not what the LDS held before
+    if (g.ny & 1)
+        for (int k = threadIdx.x; k < g.nzc; k += blockDim.x) S[g.ny * g.P + k] = T2{0, 0};
 }
 
 template <typename T2>

@@ -770,7 +770,10 @@ void Solver<T>::propagator(int prop, int n, int hf, double* out, bool on_device)
 
 std::unique_ptr<SolverBase> make_solver(Grid* g, const fts_plan* plan, int precision) {
     if (precision == FTS_FP64) return std::unique_ptr<SolverBase>(new Solver<double>(g, plan));
-    if (precision == FTS_FP32) return std::unique_ptr<SolverBase>(new Solver<float>(g, plan));
+    if (precision == FTS_FP32) {
+        FTS_REQUIRE(!g->big, "FP32 propagators need grid sizes whose prime factors are 2, 3 and 5");
+        return std::unique_ptr<SolverBase>(new Solver<float>(g, plan));
+    }
     throw Error("precision must be FTS_FP64 or FTS_FP32");
 }
 

@@ -32,6 +32,16 @@ def ctx(capi):
     return capi.Context(0)
 
 
+def _prime_factors(n):
+    out, f = [], 2
+    while n > 1:
+        while n % f == 0:
+            out.append(f)
+            n //= f
+        f += 1
+    return out
+
+
 def rel(a, b):
     a = np.asarray(a)
     b = np.asarray(b)
@@ -42,6 +52,9 @@ def rel(a, b):
                                    (23, 27, 25), (12, 10), (16,), (96, 96, 96)])
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
 def test_fft_matches_numpy(ctx, capi, shape, prec):
+    if prec == "fp32" and any(f > 5 for n in shape for f in _prime_factors(n)):
+        pytest.skip("FP32 kernels with radices 7..31 hung on the MI355X (cause not found); the library "
+                    "rejects FP32 on such grids")
     say(f"fft {shape} {prec}")
     rng = np.random.default_rng(11)
     a = rng.standard_normal(shape)
@@ -136,6 +149,35 @@ def test_fp64_matches_oracle(ctx, capi, name, model, agg):
     s.compute_stress()
     ref.compute_stress()
     assert rel(s.get_stress(), ref.get_stress()) < 1e-10
+
+
+@pytest.mark.parametrize("n", [32, 48])
+@pytest.mark.parametrize("name,model", [("diblock_f0.5_ds1_90", "discrete"), ("branched", "continuous"),
+                                        ("star_abc_homo_a", "continuous")])
+def test_specialised_cubic_kernels_match_oracle(ctx, capi, n, name, model):
+    """N^3 grids run the compile-time specialised slab/pencil kernels."""
+    say(f"cubic {n} {name} {model}")
+    plan, ref, s = _pair(ctx, capi, name, model, 0, [n, n, n], [4.0, 4.0, 4.0], capi.FTS_FP64)
+    for p in range(len(plan["polymers"])):
+        assert abs(s.get_total_partition(p) / ref.get_total_partition(p) - 1) < 1e-10
+    for m in plan["input"]["bonds"]:
+        assert rel(s.get_total_concentration(m), ref.get_total_concentration(m)) < 1e-10
+    s.compute_stress()
+    ref.compute_stress()
+    assert rel(s.get_stress(), ref.get_stress()) < 1e-10
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_specialised_and_runtime_kernels_agree(ctx, capi, prec, monkeypatch):
+    say(f"ct vs runtime {prec}")
+    p = capi.FTS_FP64 if prec == "fp64" else capi.FTS_FP32
+    out = []
+    for no_ct in [False, True]:
+        if no_ct:
+            monkeypatch.setenv("FTS_NO_CT", "1")
+        plan, ref, s = _pair(ctx, capi, "branched", "discrete", 1, [32, 32, 32], [3.5, 3.5, 3.5], p)
+        out.append(np.concatenate([s.get_total_concentration("A"), s.get_total_concentration("B")]))
+    assert rel(out[0], out[1]) < (1e-13 if prec == "fp64" else 1e-5)
 
 
 @pytest.mark.parametrize("name,model,agg", [("branched", "continuous", 1), ("mixture", "discrete", 1),
