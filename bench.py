@@ -1,0 +1,222 @@
+"""Langevin FTS throughput on one MI355X, with the propagator kernels' roofline.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload lfts64|lfts32]
+
+Workload "lfts64" (default): AB diblock melt, discrete chain N=90, f=0.5, chiN=17,
+64^3 periodic grid, L=6.0 R0, nbar=1e4, Langevin step dt=0.8 with PCG64 noise,
+saddle point to 1e-4 by Anderson mixing (history 20, start 0.5, mix 0.01/0.01),
+everything FP64. A "step" is one Langevin step: the Leimkuhler-Matthews update of
+w- and the saddle-point iteration of w+ (each iteration = all propagators,
+concentrations and one Anderson mixing), as examples/fts/lfts.py:687-731 does it.
+Fields start as N(0,1) from a fixed seed; W warm-up steps (and the initial saddle
+point) run untimed, then K steps are timed between stream synchronisations.
+
+Printed: one JSON line. `roofline` is the slab pass (the dominant kernel:
+inverse y/z FFT, exp(-w ds) factor, q store, forward z/y FFT) timed with HIP events
+around every launch on the library's stream during the timed steps; its
+algorithmic bytes are the spectra, fields and w each launch must move once.
+`cpu_baseline` times the CPU oracle (FP64, OpenMP FFT lines) on one saddle
+iteration of the same 64^3 workload and scales by the GPU run's iterations per step.
+"""
+import argparse
+import contextlib
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+PEAK_HBM_GBPS = 8000.0
+
+WORKLOADS = {
+    "lfts64": dict(nx=[64, 64, 64], lx=[6.0, 6.0, 6.0], model="discrete", ds=1 / 90, f=0.5, chi_n=17.0,
+                   nbar=1e4, dt=0.8, tol=1e-4, am=(20, 5e-1, 0.01, 0.01), max_iter=200),
+    "lfts32": dict(nx=[32, 32, 32], lx=[4.0, 4.0, 4.0], model="discrete", ds=1 / 90, f=0.5, chi_n=17.0,
+                   nbar=1e4, dt=0.8, tol=1e-4, am=(20, 5e-1, 0.01, 0.01), max_iter=200),
+}
+
+
+@contextlib.contextmanager
+def stdout_to_stderr():
+    """Send everything written to fd 1 (Python and the C++ layer) to fd 2."""
+    sys.stdout.flush()
+    saved = os.dup(1)
+    os.dup2(2, 1)
+    try:
+        with contextlib.redirect_stdout(sys.stderr):
+            yield
+    finally:
+        sys.stdout.flush()
+        os.dup2(saved, 1)
+        os.close(saved)
+
+
+def describe(name, w):
+    n = round(1 / w["ds"])
+    return (f"L-FTS AB diblock, {w['model']} N={n}, f={w['f']}, chiN={w['chi_n']}, "
+            f"{'x'.join(map(str, w['nx']))} grid, L={w['lx'][0]}, nbar={w['nbar']:g}, dt={w['dt']}, "
+            f"saddle tol {w['tol']:g}")
+
+
+def lfts_params(w, rec_dir):
+    return {
+        "nx": w["nx"], "lx": w["lx"], "chain_model": w["model"], "ds": w["ds"],
+        "segment_lengths": {"A": 1.0, "B": 1.0}, "chi_n": {"A,B": w["chi_n"]},
+        "distinct_polymers": [{"volume_fraction": 1.0, "blocks": [{"type": "A", "length": w["f"]},
+                                                                 {"type": "B", "length": 1 - w["f"]}]}],
+        "langevin": {"max_step": 0, "dt": w["dt"], "nbar": w["nbar"]},
+        "recording": {"dir": rec_dir, "recording_period": 10 ** 9, "sf_computing_period": 10 ** 9,
+                      "sf_recording_period": 10 ** 9},
+        "saddle": {"max_iter": w["max_iter"], "tolerance": w["tol"]},
+        "am": {"max_hist": w["am"][0], "start_error": w["am"][1], "mix_min": w["am"][2], "mix_init": w["am"][3]},
+        "verbose_level": 0,
+    }
+
+
+def langevin_steps(sim, n, stats):
+    """n Langevin steps on the device state (lfts.py:687-731)."""
+    dev = sim.saddle_device
+    M = sim.cb.get_n_grid()
+    R = len(sim.mpt.aux_fields_real_idx)
+    tol = sim.saddle["tolerance"]
+    for _ in range(n):
+        dev.backup()
+        noise = sim.random.normal(0.0, sim.langevin["sigma"], [R, M])
+        dev.langevin_update(noise, sim.langevin["dt"])
+        it, err, _ = sim._find_saddle_point_device()
+        stats["iterations"] += it
+        if np.isnan(err) or err >= tol:
+            dev.restore()
+            stats["failures"] += 1
+
+
+def cpu_baseline(w, iterations_per_step, budget_s):
+    """FP64 oracle on one or more saddle iterations of the same workload."""
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from oracle import oracle as O
+    from langevin_fts_amd import langevinfts as lf
+
+    # the propagator plan of this molecule from the product's analyzer (host code only)
+    factory = lf.PlatformSelector.create_factory("hip", False)
+    mol = factory.create_molecules_information(w["model"], w["ds"], {"A": 1.0, "B": 1.0})
+    mol.add_polymer(1.0, [["A", w["f"], 0, 1], ["B", 1 - w["f"], 1, 2]])
+    an = factory.create_propagator_analyzer(mol, True)
+    plan = plan_dict(mol, an, w["model"])
+    M = int(np.prod(w["nx"]))
+    grid = O.Grid(w["nx"], w["lx"])
+    solver = O.Solver(grid, plan, w["ds"], {"A": 1.0, "B": 1.0})
+    am = O.AndersonMixing(M, *w["am"])
+    rng = np.random.default_rng(1)
+    w_aux = np.stack([rng.standard_normal(M), rng.standard_normal(M)])
+    n, t0 = 0, time.perf_counter()
+    err = 1e20
+    while True:
+        solver.compute_statistics({"A": w_aux[0] + w_aux[1], "B": -w_aux[0] + w_aux[1]})
+        h = -(solver.get_total_concentration("A") + solver.get_total_concentration("B") - 1.0)
+        old, err = err, np.std(h)
+        w_aux[1] = am.calculate_new_fields(w_aux[1], -h, old, err)
+        n += 1
+        if time.perf_counter() - t0 > budget_s:
+            break
+    t_iter = (time.perf_counter() - t0) / n
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": 1.0 / (t_iter * iterations_per_step), "unit": "steps/s", "cores": threads, "kind": "port",
+            "sample": f"{n} saddle iteration(s) of the same 64^3 workload on the FP64 C oracle "
+                      f"({t_iter:.2f} s each, FFT lines over {threads} OpenMP threads) x "
+                      f"{iterations_per_step:.1f} iterations per Langevin step measured on the GPU"}
+
+
+def plan_dict(mol, an, model):
+    """The analyzer's propagators and blocks in the oracle's plan form."""
+    props = []
+    for key, e in an.get_computation_propagators().items():
+        props.append({"key": key, "max_n_segment": e.max_n_segment, "monomer": e.monomer_type,
+                      "deps": [list(d) for d in e.deps], "junction_ends": sorted(e.junction_ends)})
+    blocks = []
+    for (p, left, right), b in an.get_computation_blocks().items():
+        blocks.append({"polymer": p, "left": left, "right": right, "monomer": b.monomer_type,
+                       "n_left": b.n_segment_left, "n_right": b.n_segment_right, "n_repeated": b.n_repeated,
+                       "v_u": [list(x) for x in b.v_u]})
+    pols = [{"alpha": mol.get_polymer(p).get_alpha(), "volume_fraction": mol.get_polymer(p).get_volume_fraction()}
+            for p in range(mol.get_n_polymer_types())]
+    return {"model": model, "polymers": pols, "propagators": props, "blocks": blocks}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="lfts64", choices=sorted(WORKLOADS))
+    ap.add_argument("--seed", type=int, default=12345)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline budget (0: skip)")
+    ap.add_argument("--no-kernel-timing", action="store_true", help="time the steps without launch events")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported (the L-FTS path here is a "
+              "single-GPU build)", file=sys.stderr)
+        sys.exit(2)
+
+    from langevin_fts_amd import langevinfts as lf
+    from langevin_fts_amd import lfts
+
+    w = WORKLOADS[args.workload]
+    M = int(np.prod(w["nx"]))
+    with stdout_to_stderr():
+        sim = lfts.LFTS(lfts_params(w, "/tmp/lfts_bench"), random_seed=args.seed)
+        rng = np.random.Generator(np.random.PCG64(args.seed))
+        w_mono = np.stack([rng.standard_normal(M), rng.standard_normal(M)])
+        sim.saddle_device.set_fields(np.ascontiguousarray(sim.mpt.to_aux_fields(w_mono)))
+        sim.saddle_device.set_noise_prev(np.zeros([len(sim.mpt.aux_fields_real_idx), M]))
+        sim._find_saddle_point_device()
+        warm = {"iterations": 0, "failures": 0}
+        langevin_steps(sim, args.warmup, warm)
+
+        if not args.no_kernel_timing:
+            lf.set_kernel_timing(True)
+        stats = {"iterations": 0, "failures": 0}
+        sim.saddle_device.field_moments()   # synchronises the stream
+        t0 = time.perf_counter()
+        langevin_steps(sim, args.steps, stats)
+        sim.saddle_device.field_moments()
+        elapsed = time.perf_counter() - t0
+        kern = {}
+        if not args.no_kernel_timing:
+            kern = {f: lf.get_kernel_timing(f) for f in ("slab", "pencil")}
+            lf.set_kernel_timing(False)
+
+    steps_per_s = args.steps / elapsed
+    iters = stats["iterations"] / args.steps
+    out = {
+        "metric": "L-FTS steps/s @64³ (learned vs plain AM guess, same tol); propagator HBM GB/s",
+        "value": steps_per_s, "unit": "steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / args.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic: w_A, w_B ~ N(0,1) from PCG64(seed); Langevin noise from numpy PCG64",
+        "config": {"workload": describe(args.workload, w), "guess": "plain Anderson mixing",
+                   "saddle_iterations_per_step": iters, "saddle_failures": stats["failures"],
+                   "propagator_evaluations_per_s": stats["iterations"] / elapsed},
+    }
+    if kern:
+        sl, pe = kern["slab"], kern["pencil"]
+        achieved = sl["bytes"] / (sl["milliseconds"] * 1e-3) / 1e9
+        out["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBPS, "unit": "GB/s",
+                           "frac": achieved / PEAK_HBM_GBPS, "traffic": None,
+                           "kernel": "slab pass (slab_kernel_ct, 64^3 FP64)",
+                           "launches": sl["launches"],
+                           "avg_launch_us": 1e3 * sl["milliseconds"] / max(sl["launches"], 1),
+                           "bytes_per_launch": sl["bytes"] / max(sl["launches"], 1)}
+        both_ms = sl["milliseconds"] + pe["milliseconds"]
+        out["config"]["propagator_GBps"] = (sl["bytes"] + pe["bytes"]) / (both_ms * 1e-3) / 1e9
+        out["config"]["pencil_GBps"] = pe["bytes"] / (pe["milliseconds"] * 1e-3) / 1e9
+        out["config"]["pencil_avg_launch_us"] = 1e3 * pe["milliseconds"] / max(pe["launches"], 1)
+        out["config"]["propagator_kernel_share"] = both_ms * 1e-3 / elapsed
+    if args.cpu_seconds > 0:
+        with stdout_to_stderr():
+            out["cpu_baseline"] = cpu_baseline(w, iters, args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

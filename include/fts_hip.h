@@ -20,6 +20,7 @@ typedef struct fts_context fts_context;
 typedef struct fts_grid fts_grid;
 typedef struct fts_solver fts_solver;
 typedef struct fts_am fts_am;
+typedef struct fts_saddle fts_saddle;
 
 enum { FTS_OK = 0, FTS_ERROR = 1 };
 enum { FTS_CONTINUOUS = 0, FTS_DISCRETE = 1 };
@@ -28,6 +29,15 @@ enum { FTS_FP64 = 0, FTS_FP32 = 1 };
 enum { FTS_LEAF = 0, FTS_LEAF_QINIT = 1, FTS_JUNCTION = 2, FTS_AGGREGATED = 3 };
 /* sources of fts_solver_sum_concentrations */
 enum { FTS_SRC_BLOCK = 0, FTS_SRC_SOLVENT = 1 };
+/* kernel families of fts_context_get_kernel_timing */
+enum { FTS_KERNEL_SLAB = 0, FTS_KERNEL_PENCIL = 1, FTS_KERNEL_FAMILIES = 2 };
+typedef struct fts_kernel_stats {
+    double milliseconds;   /* summed launch durations (HIP events around each launch) */
+    double bytes;          /* summed algorithmic bytes: each field, spectrum and w read or written once */
+    long long launches;
+} fts_kernel_stats;
+/* most monomer types of a field theory handled by fts_saddle */
+#define FTS_MAX_TYPES 8
 
 /* Last error message of the calling thread ("" if none). */
 const char* fts_last_error(void);
@@ -38,6 +48,11 @@ int fts_abi_version(void);
  *      CudaCommon.cu:15-113) */
 int fts_context_create(int device, fts_context** out);
 int fts_context_destroy(fts_context* ctx);
+/* Kernel timing (no reference counterpart: measurement of the slab and pencil
+ * passes for roofline reports). Enabling clears the totals; get synchronises the
+ * stream and returns the totals of one family since the last enable. */
+int fts_context_set_kernel_timing(fts_context* ctx, int enable);
+int fts_context_get_kernel_timing(fts_context* ctx, int family, fts_kernel_stats* out);
 int fts_context_synchronize(fts_context* ctx);
 /* hipStream_t of the context, returned as void* */
 int fts_context_stream(fts_context* ctx, void** stream);
@@ -145,6 +160,66 @@ int fts_am_destroy(fts_am* am);
 int fts_am_reset(fts_am* am);
 int fts_am_calculate_new_fields(fts_am* am, double* w_new, const double* w_current, const double* w_deriv,
                                 double old_error_level, double error_level, int on_device);
+
+/* ---- L-FTS saddle point and Langevin update, device-resident
+ *      (examples/fts/lfts.py:631-868: LFTS.run, find_saddle_point,
+ *      compute_concentrations; SymmetricPolymerTheory.compute_func_deriv and
+ *      compute_hamiltonian, lfts.py:197-245). The field theory arrives as the
+ *      small coefficient tables the driver computes on the host:
+ *        solver field m   = sum_j w_map[m][j] w_aux[j]                (lfts.py:534-545)
+ *        phi[type i]      = sum_t scale_t * (block or solvent term t)  (lfts.py:550-565)
+ *        dH/dw_aux[imag c] = sum_j imag_cw[c][j] w_aux[j] + imag_cp[c][j] phi[j] + imag_c0[c]
+ *                           (sign of lfts.py:243-244 already applied)
+ *        dH/dw_aux[real c] = the same with real_cw, real_cp, real_c0
+ *        H = sum_j h_lin[j] <w_aux[j]> + h_sq[j] <w_aux[j]^2>
+ *            - sum_p polymer_weight[p] ln Q_p + h_const               (lfts.py:197-216)
+ *      <.> is the plain grid mean (numpy mean). */
+typedef struct fts_field_theory {
+    int n_types;           /* S, at most FTS_MAX_TYPES; w_aux and phi are [S][n_grid] */
+    int n_real, n_imag;    /* R and I */
+    const int* real_idx;   /* R indices into w_aux */
+    const int* imag_idx;   /* I indices into w_aux (the last one is the pressure field) */
+    int n_solver_fields;   /* monomer fields the solver takes, in its monomer order (<= 8) */
+    const double* w_map;   /* n_solver_fields x S */
+    const int* phi_term_off;      /* S + 1 offsets into the term arrays */
+    const int* phi_term_kind;     /* FTS_SRC_BLOCK or FTS_SRC_SOLVENT */
+    const int* phi_term_index;    /* block or solvent index of the solver */
+    const double* phi_term_scale;
+    const double* real_cw; const double* real_cp; const double* real_c0;   /* R x S, R x S, R */
+    const double* imag_cw; const double* imag_cp; const double* imag_c0;   /* I x S, I x S, I */
+    const double* dt_scaling;     /* S (lfts.py:498-501) */
+    const double* h_lin;          /* S */
+    const double* h_sq;           /* S */
+    double h_const;
+    int n_polymers;
+    const double* polymer_weight; /* volume fraction / alpha per polymer */
+} fts_field_theory;
+
+int fts_saddle_create(fts_context* ctx, fts_solver* solver, fts_am* am, int n_grid, const fts_field_theory* ft,
+                      fts_saddle** out);
+int fts_saddle_destroy(fts_saddle* s);
+/* w_aux [S][n_grid] in and out */
+int fts_saddle_set_fields(fts_saddle* s, const double* w_aux, int on_device);
+int fts_saddle_get_fields(fts_saddle* s, double* w_aux, int on_device);
+/* phi of monomer type i from the last saddle iteration */
+int fts_saddle_get_concentration(fts_saddle* s, int type, double* out, int on_device);
+/* normal_noise_prev [R][n_grid] (lfts.py:669-672, :712; checkpoint state) */
+int fts_saddle_set_noise(fts_saddle* s, const double* noise_prev, int on_device);
+int fts_saddle_get_noise(fts_saddle* s, double* noise_prev, int on_device);
+/* copy of w_aux and phi taken before a Langevin step and put back when the saddle
+ * point fails (lfts.py:696-698, :727-729) */
+int fts_saddle_backup(fts_saddle* s);
+int fts_saddle_restore(fts_saddle* s);
+/* find_saddle_point (lfts.py:806-867). verbose 2: log row per iteration; verbose 1:
+ * one row for the final iteration; row = [iter, mass error, H, Q_0..Q_{P-1},
+ * error_0..error_{I-1}] (3 + P + I doubles). hamiltonian: H at the final iteration. */
+int fts_saddle_find(fts_saddle* s, double tolerance, int max_iter, int verbose, double* log, int* n_iter,
+                    double* error_level, double* hamiltonian);
+/* Leimkuhler-Matthews update of the real fields with this step's noise [R][n_grid]
+ * (lfts.py:701-713); the noise then becomes normal_noise_prev. */
+int fts_saddle_langevin_update(fts_saddle* s, const double* noise, int on_device, double dt);
+/* plain means of w_aux[j] and w_aux[j]^2, S each (lfts.py:250-262 dH/dchiN terms) */
+int fts_saddle_field_moments(fts_saddle* s, double* mean, double* mean_sq);
 
 #ifdef __cplusplus
 }

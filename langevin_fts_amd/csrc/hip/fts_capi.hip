@@ -60,6 +60,24 @@ int fts_context_destroy(fts_context* ctx) {
     });
 }
 
+int fts_context_set_kernel_timing(fts_context* ctx, int enable) {
+    return guarded([&] {
+        need(ctx, "ctx");
+        FTS_HIP_CHECK(hipSetDevice(ctx->device));
+        ctx->timer.enable(enable != 0);
+    });
+}
+
+int fts_context_get_kernel_timing(fts_context* ctx, int family, fts_kernel_stats* out) {
+    return guarded([&] {
+        need(ctx, "ctx");
+        need(out, "out");
+        FTS_REQUIRE(family >= 0 && family < FTS_KERNEL_FAMILIES, "kernel family out of range");
+        FTS_HIP_CHECK(hipSetDevice(ctx->device));
+        *out = ctx->timer.get(family);
+    });
+}
+
 int fts_context_synchronize(fts_context* ctx) {
     return guarded([&] {
         need(ctx, "ctx");
@@ -305,6 +323,132 @@ int fts_am_calculate_new_fields(fts_am* am, double* w_new, const double* w_curre
         need(w_current, "w_current");
         need(w_deriv, "w_deriv");
         am->a->calculate_new_fields(w_new, w_current, w_deriv, old_error_level, error_level, on_device != 0);
+    });
+}
+
+int fts_saddle_create(fts_context* ctx, fts_solver* solver, fts_am* am, int n_grid, const fts_field_theory* ft,
+                      fts_saddle** out) {
+    return guarded([&] {
+        need(ctx, "ctx");
+        need(solver, "solver");
+        need(am, "am");
+        need(ft, "field theory");
+        need(out, "out");
+        const int S = ft->n_types, R = ft->n_real, I = ft->n_imag;
+        FTS_REQUIRE(S >= 1 && S <= FTS_MAX_TYPES, "n_types must be in [1, 8]");
+        need(ft->imag_idx, "imag_idx");
+        need(ft->w_map, "w_map");
+        need(ft->phi_term_off, "phi_term_off");
+        if (ft->phi_term_off[S] > 0) {
+            need(ft->phi_term_kind, "phi_term_kind");
+            need(ft->phi_term_index, "phi_term_index");
+            need(ft->phi_term_scale, "phi_term_scale");
+        }
+        if (R > 0) {
+            need(ft->real_idx, "real_idx");
+            need(ft->real_cw, "real_cw");
+            need(ft->real_cp, "real_cp");
+            need(ft->real_c0, "real_c0");
+        }
+        FTS_REQUIRE(I >= 1, "n_imag must be positive");
+        need(ft->imag_cw, "imag_cw");
+        need(ft->imag_cp, "imag_cp");
+        need(ft->imag_c0, "imag_c0");
+        need(ft->dt_scaling, "dt_scaling");
+        need(ft->h_lin, "h_lin");
+        need(ft->h_sq, "h_sq");
+        need(ft->polymer_weight, "polymer_weight");
+        auto* s = new fts_saddle;
+        try {
+            s->s = std::make_unique<fts::SaddlePoint>(ctx, solver->s.get(), am->a.get(), n_grid, *ft);
+        } catch (...) {
+            delete s;
+            throw;
+        }
+        *out = s;
+    });
+}
+
+int fts_saddle_destroy(fts_saddle* s) {
+    return guarded([&] { delete s; });
+}
+
+int fts_saddle_set_fields(fts_saddle* s, const double* w_aux, int on_device) {
+    return guarded([&] {
+        need(s, "saddle");
+        need(w_aux, "w_aux");
+        s->s->set_fields(w_aux, on_device != 0);
+    });
+}
+
+int fts_saddle_get_fields(fts_saddle* s, double* w_aux, int on_device) {
+    return guarded([&] {
+        need(s, "saddle");
+        need(w_aux, "w_aux");
+        s->s->get_fields(w_aux, on_device != 0);
+    });
+}
+
+int fts_saddle_get_concentration(fts_saddle* s, int type, double* out, int on_device) {
+    return guarded([&] {
+        need(s, "saddle");
+        need(out, "out");
+        s->s->get_concentration(type, out, on_device != 0);
+    });
+}
+
+int fts_saddle_set_noise(fts_saddle* s, const double* noise_prev, int on_device) {
+    return guarded([&] {
+        need(s, "saddle");
+        need(noise_prev, "noise_prev");
+        s->s->set_noise(noise_prev, on_device != 0);
+    });
+}
+
+int fts_saddle_get_noise(fts_saddle* s, double* noise_prev, int on_device) {
+    return guarded([&] {
+        need(s, "saddle");
+        need(noise_prev, "noise_prev");
+        s->s->get_noise(noise_prev, on_device != 0);
+    });
+}
+
+int fts_saddle_backup(fts_saddle* s) {
+    return guarded([&] {
+        need(s, "saddle");
+        s->s->backup();
+    });
+}
+
+int fts_saddle_restore(fts_saddle* s) {
+    return guarded([&] {
+        need(s, "saddle");
+        s->s->restore();
+    });
+}
+
+int fts_saddle_find(fts_saddle* s, double tolerance, int max_iter, int verbose, double* log, int* n_iter,
+                    double* error_level, double* hamiltonian) {
+    return guarded([&] {
+        need(s, "saddle");
+        s->s->find(tolerance, max_iter, verbose, log, n_iter, error_level, hamiltonian);
+    });
+}
+
+int fts_saddle_langevin_update(fts_saddle* s, const double* noise, int on_device, double dt) {
+    return guarded([&] {
+        need(s, "saddle");
+        need(noise, "noise");
+        s->s->langevin_update(noise, on_device != 0, dt);
+    });
+}
+
+int fts_saddle_field_moments(fts_saddle* s, double* mean, double* mean_sq) {
+    return guarded([&] {
+        need(s, "saddle");
+        need(mean, "mean");
+        need(mean_sq, "mean_sq");
+        s->s->field_moments(mean, mean_sq);
     });
 }
 

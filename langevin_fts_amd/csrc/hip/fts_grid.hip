@@ -8,6 +8,66 @@
 
 namespace fts {
 
+// ------------------------------------------------------------------ kernel timer
+KernelTimer::~KernelTimer() {
+    for (auto& p : pending_) {
+        (void)hipEventDestroy(p.a);
+        (void)hipEventDestroy(p.b);
+    }
+    for (auto e : pool_) (void)hipEventDestroy(e);
+}
+
+void KernelTimer::enable(bool on) {
+    drain(pending_.size());
+    for (auto& s : stats_) s = fts_kernel_stats{};
+    on_ = on;
+}
+
+hipEvent_t KernelTimer::take() {
+    if (pool_.empty()) {
+        hipEvent_t e;
+        FTS_HIP_CHECK(hipEventCreate(&e));
+        return e;
+    }
+    hipEvent_t e = pool_.back();
+    pool_.pop_back();
+    return e;
+}
+
+hipEvent_t KernelTimer::begin(hipStream_t st) {
+    hipEvent_t a = take();
+    FTS_HIP_CHECK(hipEventRecord(a, st));
+    return a;
+}
+
+void KernelTimer::end(hipStream_t st, hipEvent_t a, int family, double bytes) {
+    hipEvent_t b = take();
+    FTS_HIP_CHECK(hipEventRecord(b, st));
+    pending_.push_back({a, b, family, bytes});
+    // the older half finished long ago: reading it never waits on the GPU's queue
+    if (pending_.size() >= 8192) drain(pending_.size() / 2);
+}
+
+void KernelTimer::drain(size_t n) {
+    for (size_t i = 0; i < n; i++) {
+        Pending& p = pending_[i];
+        FTS_HIP_CHECK(hipEventSynchronize(p.b));
+        float ms = 0.f;
+        FTS_HIP_CHECK(hipEventElapsedTime(&ms, p.a, p.b));
+        stats_[p.family].milliseconds += ms;
+        stats_[p.family].bytes += p.bytes;
+        stats_[p.family].launches += 1;
+        pool_.push_back(p.a);
+        pool_.push_back(p.b);
+    }
+    pending_.erase(pending_.begin(), pending_.begin() + n);
+}
+
+fts_kernel_stats KernelTimer::get(int family) {
+    drain(pending_.size());
+    return stats_[family];
+}
+
 namespace {
 
 // ~8 elements per thread, whole waves, at most 1024 threads
@@ -266,44 +326,76 @@ static void require_supported(const Grid& g) {
         throw Error("FP32 propagators need grid sizes whose prime factors are 2, 3 and 5");
 }
 
+// algorithmic bytes of one slab launch: spectra, fields and w each moved once
+template <typename T>
+static double slab_bytes(const Grid& g, int mode, const SlabBatch<T>& b) {
+    const double e = sizeof(T), spec = 2.0 * e * g.Mc, field = e * g.M;
+    double bytes = 0.0;
+    for (int i = 0; i < b.n; i++) {
+        const SlabItem<T>& it = b.it[i];
+        if (it.w) bytes += field;
+        if (it.mask) bytes += field;
+        if (mode == SLAB_FWD) {
+            bytes += field + it.n_out * spec;
+        } else {
+            const int n_spec = mode == SLAB_INV2 ? 2 : 1;
+            bytes += n_spec * spec;
+            if (it.qout) bytes += field;
+            if (it.forward) bytes += n_spec * spec;
+        }
+    }
+    return bytes;
+}
+
 template <typename T>
 void Grid::launch_slab(int mode, const SlabBatch<T>& b) const {
     require_supported<T>(*this);
     if (b.n == 0) return;
+    KernelTimer& timer = ctx->timer;
+    hipEvent_t start = timer.enabled() ? timer.begin(stream()) : nullptr;
     switch (ct_n) {
-        case 32: launch_slab_ct<T, 32>(*this, mode, b); FTS_HIP_CHECK(hipGetLastError()); return;
-        case 48: launch_slab_ct<T, 48>(*this, mode, b); FTS_HIP_CHECK(hipGetLastError()); return;
-        case 64: launch_slab_ct<T, 64>(*this, mode, b); FTS_HIP_CHECK(hipGetLastError()); return;
-        case 96: launch_slab_ct<T, 96>(*this, mode, b); FTS_HIP_CHECK(hipGetLastError()); return;
-        case 128: launch_slab_ct<T, 128>(*this, mode, b); FTS_HIP_CHECK(hipGetLastError()); return;
-        default: break;
+        case 32: launch_slab_ct<T, 32>(*this, mode, b); break;
+        case 48: launch_slab_ct<T, 48>(*this, mode, b); break;
+        case 64: launch_slab_ct<T, 64>(*this, mode, b); break;
+        case 96: launch_slab_ct<T, 96>(*this, mode, b); break;
+        case 128: launch_slab_ct<T, 128>(*this, mode, b); break;
+        default: {
+            const dim3 grid(tnx[0], b.n), block(slab_threads);
+            const size_t lds = slab_lds<T>();
+            if (mode == SLAB_FWD) launch_slab_mode<T, SLAB_FWD>(*this, b, grid, block, lds);
+            else if (mode == SLAB_INV1) launch_slab_mode<T, SLAB_INV1>(*this, b, grid, block, lds);
+            else launch_slab_mode<T, SLAB_INV2>(*this, b, grid, block, lds);
+        }
     }
-    const dim3 grid(tnx[0], b.n), block(slab_threads);
-    const size_t lds = slab_lds<T>();
-    if (mode == SLAB_FWD) launch_slab_mode<T, SLAB_FWD>(*this, b, grid, block, lds);
-    else if (mode == SLAB_INV1) launch_slab_mode<T, SLAB_INV1>(*this, b, grid, block, lds);
-    else launch_slab_mode<T, SLAB_INV2>(*this, b, grid, block, lds);
     FTS_HIP_CHECK(hipGetLastError());
+    if (start) timer.end(stream(), start, FTS_KERNEL_SLAB, slab_bytes(*this, mode, b));
 }
 
 template <typename T>
 void Grid::launch_xpass(const XBatch<T>& b, const T* bx, const T* by, const T* bz) const {
     require_supported<T>(*this);
     if (b.n == 0) return;
+    KernelTimer& timer = ctx->timer;
+    hipEvent_t start = timer.enabled() ? timer.begin(stream()) : nullptr;
     switch (ct_n) {
-        case 32: launch_xpass_ct<T, 32>(*this, b, bx, by, bz); FTS_HIP_CHECK(hipGetLastError()); return;
-        case 48: launch_xpass_ct<T, 48>(*this, b, bx, by, bz); FTS_HIP_CHECK(hipGetLastError()); return;
-        case 64: launch_xpass_ct<T, 64>(*this, b, bx, by, bz); FTS_HIP_CHECK(hipGetLastError()); return;
-        case 96: launch_xpass_ct<T, 96>(*this, b, bx, by, bz); FTS_HIP_CHECK(hipGetLastError()); return;
-        case 128: launch_xpass_ct<T, 128>(*this, b, bx, by, bz); FTS_HIP_CHECK(hipGetLastError()); return;
-        default: break;
+        case 32: launch_xpass_ct<T, 32>(*this, b, bx, by, bz); break;
+        case 48: launch_xpass_ct<T, 48>(*this, b, bx, by, bz); break;
+        case 64: launch_xpass_ct<T, 64>(*this, b, bx, by, bz); break;
+        case 96: launch_xpass_ct<T, 96>(*this, b, bx, by, bz); break;
+        case 128: launch_xpass_ct<T, 128>(*this, b, bx, by, bz); break;
+        default: {
+            const dim3 grid(tnx[1] * xg.n_tiles, b.n), block(x_threads);
+            if (big)
+                hipLaunchKernelGGL((xpass_kernel<T, true>), grid, block, x_lds<T>(), stream(), xg, b, tw<T>(0), bx,
+                                   by, bz);
+            else
+                hipLaunchKernelGGL((xpass_kernel<T, false>), grid, block, x_lds<T>(), stream(), xg, b, tw<T>(0), bx,
+                                   by, bz);
+        }
     }
-    const dim3 grid(tnx[1] * xg.n_tiles, b.n), block(x_threads);
-    if (big)
-        hipLaunchKernelGGL((xpass_kernel<T, true>), grid, block, x_lds<T>(), stream(), xg, b, tw<T>(0), bx, by, bz);
-    else
-        hipLaunchKernelGGL((xpass_kernel<T, false>), grid, block, x_lds<T>(), stream(), xg, b, tw<T>(0), bx, by, bz);
     FTS_HIP_CHECK(hipGetLastError());
+    // each spectrum read and written once
+    if (start) timer.end(stream(), start, FTS_KERNEL_PENCIL, 4.0 * sizeof(T) * Mc * b.n);
 }
 
 template <typename T>

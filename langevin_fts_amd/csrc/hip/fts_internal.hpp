@@ -10,9 +10,40 @@
 #include "fts_fft.hpp"
 #include "fts_fft_ct.hpp"
 
+namespace fts {
+
+// Opt-in timing of the propagator kernels with HIP events on the context's stream.
+// Each timed launch records an event pair and its algorithmic bytes (the bytes the
+// launch must read and write once: its fields, spectra and w, not re-reads).
+class KernelTimer {
+public:
+    ~KernelTimer();
+    void enable(bool on);
+    bool enabled() const { return on_; }
+    hipEvent_t begin(hipStream_t st);
+    void end(hipStream_t st, hipEvent_t start, int family, double bytes);
+    fts_kernel_stats get(int family);
+
+private:
+    struct Pending {
+        hipEvent_t a, b;
+        int family;
+        double bytes;
+    };
+    hipEvent_t take();
+    void drain(size_t n);
+    bool on_ = false;
+    std::vector<Pending> pending_;
+    std::vector<hipEvent_t> pool_;
+    fts_kernel_stats stats_[FTS_KERNEL_FAMILIES] = {};
+};
+
+}  // namespace fts
+
 struct fts_context {
     int device = 0;
     hipStream_t stream = nullptr;
+    fts::KernelTimer timer;
 };
 
 namespace fts {
@@ -145,6 +176,44 @@ private:
     int dots_start_ = 0, dots_n_ = 0;
 };
 
+class SaddlePoint {
+public:
+    SaddlePoint(fts_context* ctx, SolverBase* solver, AndersonMixingDevice* am, int M, const fts_field_theory& ft);
+    void set_fields(const double* w, bool on_device);
+    void get_fields(double* w, bool on_device);
+    void get_concentration(int type, double* out, bool on_device);
+    void set_noise(const double* noise, bool on_device);
+    void get_noise(double* noise, bool on_device);
+    void backup();
+    void restore();
+    void find(double tolerance, int max_iter, int verbose, double* log, int* n_iter, double* error,
+              double* hamiltonian);
+    void langevin_update(const double* noise, bool on_device, double dt);
+    void field_moments(double* mean, double* mean_sq);
+
+private:
+    void concentrations();
+    void run_lincomb(const std::vector<double>& cw, const std::vector<double>& cp, const std::vector<double>& c0,
+                     int n_out, double* out);
+    void row_sums(const std::vector<const double*>& rows, const double* centre, bool square, double* out_dev);
+    double hamiltonian();
+
+    fts_context* ctx_;
+    SolverBase* solver_;
+    AndersonMixingDevice* am_;
+    int M_, S_, R_, I_, P_, n_solver_w_;
+    std::vector<int> real_idx_, imag_idx_;
+    std::vector<double> w_map_;   // n_solver_w x S
+    std::vector<int> term_off_, term_kind_, term_index_;
+    std::vector<double> term_scale_;
+    std::vector<double> lam_cw_, lam_cp_, lam_c0_;   // R x S, R x S, R
+    std::vector<double> h_cw_, h_cp_, h_c0_;         // I x S, I x S, I
+    std::vector<double> dt_scaling_, h_lin_, h_sq_, pol_weight_;
+    double h_const_;
+    DeviceBuffer w_aux_, phi_, w_solver_, h_, lam_, am_w_, am_d_, noise_prev_, noise_cur_, w_backup_, phi_backup_,
+        partials_, stats_;
+};
+
 }  // namespace fts
 
 struct fts_grid {
@@ -155,4 +224,7 @@ struct fts_solver {
 };
 struct fts_am {
     std::unique_ptr<fts::AndersonMixingDevice> a;
+};
+struct fts_saddle {
+    std::unique_ptr<fts::SaddlePoint> s;
 };

@@ -1,0 +1,432 @@
+// Device-resident saddle-point iteration and Leimkuhler-Matthews update of the
+// Langevin field-theoretic simulation (examples/fts/lfts.py:631-868).
+//
+// The auxiliary fields w_aux[S][M], the concentrations phi[S][M], the functional
+// derivatives and both Langevin noise arrays stay in HBM for the whole run. One
+// saddle iteration is: monomer fields as linear combinations of w_aux
+// (lfts.py:163-164, :534-545), propagators + concentrations (the solver), per-type
+// concentration sums, dH/dw of the imaginary fields (lfts.py:219-245), their
+// standard deviations (lfts.py:829-831), Anderson mixing on the imaginary rows
+// (lfts.py:860). Only 2 I doubles (mean and std of each derivative) and the
+// Anderson dot products cross to the host per iteration.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "fts_internal.hpp"
+#include "fts_reduce.hpp"
+
+namespace fts {
+
+namespace {
+
+constexpr int LC_MAX_OUT = 8;
+constexpr int LC_MAX_IN = 2 * FTS_MAX_TYPES;
+constexpr int STAT_BLOCKS = 128;
+constexpr int STAT_THREADS = 256;
+
+// out[r][i] = c0[r] + sum_j c[r][j] * in[j][i]
+struct LinComb {
+    int n_out, n_in;
+    const double* in[LC_MAX_IN];
+    double* out[LC_MAX_OUT];
+    double c[LC_MAX_OUT][LC_MAX_IN];
+    double c0[LC_MAX_OUT];
+};
+
+__global__ void __launch_bounds__(256) lincomb_kernel(LinComb a, int M) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < M; i += gridDim.x * blockDim.x) {
+        double x[LC_MAX_IN];
+#pragma unroll
+        for (int j = 0; j < LC_MAX_IN; j++)
+            if (j < a.n_in) x[j] = a.in[j][i];
+#pragma unroll
+        for (int r = 0; r < LC_MAX_OUT; r++) {
+            if (r >= a.n_out) break;
+            double v = a.c0[r];
+#pragma unroll
+            for (int j = 0; j < LC_MAX_IN; j++)
+                if (j < a.n_in) v += a.c[r][j] * x[j];
+            a.out[r][i] = v;
+        }
+    }
+}
+
+// Row statistics of up to LC_MAX_OUT rows: sum, then sum of squared deviations from
+// the row mean, each with a fixed-order two-stage reduction.
+struct Rows {
+    const double* row[2 * FTS_MAX_TYPES];
+};
+
+__global__ void __launch_bounds__(STAT_THREADS) row_sum_stage1(Rows r, int M, const double* centre, int square,
+                                                              double* partials) {
+    __shared__ double ws[16];
+    const double* x = r.row[blockIdx.y];
+    const double c = centre ? centre[blockIdx.y] / M : 0.0;
+    double acc = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < M; i += gridDim.x * blockDim.x) {
+        const double v = x[i] - c;
+        acc += square ? v * v : v;
+    }
+    acc = block_sum(acc, ws);
+    if (threadIdx.x == 0) partials[blockIdx.y * gridDim.x + blockIdx.x] = acc;
+}
+
+__global__ void __launch_bounds__(STAT_BLOCKS) row_sum_stage2(const double* partials, double* out) {
+    __shared__ double ws[16];
+    double v = partials[blockIdx.x * blockDim.x + threadIdx.x];
+    v = block_sum(v, ws);
+    if (threadIdx.x == 0) out[blockIdx.x] = v;
+}
+
+// w_aux[S-1] -= mean(w_aux[S-1]) with the row sum already on the device
+__global__ void subtract_mean_kernel(double* x, const double* sum, int M) {
+    const double m = sum[0] / M;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < M; i += gridDim.x * blockDim.x) x[i] -= m;
+}
+
+// Anderson input of the imaginary fields: w_im[c] = w_aux[idx[c]], d[c] = -scale[c] * h[c]
+struct Gather {
+    int n;
+    const double* src[FTS_MAX_TYPES];
+    const double* h[FTS_MAX_TYPES];
+    double* w_out[FTS_MAX_TYPES];
+    double* d_out[FTS_MAX_TYPES];
+    double scale[FTS_MAX_TYPES];
+};
+
+__global__ void gather_kernel(Gather g, int M) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < M; i += gridDim.x * blockDim.x)
+        for (int c = 0; c < g.n; c++) {
+            if (g.w_out[c]) g.w_out[c][i] = g.src[c][i];
+            g.d_out[c][i] = -g.scale[c] * g.h[c][i];
+        }
+}
+
+struct Scatter {
+    int n;
+    const double* src[FTS_MAX_TYPES];
+    double* dst[FTS_MAX_TYPES];
+};
+
+__global__ void scatter_kernel(Scatter s, int M) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < M; i += gridDim.x * blockDim.x)
+        for (int c = 0; c < s.n; c++) s.dst[c][i] = s.src[c][i];
+}
+
+// Leimkuhler-Matthews step of the real fields (lfts.py:705-709):
+// w[c] += -lambda[c] * dt * s_c + 0.5 * (prev[c] + cur[c]) * sqrt(s_c)
+struct LmArgs {
+    int n;
+    double* w[FTS_MAX_TYPES];
+    const double* lambda[FTS_MAX_TYPES];
+    const double* prev[FTS_MAX_TYPES];
+    const double* cur[FTS_MAX_TYPES];
+    double dt_s[FTS_MAX_TYPES];
+    double sqrt_s[FTS_MAX_TYPES];
+};
+
+__global__ void lm_kernel(LmArgs a, int M) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < M; i += gridDim.x * blockDim.x)
+        for (int c = 0; c < a.n; c++)
+            a.w[c][i] += -a.lambda[c][i] * a.dt_s[c] + 0.5 * (a.prev[c][i] + a.cur[c][i]) * a.sqrt_s[c];
+}
+
+int grid_blocks(int M) { return std::min(2048, ceil_div(M, 256)); }
+
+}  // namespace
+
+SaddlePoint::SaddlePoint(fts_context* ctx, SolverBase* solver, AndersonMixingDevice* am, int M,
+                         const fts_field_theory& ft)
+    : ctx_(ctx), solver_(solver), am_(am), M_(M) {
+    S_ = ft.n_types;
+    R_ = ft.n_real;
+    I_ = ft.n_imag;
+    P_ = ft.n_polymers;
+    n_solver_w_ = ft.n_solver_fields;
+    FTS_REQUIRE(M > 0, "n_grid must be positive");
+    FTS_REQUIRE(S_ >= 1 && S_ <= FTS_MAX_TYPES, "n_types must be in [1, 8]");
+    FTS_REQUIRE(R_ >= 0 && I_ >= 1 && R_ + I_ <= S_, "invalid numbers of real and imaginary fields");
+    FTS_REQUIRE(n_solver_w_ >= 1 && n_solver_w_ <= LC_MAX_OUT, "n_solver_fields must be in [1, 8]");
+    FTS_REQUIRE(P_ >= 1, "n_polymers must be positive");
+    for (int c = 0; c < R_; c++) {
+        FTS_REQUIRE(ft.real_idx[c] >= 0 && ft.real_idx[c] < S_, "real_idx out of range");
+        real_idx_.push_back(ft.real_idx[c]);
+    }
+    for (int c = 0; c < I_; c++) {
+        FTS_REQUIRE(ft.imag_idx[c] >= 0 && ft.imag_idx[c] < S_, "imag_idx out of range");
+        imag_idx_.push_back(ft.imag_idx[c]);
+    }
+    w_map_.assign(ft.w_map, ft.w_map + (size_t)n_solver_w_ * S_);
+    term_off_.assign(ft.phi_term_off, ft.phi_term_off + S_ + 1);
+    const int nt = term_off_[S_];
+    FTS_REQUIRE(term_off_[0] == 0 && nt >= 0, "invalid phi_term_off");
+    for (int i = 0; i < S_; i++) FTS_REQUIRE(term_off_[i + 1] >= term_off_[i], "phi_term_off must be ascending");
+    term_kind_.assign(ft.phi_term_kind, ft.phi_term_kind + nt);
+    term_index_.assign(ft.phi_term_index, ft.phi_term_index + nt);
+    term_scale_.assign(ft.phi_term_scale, ft.phi_term_scale + nt);
+    lam_cw_.assign(ft.real_cw, ft.real_cw + (size_t)R_ * S_);
+    lam_cp_.assign(ft.real_cp, ft.real_cp + (size_t)R_ * S_);
+    lam_c0_.assign(ft.real_c0, ft.real_c0 + R_);
+    h_cw_.assign(ft.imag_cw, ft.imag_cw + (size_t)I_ * S_);
+    h_cp_.assign(ft.imag_cp, ft.imag_cp + (size_t)I_ * S_);
+    h_c0_.assign(ft.imag_c0, ft.imag_c0 + I_);
+    dt_scaling_.assign(ft.dt_scaling, ft.dt_scaling + S_);
+    h_lin_.assign(ft.h_lin, ft.h_lin + S_);
+    h_sq_.assign(ft.h_sq, ft.h_sq + S_);
+    pol_weight_.assign(ft.polymer_weight, ft.polymer_weight + P_);
+    h_const_ = ft.h_const;
+    for (int i : real_idx_) FTS_REQUIRE(dt_scaling_[i] > 0.0, "dt_scaling of the real fields must be positive");
+    for (int i : imag_idx_) FTS_REQUIRE(dt_scaling_[i] > 0.0, "dt_scaling of the imaginary fields must be positive");
+
+    const size_t f = sizeof(double) * M;
+    w_aux_.allocate(f * S_);
+    phi_.allocate(f * S_);
+    w_solver_.allocate(f * n_solver_w_);
+    h_.allocate(f * I_);
+    lam_.allocate(f * std::max(R_, 1));
+    am_w_.allocate(f * I_);
+    am_d_.allocate(f * I_);
+    noise_prev_.allocate(f * std::max(R_, 1));
+    noise_cur_.allocate(f * std::max(R_, 1));
+    w_backup_.allocate(f * S_);
+    phi_backup_.allocate(f * S_);
+    partials_.allocate(sizeof(double) * 2 * FTS_MAX_TYPES * STAT_BLOCKS);
+    stats_.allocate(sizeof(double) * 4 * FTS_MAX_TYPES);
+    FTS_HIP_CHECK(hipMemsetAsync(w_aux_.get(), 0, f * S_, ctx_->stream));
+    FTS_HIP_CHECK(hipMemsetAsync(phi_.get(), 0, f * S_, ctx_->stream));
+    FTS_HIP_CHECK(hipMemsetAsync(noise_prev_.get(), 0, f * std::max(R_, 1), ctx_->stream));
+    FTS_HIP_CHECK(hipStreamSynchronize(ctx_->stream));
+}
+
+void SaddlePoint::set_fields(const double* w, bool on_device) {
+    FTS_HIP_CHECK(hipMemcpyAsync(w_aux_.get(), w, sizeof(double) * M_ * S_,
+                                 on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, ctx_->stream));
+    FTS_HIP_CHECK(hipStreamSynchronize(ctx_->stream));
+}
+
+void SaddlePoint::get_fields(double* w, bool on_device) {
+    FTS_HIP_CHECK(hipMemcpyAsync(w, w_aux_.get(), sizeof(double) * M_ * S_,
+                                 on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, ctx_->stream));
+    FTS_HIP_CHECK(hipStreamSynchronize(ctx_->stream));
+}
+
+void SaddlePoint::get_concentration(int type, double* out, bool on_device) {
+    FTS_REQUIRE(type >= 0 && type < S_, "type index out of range");
+    FTS_HIP_CHECK(hipMemcpyAsync(out, phi_.as<double>() + (size_t)type * M_, sizeof(double) * M_,
+                                 on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, ctx_->stream));
+    FTS_HIP_CHECK(hipStreamSynchronize(ctx_->stream));
+}
+
+void SaddlePoint::set_noise(const double* noise, bool on_device) {
+    if (R_ == 0) return;
+    FTS_HIP_CHECK(hipMemcpyAsync(noise_prev_.get(), noise, sizeof(double) * M_ * R_,
+                                 on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, ctx_->stream));
+    FTS_HIP_CHECK(hipStreamSynchronize(ctx_->stream));
+}
+
+void SaddlePoint::get_noise(double* noise, bool on_device) {
+    if (R_ == 0) return;
+    FTS_HIP_CHECK(hipMemcpyAsync(noise, noise_prev_.get(), sizeof(double) * M_ * R_,
+                                 on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, ctx_->stream));
+    FTS_HIP_CHECK(hipStreamSynchronize(ctx_->stream));
+}
+
+void SaddlePoint::backup() {
+    const size_t b = sizeof(double) * M_ * S_;
+    FTS_HIP_CHECK(hipMemcpyAsync(w_backup_.get(), w_aux_.get(), b, hipMemcpyDeviceToDevice, ctx_->stream));
+    FTS_HIP_CHECK(hipMemcpyAsync(phi_backup_.get(), phi_.get(), b, hipMemcpyDeviceToDevice, ctx_->stream));
+}
+
+void SaddlePoint::restore() {
+    const size_t b = sizeof(double) * M_ * S_;
+    FTS_HIP_CHECK(hipMemcpyAsync(w_aux_.get(), w_backup_.get(), b, hipMemcpyDeviceToDevice, ctx_->stream));
+    FTS_HIP_CHECK(hipMemcpyAsync(phi_.get(), phi_backup_.get(), b, hipMemcpyDeviceToDevice, ctx_->stream));
+    FTS_HIP_CHECK(hipStreamSynchronize(ctx_->stream));
+}
+
+void SaddlePoint::run_lincomb(const std::vector<double>& cw, const std::vector<double>& cp,
+                              const std::vector<double>& c0, int n_out, double* out) {
+    // inputs: w_aux rows 0..S-1, then phi rows 0..S-1 (cp empty: w_aux only)
+    LinComb a{};
+    a.n_out = n_out;
+    a.n_in = cp.empty() ? S_ : 2 * S_;
+    for (int j = 0; j < S_; j++) {
+        a.in[j] = w_aux_.as<double>() + (size_t)j * M_;
+        if (!cp.empty()) a.in[S_ + j] = phi_.as<double>() + (size_t)j * M_;
+    }
+    for (int r = 0; r < n_out; r++) {
+        a.out[r] = out + (size_t)r * M_;
+        a.c0[r] = c0.empty() ? 0.0 : c0[r];
+        for (int j = 0; j < S_; j++) {
+            a.c[r][j] = cw[(size_t)r * S_ + j];
+            if (!cp.empty()) a.c[r][S_ + j] = cp[(size_t)r * S_ + j];
+        }
+    }
+    hipLaunchKernelGGL(lincomb_kernel, dim3(grid_blocks(M_)), dim3(256), 0, ctx_->stream, a, M_);
+    FTS_HIP_CHECK(hipGetLastError());
+}
+
+void SaddlePoint::row_sums(const std::vector<const double*>& rows, const double* centre, bool square,
+                           double* out_dev) {
+    Rows r{};
+    const int n = (int)rows.size();
+    FTS_REQUIRE(n <= 2 * FTS_MAX_TYPES, "too many rows");
+    for (int i = 0; i < n; i++) r.row[i] = rows[i];
+    hipLaunchKernelGGL(row_sum_stage1, dim3(STAT_BLOCKS, n), dim3(STAT_THREADS), 0, ctx_->stream, r, M_, centre,
+                       square ? 1 : 0, partials_.as<double>());
+    hipLaunchKernelGGL(row_sum_stage2, dim3(n), dim3(STAT_BLOCKS), 0, ctx_->stream, partials_.as<double>(),
+                       out_dev);
+    FTS_HIP_CHECK(hipGetLastError());
+}
+
+void SaddlePoint::concentrations() {
+    // monomer fields of the solver (lfts.py:534-545)
+    run_lincomb(w_map_, {}, {}, n_solver_w_, w_solver_.as<double>());
+    std::vector<const double*> w(n_solver_w_);
+    for (int m = 0; m < n_solver_w_; m++) w[m] = w_solver_.as<double>() + (size_t)m * M_;
+    solver_->compute_propagators(w.data(), nullptr, true);
+    solver_->compute_concentrations();
+    for (int i = 0; i < S_; i++) {
+        const int o = term_off_[i], n = term_off_[i + 1] - o;
+        solver_->sum_concentrations(n, term_kind_.data() + o, term_index_.data() + o, term_scale_.data() + o,
+                                    phi_.as<double>() + (size_t)i * M_, true);
+    }
+}
+
+void SaddlePoint::field_moments(double* mean, double* mean_sq) {
+    std::vector<const double*> rows;
+    for (int j = 0; j < S_; j++) rows.push_back(w_aux_.as<double>() + (size_t)j * M_);
+    double* st = stats_.as<double>();
+    row_sums(rows, nullptr, false, st);
+    row_sums(rows, nullptr, true, st + S_);
+    std::vector<double> h(2 * S_);
+    FTS_HIP_CHECK(hipMemcpyAsync(h.data(), st, sizeof(double) * 2 * S_, hipMemcpyDeviceToHost, ctx_->stream));
+    FTS_HIP_CHECK(hipStreamSynchronize(ctx_->stream));
+    for (int j = 0; j < S_; j++) {
+        mean[j] = h[j] / M_;
+        mean_sq[j] = h[S_ + j] / M_;
+    }
+}
+
+// lfts.py:197-216: H = sum_j (h_sq_j <w_j^2> + h_lin_j <w_j>) - sum_p phi_p/alpha_p ln Q_p + h_const
+double SaddlePoint::hamiltonian() {
+    std::vector<double> mean(S_), mean_sq(S_);
+    field_moments(mean.data(), mean_sq.data());
+    double h = 0.0;
+    for (int j = 0; j < S_; j++) h += h_lin_[j] * mean[j];
+    for (int j = 0; j < S_; j++) h += h_sq_[j] * mean_sq[j];
+    double hp = 0.0;
+    for (int p = 0; p < P_; p++) hp -= pol_weight_[p] * std::log(solver_->partition(p));
+    return hp + h + h_const_;
+}
+
+void SaddlePoint::find(double tolerance, int max_iter, int verbose, double* log, int* n_iter, double* error,
+                       double* hamiltonian_out) {
+    FTS_REQUIRE(max_iter >= 1, "max_iter must be positive");
+    hipStream_t st = ctx_->stream;
+    double err = 1e20;
+    double ham = std::nan("");
+    int it = 1;
+    am_->reset();
+    const int row = 3 + P_ + I_;
+    std::vector<double> stats(2 * I_);
+    for (; it <= max_iter; it++) {
+        concentrations();
+        // dH/dw of the imaginary fields, sign already flipped (lfts.py:219-245)
+        run_lincomb(h_cw_, h_cp_, h_c0_, I_, h_.as<double>());
+        std::vector<const double*> rows;
+        for (int c = 0; c < I_; c++) rows.push_back(h_.as<double>() + (size_t)c * M_);
+        double* sd = stats_.as<double>();
+        row_sums(rows, nullptr, false, sd);
+        row_sums(rows, sd, true, sd + I_);
+        FTS_HIP_CHECK(hipMemcpyAsync(stats.data(), sd, sizeof(double) * 2 * I_, hipMemcpyDeviceToHost, st));
+        FTS_HIP_CHECK(hipStreamSynchronize(st));
+        const double old_err = err;
+        err = 0.0;
+        std::vector<double> err_i(I_);
+        for (int c = 0; c < I_; c++) {
+            err_i[c] = std::sqrt(stats[I_ + c] / M_);
+            if (c == 0 || err_i[c] > err || std::isnan(err_i[c])) err = err_i[c];
+        }
+        // the Hamiltonian and the printed row of lfts.py:834-849, at every iteration
+        // (verbose 2) or at the final one
+        const bool last = err < tolerance || it == max_iter;
+        if (verbose == 2 || last) {
+            ham = hamiltonian();
+            if (log && verbose >= 1 && (verbose == 2 || last)) {
+                double* r = log + (size_t)(verbose == 2 ? it - 1 : 0) * row;
+                r[0] = it;
+                r[1] = stats[I_ - 1] / M_;   // mass error: mean of the pressure-field derivative
+                r[2] = ham;
+                for (int p = 0; p < P_; p++) r[3 + p] = solver_->partition(p);
+                for (int c = 0; c < I_; c++) r[3 + P_ + c] = err_i[c];
+            }
+        }
+        if (err < tolerance) break;
+        // Anderson mixing of the imaginary fields with d = -dt_scaling * dH/dw (lfts.py:854-860)
+        Gather g{};
+        g.n = I_;
+        bool contiguous = true;
+        for (int c = 1; c < I_; c++) contiguous = contiguous && imag_idx_[c] == imag_idx_[0] + c;
+        double* w_im = contiguous ? w_aux_.as<double>() + (size_t)imag_idx_[0] * M_ : am_w_.as<double>();
+        for (int c = 0; c < I_; c++) {
+            g.src[c] = w_aux_.as<double>() + (size_t)imag_idx_[c] * M_;
+            g.h[c] = h_.as<double>() + (size_t)c * M_;
+            g.w_out[c] = contiguous ? nullptr : am_w_.as<double>() + (size_t)c * M_;
+            g.d_out[c] = am_d_.as<double>() + (size_t)c * M_;
+            g.scale[c] = dt_scaling_[imag_idx_[c]];
+        }
+        hipLaunchKernelGGL(gather_kernel, dim3(grid_blocks(M_)), dim3(256), 0, st, g, M_);
+        FTS_HIP_CHECK(hipGetLastError());
+        am_->calculate_new_fields(w_im, w_im, am_d_.as<double>(), old_err, err, true);
+        if (!contiguous) {
+            Scatter s{};
+            s.n = I_;
+            for (int c = 0; c < I_; c++) {
+                s.src[c] = am_w_.as<double>() + (size_t)c * M_;
+                s.dst[c] = w_aux_.as<double>() + (size_t)imag_idx_[c] * M_;
+            }
+            hipLaunchKernelGGL(scatter_kernel, dim3(grid_blocks(M_)), dim3(256), 0, st, s, M_);
+            FTS_HIP_CHECK(hipGetLastError());
+        }
+    }
+    // zero mean of the pressure field (lfts.py:865)
+    double* last_row = w_aux_.as<double>() + (size_t)(S_ - 1) * M_;
+    row_sums({last_row}, nullptr, false, stats_.as<double>());
+    hipLaunchKernelGGL(subtract_mean_kernel, dim3(grid_blocks(M_)), dim3(256), 0, st, last_row,
+                       stats_.as<const double>(), M_);
+    FTS_HIP_CHECK(hipGetLastError());
+    FTS_HIP_CHECK(hipStreamSynchronize(st));
+    if (n_iter) *n_iter = std::min(it, max_iter);
+    if (error) *error = err;
+    if (hamiltonian_out) *hamiltonian_out = ham;
+}
+
+void SaddlePoint::langevin_update(const double* noise, bool on_device, double dt) {
+    if (R_ == 0) return;
+    hipStream_t st = ctx_->stream;
+    FTS_HIP_CHECK(hipMemcpyAsync(noise_cur_.get(), noise, sizeof(double) * M_ * R_,
+                                 on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st));
+    // dH/dw of the real fields (no sign flip) from the current w_aux and phi
+    run_lincomb(lam_cw_, lam_cp_, lam_c0_, R_, lam_.as<double>());
+    LmArgs a{};
+    a.n = R_;
+    for (int c = 0; c < R_; c++) {
+        const double s = dt_scaling_[real_idx_[c]];
+        a.w[c] = w_aux_.as<double>() + (size_t)real_idx_[c] * M_;
+        a.lambda[c] = lam_.as<double>() + (size_t)c * M_;
+        a.prev[c] = noise_prev_.as<double>() + (size_t)c * M_;
+        a.cur[c] = noise_cur_.as<double>() + (size_t)c * M_;
+        a.dt_s[c] = dt * s;
+        a.sqrt_s[c] = std::sqrt(s);
+    }
+    hipLaunchKernelGGL(lm_kernel, dim3(grid_blocks(M_)), dim3(256), 0, st, a, M_);
+    FTS_HIP_CHECK(hipGetLastError());
+    // the current noise becomes the previous one (lfts.py:712)
+    std::swap(noise_prev_, noise_cur_);
+    FTS_HIP_CHECK(hipStreamSynchronize(st));
+}
+
+}  // namespace fts
+

@@ -384,6 +384,25 @@ void PropagatorComputation::sum_blocks(int polymer, const std::string& monomer, 
                                             on_device ? 1 : 0));
 }
 
+void PropagatorComputation::concentration_terms(const std::string& monomer, double scale, std::vector<int>& kinds,
+                                                std::vector<int>& idx, std::vector<double>& sc) const {
+    auto& blocks = analyzer_->get_computation_blocks();
+    for (size_t b = 0; b < block_keys_.size(); b++) {
+        const BlockKey& k = block_keys_[b];
+        if (propagator_code::monomer_type_from_key(std::get<1>(k)) == monomer && blocks.at(k).n_segment_right != 0) {
+            kinds.push_back(FTS_SRC_BLOCK);
+            idx.push_back((int)b);
+            sc.push_back(scale);
+        }
+    }
+    for (int s = 0; s < molecules_->get_n_solvent_types(); s++)
+        if (std::get<1>(molecules_->get_solvent(s)) == monomer) {
+            kinds.push_back(FTS_SRC_SOLVENT);
+            idx.push_back(s);
+            sc.push_back(scale);
+        }
+}
+
 void PropagatorComputation::get_total_concentration(const std::string& monomer_type, double* phi, bool on_device) {
     sum_blocks(-1, monomer_type, true, 1.0, phi, on_device);
 }
@@ -536,6 +555,124 @@ void AndersonMixing::calculate_new_fields(double* w_new, const double* w_current
                                           double old_error_level, double error_level, bool on_device) {
     check_fts(fts_am_calculate_new_fields(am_, w_new, w_current, w_deriv, old_error_level, error_level,
                                           on_device ? 1 : 0));
+}
+
+// ------------------------------------------------------------------ L-FTS saddle point
+DeviceSaddlePoint::DeviceSaddlePoint(PropagatorComputation* solver, AndersonMixing* am, const FieldTheoryTables& t) {
+    if (!solver || !am) LFTS_THROW("solver and am are required");
+    S_ = (int)t.monomer_types.size();
+    R_ = (int)t.real_idx.size();
+    I_ = (int)t.imag_idx.size();
+    M_ = solver->get_n_grid();
+    Molecules* mol = solver->molecules();
+    P_ = mol->get_n_polymer_types();
+    auto need = [&](const std::vector<double>& v, size_t n, const char* what) {
+        if (v.size() != n) LFTS_THROW(std::string(what) + " has size " + std::to_string(v.size()) + ", expected " +
+                                      std::to_string(n));
+    };
+    if (S_ < 1 || S_ > FTS_MAX_TYPES) LFTS_THROW("the number of monomer types must be in [1, 8]");
+    need(t.matrix_a, (size_t)S_ * S_, "matrix_a");
+    need(t.real_cw, (size_t)R_ * S_, "real_cw");
+    need(t.real_cp, (size_t)R_ * S_, "real_cp");
+    need(t.real_c0, (size_t)R_, "real_c0");
+    need(t.imag_cw, (size_t)I_ * S_, "imag_cw");
+    need(t.imag_cp, (size_t)I_ * S_, "imag_cp");
+    need(t.imag_c0, (size_t)I_, "imag_c0");
+    need(t.dt_scaling, (size_t)S_, "dt_scaling");
+    need(t.h_lin, (size_t)S_, "h_lin");
+    need(t.h_sq, (size_t)S_, "h_sq");
+    if (I_ * M_ != am->get_n_var())
+        LFTS_THROW("Anderson mixing has n_var " + std::to_string(am->get_n_var()) + ", expected " +
+                   std::to_string(I_ * M_));
+    std::map<std::string, int> type_index;
+    for (int i = 0; i < S_; i++) type_index[t.monomer_types[i]] = i;
+    // solver monomer fields as rows over w_aux: w = A w_aux for the S types, random
+    // copolymers mix the rows of their constituent types (lfts.py:534-545)
+    const auto& sm = solver->monomer_types();
+    std::vector<double> w_map(sm.size() * S_, 0.0);
+    for (size_t m = 0; m < sm.size(); m++) {
+        auto add_row = [&](int type, double f) {
+            for (int j = 0; j < S_; j++) w_map[m * S_ + j] += f * t.matrix_a[(size_t)type * S_ + j];
+        };
+        auto ti = type_index.find(sm[m]);
+        auto ri = t.random_fraction.find(sm[m]);
+        if (ti != type_index.end()) {
+            add_row(ti->second, 1.0);
+        } else if (ri != t.random_fraction.end()) {
+            for (const auto& f : ri->second) {
+                auto fi = type_index.find(f.first);
+                if (fi == type_index.end()) LFTS_THROW("random copolymer type \"" + f.first + "\" is unknown");
+                add_row(fi->second, f.second);
+            }
+        } else {
+            LFTS_THROW("monomer type \"" + sm[m] + "\" is neither a field type nor a random copolymer");
+        }
+    }
+    // concentration of each type: its own blocks and solvents, plus the random
+    // copolymer concentrations times their fractions (lfts.py:550-565)
+    std::vector<int> off{0}, kinds, idx;
+    std::vector<double> sc;
+    for (int i = 0; i < S_; i++) {
+        solver->concentration_terms(t.monomer_types[i], 1.0, kinds, idx, sc);
+        for (const auto& r : t.random_fraction) {
+            auto f = r.second.find(t.monomer_types[i]);
+            if (f != r.second.end()) solver->concentration_terms(r.first, f->second, kinds, idx, sc);
+        }
+        off.push_back((int)kinds.size());
+    }
+    std::vector<double> weight(P_);
+    for (int p = 0; p < P_; p++)
+        weight[p] = mol->get_polymer(p).get_volume_fraction() / mol->get_polymer(p).get_alpha();
+
+    fts_field_theory ft{};
+    ft.n_types = S_;
+    ft.n_real = R_;
+    ft.n_imag = I_;
+    ft.real_idx = t.real_idx.data();
+    ft.imag_idx = t.imag_idx.data();
+    ft.n_solver_fields = (int)sm.size();
+    ft.w_map = w_map.data();
+    ft.phi_term_off = off.data();
+    ft.phi_term_kind = kinds.data();
+    ft.phi_term_index = idx.data();
+    ft.phi_term_scale = sc.data();
+    ft.real_cw = t.real_cw.data();
+    ft.real_cp = t.real_cp.data();
+    ft.real_c0 = t.real_c0.data();
+    ft.imag_cw = t.imag_cw.data();
+    ft.imag_cp = t.imag_cp.data();
+    ft.imag_c0 = t.imag_c0.data();
+    ft.dt_scaling = t.dt_scaling.data();
+    ft.h_lin = t.h_lin.data();
+    ft.h_sq = t.h_sq.data();
+    ft.h_const = t.h_const;
+    ft.n_polymers = P_;
+    ft.polymer_weight = weight.data();
+    check_fts(fts_saddle_create(hip_context(), solver->handle(), am->handle(), M_, &ft, &saddle_));
+}
+
+DeviceSaddlePoint::~DeviceSaddlePoint() {
+    if (saddle_) fts_saddle_destroy(saddle_);
+}
+
+void DeviceSaddlePoint::set_fields(const double* w_aux) { check_fts(fts_saddle_set_fields(saddle_, w_aux, 0)); }
+void DeviceSaddlePoint::get_fields(double* w_aux) { check_fts(fts_saddle_get_fields(saddle_, w_aux, 0)); }
+void DeviceSaddlePoint::get_concentration(int type, double* phi) {
+    check_fts(fts_saddle_get_concentration(saddle_, type, phi, 0));
+}
+void DeviceSaddlePoint::set_noise_prev(const double* noise) { check_fts(fts_saddle_set_noise(saddle_, noise, 0)); }
+void DeviceSaddlePoint::get_noise_prev(double* noise) { check_fts(fts_saddle_get_noise(saddle_, noise, 0)); }
+void DeviceSaddlePoint::backup() { check_fts(fts_saddle_backup(saddle_)); }
+void DeviceSaddlePoint::restore() { check_fts(fts_saddle_restore(saddle_)); }
+void DeviceSaddlePoint::find(double tolerance, int max_iter, int verbose, double* log, int* n_iter, double* error,
+                             double* ham) {
+    check_fts(fts_saddle_find(saddle_, tolerance, max_iter, verbose, log, n_iter, error, ham));
+}
+void DeviceSaddlePoint::langevin_update(const double* noise, double dt) {
+    check_fts(fts_saddle_langevin_update(saddle_, noise, 0, dt));
+}
+void DeviceSaddlePoint::field_moments(double* mean, double* mean_sq) {
+    check_fts(fts_saddle_field_moments(saddle_, mean, mean_sq));
 }
 
 // ------------------------------------------------------------------ factory

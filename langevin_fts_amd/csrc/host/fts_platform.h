@@ -86,6 +86,12 @@ public:
     bool check_total_partition();
 
     const std::vector<std::string>& monomer_types() const { return monomers_; }
+    // blocks (and solvents) whose concentrations make up monomer type `monomer`,
+    // each term scaled by `scale` (get_total_concentration without a polymer index)
+    void concentration_terms(const std::string& monomer, double scale, std::vector<int>& kinds,
+                             std::vector<int>& idx, std::vector<double>& sc) const;
+    fts_solver* handle() const { return solver_; }
+    Molecules* molecules() const { return molecules_; }
     const std::vector<std::vector<Scheduler::Job>>& schedule() const { return schedule_; }
     int precision() const { return precision_; }
     int n_streams() const { return n_streams_; }
@@ -119,9 +125,52 @@ public:
     void calculate_new_fields(double* w_new, const double* w_current, const double* w_deriv, double old_error_level,
                               double error_level, bool on_device = false);
 
+    fts_am* handle() const { return am_; }
+
 private:
     int n_var_;
     fts_am* am_ = nullptr;
+};
+
+// Device-resident saddle-point iteration and Langevin update for the L-FTS driver
+// (examples/fts/lfts.py:631-868). The driver hands over the field theory's small
+// matrices; the fields stay on the device between calls.
+struct FieldTheoryTables {
+    std::vector<std::string> monomer_types;                  // S sorted types (lfts.py:280)
+    std::map<std::string, std::map<std::string, double>> random_fraction;   // lfts.py:365-397
+    std::vector<double> matrix_a;                            // S x S, w = A w_aux
+    std::vector<int> real_idx, imag_idx;
+    std::vector<double> real_cw, real_cp, real_c0, imag_cw, imag_cp, imag_c0;
+    std::vector<double> dt_scaling, h_lin, h_sq;
+    double h_const = 0.0;
+};
+
+class DeviceSaddlePoint {
+public:
+    DeviceSaddlePoint(PropagatorComputation* solver, AndersonMixing* am, const FieldTheoryTables& t);
+    ~DeviceSaddlePoint();
+    DeviceSaddlePoint(const DeviceSaddlePoint&) = delete;
+    DeviceSaddlePoint& operator=(const DeviceSaddlePoint&) = delete;
+    int n_types() const { return S_; }
+    int n_real() const { return R_; }
+    int n_imag() const { return I_; }
+    int n_polymers() const { return P_; }
+    int n_grid() const { return M_; }
+    void set_fields(const double* w_aux);
+    void get_fields(double* w_aux);
+    void get_concentration(int type, double* phi);
+    void set_noise_prev(const double* noise);
+    void get_noise_prev(double* noise);
+    void backup();
+    void restore();
+    // returns (iterations, error level, hamiltonian); log rows as fts_saddle_find
+    void find(double tolerance, int max_iter, int verbose, double* log, int* n_iter, double* error, double* ham);
+    void langevin_update(const double* noise, double dt);
+    void field_moments(double* mean, double* mean_sq);
+
+private:
+    int S_, R_, I_, P_, M_;
+    fts_saddle* saddle_ = nullptr;
 };
 
 class AbstractFactory {

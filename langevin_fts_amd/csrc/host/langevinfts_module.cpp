@@ -250,6 +250,101 @@ PYBIND11_MODULE(langevinfts, m) {
                                           reinterpret_cast<const double*>(w_deriv), old_err, err, true);
              });
 
+    // Device-resident saddle point of the L-FTS driver (examples/fts/lfts.py:631-868);
+    // an addition to the reference's surface, used by langevin_fts_amd.lfts.
+    py::class_<DeviceSaddlePoint>(m, "DeviceSaddlePoint")
+        .def(py::init([](PropagatorComputation* solver, AndersonMixing* am, std::vector<std::string> monomer_types,
+                         std::map<std::string, std::map<std::string, double>> random_fraction,
+                         std::vector<double> matrix_a, std::vector<int> real_idx, std::vector<int> imag_idx,
+                         std::vector<double> real_cw, std::vector<double> real_cp, std::vector<double> real_c0,
+                         std::vector<double> imag_cw, std::vector<double> imag_cp, std::vector<double> imag_c0,
+                         std::vector<double> dt_scaling, std::vector<double> h_lin, std::vector<double> h_sq,
+                         double h_const) {
+                 FieldTheoryTables t;
+                 t.monomer_types = std::move(monomer_types);
+                 t.random_fraction = std::move(random_fraction);
+                 t.matrix_a = std::move(matrix_a);
+                 t.real_idx = std::move(real_idx);
+                 t.imag_idx = std::move(imag_idx);
+                 t.real_cw = std::move(real_cw);
+                 t.real_cp = std::move(real_cp);
+                 t.real_c0 = std::move(real_c0);
+                 t.imag_cw = std::move(imag_cw);
+                 t.imag_cp = std::move(imag_cp);
+                 t.imag_c0 = std::move(imag_c0);
+                 t.dt_scaling = std::move(dt_scaling);
+                 t.h_lin = std::move(h_lin);
+                 t.h_sq = std::move(h_sq);
+                 t.h_const = h_const;
+                 return new DeviceSaddlePoint(solver, am, t);
+             }),
+             py::keep_alive<1, 2>(), py::keep_alive<1, 3>())
+        .def("get_n_types", &DeviceSaddlePoint::n_types)
+        .def("get_n_real", &DeviceSaddlePoint::n_real)
+        .def("get_n_imag", &DeviceSaddlePoint::n_imag)
+        .def("set_fields",
+             [](DeviceSaddlePoint& o, const darr& w) {
+                 const ssize_t n = (ssize_t)o.n_types() * o.n_grid();
+                 if (w.size() != n)
+                     LFTS_THROW("Size of w_aux (" + std::to_string(w.size()) + ") must be " + std::to_string(n));
+                 o.set_fields(w.data());
+             })
+        .def("get_fields",
+             [](DeviceSaddlePoint& o) {
+                 py::array_t<double> w({(ssize_t)o.n_types(), (ssize_t)o.n_grid()});
+                 o.get_fields(w.mutable_data());
+                 return w;
+             })
+        .def("get_concentration",
+             [](DeviceSaddlePoint& o, int type) {
+                 if (type < 0 || type >= o.n_types()) LFTS_THROW("type index out of range");
+                 py::array_t<double> phi(o.n_grid());
+                 o.get_concentration(type, phi.mutable_data());
+                 return phi;
+             })
+        .def("set_noise_prev",
+             [](DeviceSaddlePoint& o, const darr& noise) {
+                 const ssize_t n = (ssize_t)o.n_real() * o.n_grid();
+                 if (noise.size() != n)
+                     LFTS_THROW("Size of normal_noise_prev (" + std::to_string(noise.size()) + ") must be " +
+                                std::to_string(n));
+                 if (n > 0) o.set_noise_prev(noise.data());
+             })
+        .def("get_noise_prev",
+             [](DeviceSaddlePoint& o) {
+                 py::array_t<double> w({(ssize_t)o.n_real(), (ssize_t)o.n_grid()});
+                 if (o.n_real() > 0) o.get_noise_prev(w.mutable_data());
+                 return w;
+             })
+        .def("backup", &DeviceSaddlePoint::backup)
+        .def("restore", &DeviceSaddlePoint::restore)
+        .def(
+            "find_saddle_point",
+            [](DeviceSaddlePoint& o, double tolerance, int max_iter, int verbose) {
+                const int row = 3 + o.n_polymers() + o.n_imag();
+                const int rows = verbose == 2 ? std::max(max_iter, 1) : 1;
+                py::array_t<double> log({(ssize_t)rows, (ssize_t)row});
+                std::fill(log.mutable_data(), log.mutable_data() + (size_t)rows * row, 0.0);
+                int n_iter = 0;
+                double err = 0.0, ham = 0.0;
+                o.find(tolerance, max_iter, verbose, log.mutable_data(), &n_iter, &err, &ham);
+                return py::make_tuple(n_iter, err, ham, log);
+            },
+            py::arg("tolerance"), py::arg("max_iter"), py::arg("verbose") = 1)
+        .def("langevin_update",
+             [](DeviceSaddlePoint& o, const darr& noise, double dt) {
+                 const ssize_t n = (ssize_t)o.n_real() * o.n_grid();
+                 if (noise.size() != n)
+                     LFTS_THROW("Size of the noise (" + std::to_string(noise.size()) + ") must be " +
+                                std::to_string(n));
+                 if (n > 0) o.langevin_update(noise.data(), dt);
+             })
+        .def("field_moments", [](DeviceSaddlePoint& o) {
+            std::vector<double> mean(o.n_types()), mean_sq(o.n_types());
+            o.field_moments(mean.data(), mean_sq.data());
+            return py::make_tuple(mean, mean_sq);
+        });
+
     py::class_<AbstractFactory>(m, "AbstractFactory")
         .def(
             "create_computation_box",
@@ -283,6 +378,22 @@ PYBIND11_MODULE(langevinfts, m) {
         .def_static("create_factory", py::overload_cast<const std::string&>(&PlatformSelector::create_factory))
         .def_static("create_factory", py::overload_cast<const std::string&, bool>(&PlatformSelector::create_factory));
 
+    // Kernel timing of the slab and pencil passes on the device context (an
+    // addition to the reference's surface, for roofline reports).
+    m.def("set_kernel_timing", [](bool enable) { check_fts(fts_context_set_kernel_timing(hip_context(), enable)); });
+    m.def("get_kernel_timing", [](const std::string& family) {
+        int f;
+        if (family == "slab") f = FTS_KERNEL_SLAB;
+        else if (family == "pencil") f = FTS_KERNEL_PENCIL;
+        else LFTS_THROW("kernel family must be \"slab\" or \"pencil\"");
+        fts_kernel_stats st{};
+        check_fts(fts_context_get_kernel_timing(hip_context(), f, &st));
+        py::dict d;
+        d["milliseconds"] = st.milliseconds;
+        d["bytes"] = st.bytes;
+        d["launches"] = st.launches;
+        return d;
+    });
     m.def("scheduler", [](PropagatorAnalyzer& a, int n_streams) {
         Scheduler s(a.get_computation_propagators(), n_streams);
         return s.get_schedule();

@@ -209,6 +209,38 @@ def molecule_set():
     return mols
 
 
+def driver_records():
+    """Rows the reference's example drivers record in their trailing comments:
+    fts/Lamella.py (L-FTS: iterations, mass error, Q, H, saddle error for the
+    initial saddle point and Langevin steps 1-5) and scft/Gyroid.py (SCFT: iteration,
+    mass error, Q, H, error, box for iterations 1-5). Only the numbers are kept."""
+    num = r"([-+]?\d+\.?\d*(?:[eE][-+]?\d+)?)"
+    out = {}
+    path = os.path.join(REF, "examples", "fts", "Lamella.py")
+    with open(path) as f:
+        lines = f.read().split("\n")
+    rows = []
+    for ln, line in enumerate(lines, 1):
+        m = re.match(r"#\s+(\d+)\s+" + num + r"\s+\[\s*" + num + r"\s*\]\s+" + num + r"\s+\[\s*" + num + r"\s*\]", line)
+        if m:
+            rows.append({"line": ln, "iter": int(m.group(1)), "mass_error": float(m.group(2)),
+                         "Q": float(m.group(3)), "H": float(m.group(4)), "error": float(m.group(5))})
+    out["lamella_lfts"] = {"source": "examples/fts/Lamella.py", "rows": rows}
+    path = os.path.join(REF, "examples", "scft", "Gyroid.py")
+    with open(path) as f:
+        lines = f.read().split("\n")
+    rows = []
+    for ln, line in enumerate(lines, 1):
+        m = re.match(r"\s*#\s+(\d+)\s+" + num + r"\s+\[\s*" + num + r"\s*\]\s+" + num + r"\s+" + num +
+                     r"\s+\[\s*" + num + r",\s*" + num + r",\s*" + num + r"\s*\]", line)
+        if m:
+            rows.append({"line": ln, "iter": int(m.group(1)), "mass_error": float(m.group(2)),
+                         "Q": float(m.group(3)), "H": float(m.group(4)), "error": float(m.group(5)),
+                         "lx": [float(m.group(6)), float(m.group(7)), float(m.group(8))]})
+    out["gyroid_scft"] = {"source": "examples/scft/Gyroid.py", "rows": rows}
+    return out
+
+
 def main():
     if not os.path.exists(DUMP):
         sys.exit("build oracle/_ref first: make -f oracle/Makefile.ref")
@@ -261,6 +293,9 @@ def main():
                     **run_dump(f"mode am\nam {n_var} {hist} {start!r} {mmin!r} {minit!r} {n_iter}\n")}
     with open(os.path.join(HERE, "am.json"), "w") as f:
         json.dump(am, f)
+    drivers = driver_records()
+    with open(os.path.join(HERE, "drivers.json"), "w") as f:
+        json.dump(drivers, f, indent=1)
     print("wrote", len(kats), "KATs,", len(plans), "plans,", len(pseudo), "pseudo grids,",
           len(keys), "keys,", len(am), "AM trajectories")
 

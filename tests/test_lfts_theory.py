@@ -1,0 +1,80 @@
+"""Field-theory tables of the L-FTS driver (examples/fts/lfts.py:20-262) against
+the closed forms of AB and symmetric ABC melts; no GPU needed."""
+import numpy as np
+import pytest
+
+from langevin_fts_amd.lfts import SymmetricPolymerTheory, calculate_sigma
+
+
+def test_ab_closed_form(capsys):
+    chi = 17.0
+    t = SymmetricPolymerTheory(["A", "B"], {"A,B": chi})
+    np.testing.assert_allclose(t.matrix_a, [[1, 1], [-1, 1]], atol=1e-14)
+    np.testing.assert_allclose(t.matrix_a_inv, [[0.5, -0.5], [0.5, 0.5]], atol=1e-14)
+    np.testing.assert_allclose(t.eigenvalues, [-chi, 0.0], atol=1e-12)
+    assert t.aux_fields_real_idx == [0] and t.aux_fields_imag_idx == [1]
+    assert abs(t.h_const - chi / 4) < 1e-13
+    np.testing.assert_allclose(t.h_coef_mu1, [0.0], atol=1e-14)
+    np.testing.assert_allclose(t.h_coef_mu2, [1 / chi], rtol=1e-14)
+    # d/dchiN with vector_s held at its unperturbed value (lfts.py:84-105): the
+    # reference energy keeps chi/4 and has no derivative; mu2 = 1/chi gives -1/chi^2
+    assert abs(t.h_const_deriv_chin["A,B"]) < 1e-8
+    np.testing.assert_allclose(t.h_coef_mu1_deriv_chin["A,B"], [0.0], atol=1e-8)
+    np.testing.assert_allclose(t.h_coef_mu2_deriv_chin["A,B"], [-1 / chi ** 2], rtol=1e-6)
+
+
+def test_ab_func_deriv_matches_closed_form():
+    chi = 13.0
+    t = SymmetricPolymerTheory(["A", "B"], {"A,B": chi})
+    rng = np.random.default_rng(0)
+    w_aux = rng.standard_normal((2, 50))
+    phi = {"A": rng.random(50), "B": rng.random(50)}
+    real, _ = t.compute_func_deriv(w_aux, phi, [0])
+    imag, _ = t.compute_func_deriv(w_aux, phi, [1])
+    np.testing.assert_allclose(real[0], 2 * w_aux[0] / chi + phi["A"] - phi["B"], rtol=1e-14, atol=1e-14)
+    np.testing.assert_allclose(imag[0], -(phi["A"] + phi["B"] - 1.0), rtol=1e-14, atol=1e-14)
+
+
+def test_abc_equal_chi_spectrum():
+    """Equal chi: P chi P = -chi P, eigenvalues (-chi, -chi, 0), two real fields."""
+    chi = 15.0
+    t = SymmetricPolymerTheory(["A", "B", "C"], {"A,B": chi, "B,C": chi, "A,C": chi})
+    np.testing.assert_allclose(np.sort(t.eigenvalues), [-chi, -chi, 0.0], atol=1e-12)
+    assert t.aux_fields_real_idx == [0, 1] and t.aux_fields_imag_idx == [2]
+    S = 3
+    np.testing.assert_allclose(t.matrix_a @ t.matrix_a_inv, np.identity(S), atol=1e-13)
+    np.testing.assert_allclose(t.matrix_a[:, 2], np.ones(S), atol=1e-14)
+    # columns are eigenvectors of P chi P scaled by sqrt(S)
+    pcp = t.matrix_p @ t._chi_matrix({"A,B": chi, "B,C": chi, "A,C": chi}) @ t.matrix_p
+    for i in range(S):
+        np.testing.assert_allclose(pcp @ t.matrix_a[:, i], t.eigenvalues[i] * t.matrix_a[:, i], atol=1e-12)
+    assert np.all(t.matrix_a[0] >= 0)
+
+
+def test_abc_general_func_deriv():
+    """Asymmetric ABC: the linear coefficient tables the device saddle point takes
+    reproduce dH/dw_aux written out term by term (lfts.py:219-245)."""
+    chi_n = {"A,B": 12.0, "B,C": 18.0, "A,C": 25.0}
+    t = SymmetricPolymerTheory(["A", "B", "C"], chi_n)
+    assert t.aux_fields_imag_idx[-1] == 2
+    np.testing.assert_allclose(t.matrix_a @ t.matrix_a_inv, np.identity(3), atol=1e-13)
+    cw, cp, c0 = t.func_deriv_coefficients([0, 1, 2])
+    rng = np.random.default_rng(1)
+    w_aux = rng.standard_normal((3, 20))
+    phi = {m: rng.random(20) for m in "ABC"}
+    h, _ = t.compute_func_deriv(w_aux, phi, [0, 1, 2])
+    for r, i in enumerate([0, 1, 2]):
+        ref = np.zeros(20)
+        if i != 2:
+            ref += 2 * t.h_coef_mu2[i] * w_aux[i] + t.h_coef_mu1[i]
+            for j, m in enumerate("ABC"):
+                ref += t.matrix_a[j, i] * phi[m]
+        else:
+            ref += phi["A"] + phi["B"] + phi["C"] - 1.0
+        if i in t.aux_fields_imag_idx:
+            ref = -ref
+        np.testing.assert_allclose(h[r], ref, rtol=1e-13, atol=1e-13)
+
+
+def test_sigma():
+    assert abs(calculate_sigma(1e4, 0.8, 64 ** 3, 6.0 ** 3) - np.sqrt(2 * 0.8 * 64 ** 3 / (216.0 * 100.0))) < 1e-12
