@@ -143,6 +143,24 @@ def plan_dict(mol, an, model):
     return {"model": model, "polymers": pols, "propagators": props, "blocks": blocks}
 
 
+def copy_calibration(mib):
+    """Plain device copies of a known size, for scaling FETCH_SIZE / WRITE_SIZE."""
+    import ctypes
+
+    from langevin_fts_amd import capi
+
+    nbytes = mib << 20
+    a = capi.DeviceArray(nbytes)
+    b = capi.DeviceArray(nbytes)
+    a.upload(np.ones(nbytes // 8))
+    rt = capi.hip_runtime()
+    for _ in range(8):
+        if rt.hipMemcpy(ctypes.c_void_p(b.ptr), ctypes.c_void_p(a.ptr), nbytes, 3) != 0:
+            raise RuntimeError("hipMemcpy device to device failed")
+    rt.hipDeviceSynchronize()
+    print(json.dumps({"copy_bytes_per_dispatch": nbytes, "dispatches": 8}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -152,7 +170,12 @@ def main():
     ap.add_argument("--seed", type=int, default=12345)
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline budget (0: skip)")
     ap.add_argument("--no-kernel-timing", action="store_true", help="time the steps without launch events")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run 8 device-to-device copies of MIB MiB (reference dispatches for HBM counters)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported (the L-FTS path here is a "
               "single-GPU build)", file=sys.stderr)

@@ -1,0 +1,495 @@
+// Wave-local slab and pencil kernels for cubic grids N = 32, 48, 64, 96, 128.
+//
+// Every 1D line transform belongs to one wavefront for all its Stockham stages:
+// a stage's butterflies take their inputs from registers, and the exchange between
+// stages goes through the wave's own LDS lines with no workgroup barrier (LDS
+// operations of one wavefront complete in issue order; wave_sync keeps the
+// compiler from moving them). Registers carry data across phases wherever the
+// output pattern of one transform is the input pattern of the next: a stage of
+// radix R reads positions j + r L/R of its line, and the last stage of a plan
+// writes exactly that pattern for its own radix, so a transform followed by the
+// reversed plan needs no exchange at the junction.
+//
+// Slab pass (one x-slab per workgroup): 3 workgroup barriers per spectrum (twiddles
+// and plane reuse, y columns -> z rows, z rows -> y columns). Pencil pass (one
+// wavefront per y row and 8 kz columns): no barrier at all.
+#pragma once
+
+#include "fts_fft_ct.hpp"
+
+namespace fts {
+
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// radix plan of a length-L transform, optionally reversed
+template <int L, bool REV>
+struct WPlan {
+    static constexpr CPlan P = make_cplan(L);
+    static constexpr int n = P.n;
+    static constexpr int r(int s) { return REV ? P.r[P.n - 1 - s] : P.r[s]; }
+    static constexpr int ns(int s) { return s == 0 ? 1 : ns(s - 1) * r(s - 1); }
+};
+
+// Butterfly g of a stage with radix R over NLW lines of a wavefront: lane g % 64,
+// register slot g / 64; line-fast order (LF) puts consecutive lanes on consecutive
+// lines, otherwise on consecutive butterflies of one line.
+template <int L, int R, int NLW, bool LF>
+struct WMap {
+    static constexpr int B = L / R;
+    static constexpr int TOTAL = NLW * B;
+    static constexpr int NT = (TOTAL + 63) / 64;
+    static __device__ __forceinline__ int line(int g) { return LF ? g % NLW : g / B; }
+    static __device__ __forceinline__ int idx(int g) { return LF ? g / NLW : g % B; }
+    static __device__ __forceinline__ bool ok(int g, int nvalid) {
+        return (TOTAL % 64 == 0 || g < TOTAL) && line(g) < nvalid;
+    }
+};
+
+template <int L, bool REV, int S, int NLW, bool LF>
+using WStageMap = WMap<L, WPlan<L, REV>::r(S), NLW, LF>;
+
+// registers of one stage's inputs: [slots][radix]
+template <typename T2, int L, bool REV, int S, int NLW, bool LF>
+struct WRegs {
+    using type = T2[WStageMap<L, REV, S, NLW, LF>::NT][WPlan<L, REV>::r(S)];
+};
+
+// Stages S.. of the plan over this wavefront's lines (line l at X + l LP, element i
+// at stride ES). v holds stage S's inputs; `done` receives the last stage's outputs
+// in registers (positions j + r L/R_last).
+template <int DIR, typename T, typename T2, int L, bool REV, int NLW, bool LF, int LP, int ES, int S, typename F>
+__device__ __forceinline__ void wl_stages(typename WRegs<T2, L, REV, S, NLW, LF>::type& v, T2* X, const T2* W,
+                                          int lane, int nvalid, F&& done) {
+    using PL = WPlan<L, REV>;
+    constexpr int R = PL::r(S);
+    constexpr int NS = PL::ns(S);
+    constexpr int TWS = L / (NS * R);
+    using M = WMap<L, R, NLW, LF>;
+#pragma unroll
+    for (int t = 0; t < M::NT; t++) {
+        const int g = lane + 64 * t;
+        if (M::ok(g, nvalid)) {
+            if constexpr (NS > 1) {
+                const int k = M::idx(g) % NS;
+#pragma unroll
+                for (int r = 1; r < R; r++) v[t][r] = cmul(v[t][r], root<DIR>(W, (r * k * TWS) % L));
+            }
+            Dft<R, DIR, T, T2>::run(v[t], W, L);
+        }
+    }
+    if constexpr (S + 1 == PL::n) {
+        done(v);
+    } else {
+        wave_sync();
+#pragma unroll
+        for (int t = 0; t < M::NT; t++) {
+            const int g = lane + 64 * t;
+            if (M::ok(g, nvalid)) {
+                const int j = M::idx(g);
+                const int d = (j / NS) * NS * R + j % NS;
+                T2* base = X + M::line(g) * LP;
+#pragma unroll
+                for (int r = 0; r < R; r++) base[(d + r * NS) * ES] = v[t][r];
+            }
+        }
+        wave_sync();
+        using M2 = WStageMap<L, REV, S + 1, NLW, LF>;
+        constexpr int R2 = PL::r(S + 1);
+        T2 u[M2::NT][R2];
+#pragma unroll
+        for (int t = 0; t < M2::NT; t++) {
+            const int g = lane + 64 * t;
+            if (M2::ok(g, nvalid)) {
+                const T2* base = X + M2::line(g) * LP;
+                const int j = M2::idx(g);
+#pragma unroll
+                for (int r = 0; r < R2; r++) u[t][r] = base[(j + r * M2::B) * ES];
+            }
+        }
+        wl_stages<DIR, T, T2, L, REV, NLW, LF, LP, ES, S + 1>(u, X, W, lane, nvalid, done);
+    }
+}
+
+// ------------------------------------------------------------------ slab pass
+template <int N>
+struct WSlab {
+    static constexpr int NZC = N / 2 + 1;
+    static constexpr int P = NZC;   // plane row pitch (complex); packed z line p spans rows 2p, 2p+1
+    static constexpr int NP = N / 2;
+    static constexpr int BD = N <= 32 ? 256 : (N <= 48 ? 384 : (N <= 64 ? 512 : 1024));
+    static constexpr int NW = BD / 64;
+    static constexpr int NLY = (NZC + NW - 1) / NW;   // y columns per wavefront
+    static constexpr int NLZ = (NP + NW - 1) / NW;    // packed z rows per wavefront
+    static constexpr int NSPEC = N * NZC;
+    static constexpr size_t lds_elems = (size_t)N * P + 2 * N;
+    using YI = WPlan<N, false>;
+    using YF = WPlan<N, false>;
+    using ZI = WPlan<N, false>;
+    using ZF = WPlan<N, true>;   // first radix = last radix of ZI: takes ZI's output registers
+    static constexpr int RZ = ZI::r(ZI::n - 1);
+    using MZ = WMap<N, RZ, NLZ, false>;   // real values of a wavefront's packed rows
+};
+
+template <typename T, int MODE, int N>
+__global__ void __launch_bounds__(WSlab<N>::BD) slab_kernel_wl(SlabBatch<T> batch, const typename vec2<T>::type* twy,
+                                                              const typename vec2<T>::type* twz, double inv_m_d) {
+    using T2 = typename vec2<T>::type;
+    using G = WSlab<N>;
+    using MZ = typename G::MZ;
+    constexpr int RZ = G::RZ, NZC = G::NZC, P = G::P;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T2* S = reinterpret_cast<T2*>(smem_raw);
+    T2* Wy = S + N * P;
+    T2* Wz = Wy + N;
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const SlabItem<T>& it = batch.it[blockIdx.y];
+    const size_t roff = (size_t)blockIdx.x * N * N;
+    const size_t coff = (size_t)blockIdx.x * G::NSPEC;
+    const T inv_m = (T)inv_m_d;
+    const bool has_w = it.w != nullptr;
+    const int col0 = wave * G::NLY;
+    const int ncol = max(0, min(G::NLY, NZC - col0));
+    const int p0 = wave * G::NLZ;
+    const int nrow = max(0, min(G::NLZ, G::NP - p0));
+
+    // w at this wavefront's real positions: rows 2p (e) and 2p + 1 (o), column k
+    T we[MZ::NT][RZ], wo[MZ::NT][RZ];
+#pragma unroll
+    for (int t = 0; t < MZ::NT; t++) {
+        const int g = lane + 64 * t;
+        const bool ok = MZ::ok(g, nrow);
+        const int p = p0 + MZ::line(g);
+#pragma unroll
+        for (int r = 0; r < RZ; r++) {
+            const int k = MZ::idx(g) + r * MZ::B;
+            we[t][r] = (has_w && ok) ? it.w[roff + (size_t)(2 * p) * N + k] : (T)0;
+            wo[t][r] = (has_w && ok) ? it.w[roff + (size_t)(2 * p + 1) * N + k] : (T)0;
+        }
+    }
+    for (int m = threadIdx.x; m < N; m += G::BD) {
+        Wy[m] = twy[m];
+        Wz[m] = twz[m];
+    }
+
+    // spectrum -> real rows times exp(-w coef) / M, in registers
+    auto inverse = [&](const T2* spec, T coef, T (&qe)[MZ::NT][RZ], T (&qo)[MZ::NT][RZ]) {
+        using MA = WStageMap<N, false, 0, G::NLY, true>;
+        T2 v[MA::NT][G::YI::r(0)];
+#pragma unroll
+        for (int t = 0; t < MA::NT; t++) {
+            const int g = lane + 64 * t;
+            if (MA::ok(g, ncol)) {
+                const int c = col0 + MA::line(g), j = MA::idx(g);
+#pragma unroll
+                for (int r = 0; r < G::YI::r(0); r++) v[t][r] = spec[(j + r * MA::B) * NZC + c];
+            }
+        }
+        __syncthreads();   // twiddles stored; the plane's previous users are done
+        wl_stages<+1, T, T2, N, false, G::NLY, true, 1, P, 0>(v, S + col0, Wy, lane, ncol, [&](auto& o) {
+            using MO = WStageMap<N, false, G::YI::n - 1, G::NLY, true>;
+            wave_sync();
+#pragma unroll
+            for (int t = 0; t < MO::NT; t++) {
+                const int g = lane + 64 * t;
+                if (MO::ok(g, ncol)) {
+                    const int c = col0 + MO::line(g), j = MO::idx(g);
+#pragma unroll
+                    for (int r = 0; r < G::YI::r(G::YI::n - 1); r++) S[(j + r * MO::B) * P + c] = o[t][r];
+                }
+            }
+        });
+        __syncthreads();
+        // packed rows: Z[k] = X_{2p}[k] + i X_{2p+1}[k] from the half spectra (Hermitian fill)
+        using MB = WStageMap<N, false, 0, G::NLZ, false>;
+        T2 u[MB::NT][G::ZI::r(0)];
+#pragma unroll
+        for (int t = 0; t < MB::NT; t++) {
+            const int g = lane + 64 * t;
+            if (MB::ok(g, nrow)) {
+                const int p = p0 + MB::line(g), j = MB::idx(g);
+#pragma unroll
+                for (int r = 0; r < G::ZI::r(0); r++) {
+                    const int k = j + r * MB::B;
+                    const bool upper = k >= NZC;
+                    const int kk = upper ? N - k : k;
+                    T2 a = S[2 * p * P + kk];
+                    T2 b = S[(2 * p + 1) * P + kk];
+                    if (kk == 0 || 2 * kk == N) {
+                        a.y = 0;
+                        b.y = 0;
+                    }
+                    if (upper) {
+                        a.y = -a.y;
+                        b.y = -b.y;
+                    }
+                    u[t][r] = T2{a.x - b.y, a.y + b.x};
+                }
+            }
+        }
+        wl_stages<+1, T, T2, N, false, G::NLZ, false, 2 * P, 1, 0>(u, S + 2 * p0 * P, Wz, lane, nrow, [&](auto& o) {
+#pragma unroll
+            for (int t = 0; t < MZ::NT; t++) {
+                const int g = lane + 64 * t;
+                if (MZ::ok(g, nrow)) {
+#pragma unroll
+                    for (int r = 0; r < RZ; r++) {
+                        qe[t][r] = o[t][r].x * factor<T>(we[t][r], has_w, coef) * inv_m;
+                        qo[t][r] = o[t][r].y * factor<T>(wo[t][r], has_w, coef) * inv_m;
+                    }
+                }
+            }
+        });
+    };
+
+    // real rows times exp(-w coef) -> spectrum
+    auto forward = [&](const T (&qe)[MZ::NT][RZ], const T (&qo)[MZ::NT][RZ], T coef, T2* spec) {
+        T2 u[MZ::NT][RZ];
+#pragma unroll
+        for (int t = 0; t < MZ::NT; t++) {
+            const int g = lane + 64 * t;
+            if (MZ::ok(g, nrow)) {
+#pragma unroll
+                for (int r = 0; r < RZ; r++)
+                    u[t][r] = T2{qe[t][r] * factor<T>(we[t][r], has_w, coef), qo[t][r] * factor<T>(wo[t][r], has_w, coef)};
+            }
+        }
+        wl_stages<-1, T, T2, N, true, G::NLZ, false, 2 * P, 1, 0>(u, S + 2 * p0 * P, Wz, lane, nrow, [&](auto& o) {
+            using MO = WStageMap<N, true, G::ZF::n - 1, G::NLZ, false>;
+            wave_sync();
+#pragma unroll
+            for (int t = 0; t < MO::NT; t++) {
+                const int g = lane + 64 * t;
+                if (MO::ok(g, nrow)) {
+                    const int p = p0 + MO::line(g), j = MO::idx(g);
+#pragma unroll
+                    for (int r = 0; r < G::ZF::r(G::ZF::n - 1); r++) S[2 * p * P + j + r * MO::B] = o[t][r];
+                }
+            }
+        });
+        wave_sync();
+        // split Z into the half spectra of rows 2p and 2p + 1
+        constexpr int TOT = G::NLZ * NZC;
+        constexpr int NT = (TOT + 63) / 64;
+        T2 va[NT], vb[NT];
+#pragma unroll
+        for (int t = 0; t < NT; t++) {
+            const int f = lane + 64 * t;
+            const int l = f / NZC, k = f - l * NZC;
+            if ((TOT % 64 == 0 || f < TOT) && l < nrow) {
+                const T2* Z = S + 2 * (p0 + l) * P;
+                const T2 A = Z[k];
+                const int km = k == 0 ? 0 : N - k;
+                const T2 B = T2{Z[km].x, -Z[km].y};
+                va[t] = T2{(T)0.5 * (A.x + B.x), (T)0.5 * (A.y + B.y)};
+                vb[t] = T2{(T)0.5 * (A.y - B.y), (T)-0.5 * (A.x - B.x)};
+            }
+        }
+        wave_sync();
+#pragma unroll
+        for (int t = 0; t < NT; t++) {
+            const int f = lane + 64 * t;
+            const int l = f / NZC, k = f - l * NZC;
+            if ((TOT % 64 == 0 || f < TOT) && l < nrow) {
+                S[2 * (p0 + l) * P + k] = va[t];
+                S[(2 * (p0 + l) + 1) * P + k] = vb[t];
+            }
+        }
+        __syncthreads();
+        using MD = WStageMap<N, false, 0, G::NLY, true>;
+        T2 v[MD::NT][G::YF::r(0)];
+#pragma unroll
+        for (int t = 0; t < MD::NT; t++) {
+            const int g = lane + 64 * t;
+            if (MD::ok(g, ncol)) {
+                const int c = col0 + MD::line(g), j = MD::idx(g);
+#pragma unroll
+                for (int r = 0; r < G::YF::r(0); r++) v[t][r] = S[(j + r * MD::B) * P + c];
+            }
+        }
+        wl_stages<-1, T, T2, N, false, G::NLY, true, 1, P, 0>(v, S + col0, Wy, lane, ncol, [&](auto& o) {
+            using MO = WStageMap<N, false, G::YF::n - 1, G::NLY, true>;
+#pragma unroll
+            for (int t = 0; t < MO::NT; t++) {
+                const int g = lane + 64 * t;
+                if (MO::ok(g, ncol)) {
+                    const int c = col0 + MO::line(g), j = MO::idx(g);
+#pragma unroll
+                    for (int r = 0; r < G::YF::r(G::YF::n - 1); r++) spec[(j + r * MO::B) * NZC + c] = o[t][r];
+                }
+            }
+        });
+    };
+
+    T qe[MZ::NT][RZ], qo[MZ::NT][RZ];
+    bool do_fwd = true;
+    if constexpr (MODE == SLAB_FWD) {
+#pragma unroll
+        for (int t = 0; t < MZ::NT; t++) {
+            const int g = lane + 64 * t;
+            const bool ok = MZ::ok(g, nrow);
+            const int p = p0 + MZ::line(g);
+#pragma unroll
+            for (int r = 0; r < RZ; r++) {
+                const int k = MZ::idx(g) + r * MZ::B;
+                qe[t][r] = ok ? it.qin[roff + (size_t)(2 * p) * N + k] : (T)0;
+                qo[t][r] = ok ? it.qin[roff + (size_t)(2 * p + 1) * N + k] : (T)0;
+            }
+        }
+        __syncthreads();   // twiddles
+    } else {
+        inverse(it.c0 + coff, it.post0, qe, qo);
+        if constexpr (MODE == SLAB_INV2) {
+            T q2e[MZ::NT][RZ], q2o[MZ::NT][RZ];
+            inverse(it.c1 + coff, it.post1, q2e, q2o);
+#pragma unroll
+            for (int t = 0; t < MZ::NT; t++)
+#pragma unroll
+                for (int r = 0; r < RZ; r++) {
+                    qe[t][r] = ((T)4.0 * q2e[t][r] - qe[t][r]) / (T)3.0;
+                    qo[t][r] = ((T)4.0 * q2o[t][r] - qo[t][r]) / (T)3.0;
+                }
+        }
+#pragma unroll
+        for (int t = 0; t < MZ::NT; t++) {
+            const int g = lane + 64 * t;
+            if (MZ::ok(g, nrow)) {
+                const int p = p0 + MZ::line(g);
+#pragma unroll
+                for (int r = 0; r < RZ; r++) {
+                    const int k = MZ::idx(g) + r * MZ::B;
+                    const size_t ie = roff + (size_t)(2 * p) * N + k, io = ie + N;
+                    if (it.mask) {
+                        qe[t][r] *= it.mask[ie];
+                        qo[t][r] *= it.mask[io];
+                    }
+                    if (it.qout) {
+                        it.qout[ie] = qe[t][r];
+                        it.qout[io] = qo[t][r];
+                    }
+                }
+            }
+        }
+        do_fwd = it.forward != 0;
+    }
+    if (!do_fwd) return;
+    const int n_out = (MODE == SLAB_FWD) ? it.n_out : (MODE == SLAB_INV2 ? 2 : 1);
+    for (int o = 0; o < n_out; o++) {
+        if (o > 0) __syncthreads();   // the plane of the previous spectrum is fully read
+        forward(qe, qo, o == 0 ? it.pre0 : it.pre1, (o == 0 ? it.c0 : it.c1) + coff);
+    }
+}
+
+// ------------------------------------------------------------------ pencil pass
+template <int N>
+struct WPencil {
+    static constexpr int NZC = N / 2 + 1;
+    static constexpr int NL = 8;   // kz columns per wavefront
+    static constexpr int TILES = (NZC + NL - 1) / NL;
+    static constexpr int BD = 64;
+    static constexpr size_t lds_elems = (size_t)N * NL + N;
+    using F = WPlan<N, false>;
+    using I = WPlan<N, true>;   // first radix = last radix of F
+};
+
+template <typename T, int N>
+__global__ void __launch_bounds__(64) xpass_kernel_wl(XBatch<T> batch, const typename vec2<T>::type* twx,
+                                                      const T* bx, const T* by, const T* bz) {
+    using T2 = typename vec2<T>::type;
+    using G = WPencil<N>;
+    using F = typename G::F;
+    using I = typename G::I;
+    constexpr int NL = G::NL, NZC = G::NZC;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T2* S = reinterpret_cast<T2*>(smem_raw);   // line l, element i at S[i NL + l]
+    T2* Wx = S + N * NL;
+    const int lane = threadIdx.x;
+    const XItem<T>& it = batch.it[blockIdx.y];
+    const int j = blockIdx.x / G::TILES;
+    const int k0 = (blockIdx.x - j * G::TILES) * NL;
+    const int nl = min(NL, NZC - k0);
+    constexpr size_t ROW = (size_t)N * NZC;
+    T2* c = it.c + (size_t)j * NZC + k0;
+    const int table = it.table;
+    for (int m = lane; m < N; m += 64) Wx[m] = twx[m];
+
+    auto store = [&](auto& o) {
+        using MO = WStageMap<N, true, I::n - 1, NL, true>;
+#pragma unroll
+        for (int t = 0; t < MO::NT; t++) {
+            const int g = lane + 64 * t;
+            if (MO::ok(g, nl)) {
+                const int l = MO::line(g), i0 = MO::idx(g);
+#pragma unroll
+                for (int r = 0; r < I::r(I::n - 1); r++) c[(i0 + r * MO::B) * ROW + l] = o[t][r];
+            }
+        }
+    };
+    if (table == -2) {
+        using M0 = WStageMap<N, true, 0, NL, true>;
+        T2 v[M0::NT][I::r(0)];
+#pragma unroll
+        for (int t = 0; t < M0::NT; t++) {
+            const int g = lane + 64 * t;
+            if (M0::ok(g, nl)) {
+                const int l = M0::line(g), i0 = M0::idx(g);
+#pragma unroll
+                for (int r = 0; r < I::r(0); r++) v[t][r] = c[(i0 + r * M0::B) * ROW + l];
+            }
+        }
+        wave_sync();
+        wl_stages<+1, T, T2, N, true, NL, true, 1, NL, 0>(v, S, Wx, lane, nl, store);
+        return;
+    }
+    using M0 = WStageMap<N, false, 0, NL, true>;
+    T2 v[M0::NT][F::r(0)];
+#pragma unroll
+    for (int t = 0; t < M0::NT; t++) {
+        const int g = lane + 64 * t;
+        if (M0::ok(g, nl)) {
+            const int l = M0::line(g), i0 = M0::idx(g);
+#pragma unroll
+            for (int r = 0; r < F::r(0); r++) v[t][r] = c[(i0 + r * M0::B) * ROW + l];
+        }
+    }
+    wave_sync();
+    wl_stages<-1, T, T2, N, false, NL, true, 1, NL, 0>(v, S, Wx, lane, nl, [&](auto& o) {
+        using MF = WStageMap<N, false, F::n - 1, NL, true>;
+        constexpr int RF = F::r(F::n - 1);
+        if (table == -1) {
+#pragma unroll
+            for (int t = 0; t < MF::NT; t++) {
+                const int g = lane + 64 * t;
+                if (MF::ok(g, nl)) {
+                    const int l = MF::line(g), i0 = MF::idx(g);
+#pragma unroll
+                    for (int r = 0; r < RF; r++) c[(i0 + r * MF::B) * ROW + l] = o[t][r];
+                }
+            }
+            return;
+        }
+        const T* tx = bx + (size_t)table * N;
+        const T byj = by[(size_t)table * N + j];
+        const T* tz = bz + (size_t)table * NZC + k0;
+#pragma unroll
+        for (int t = 0; t < MF::NT; t++) {
+            const int g = lane + 64 * t;
+            if (MF::ok(g, nl)) {
+                const int l = MF::line(g), i0 = MF::idx(g);
+                const T bl = byj * tz[l];
+#pragma unroll
+                for (int r = 0; r < RF; r++) {
+                    const T b = tx[i0 + r * MF::B] * bl;
+                    o[t][r] = T2{o[t][r].x * b, o[t][r].y * b};
+                }
+            }
+        }
+        wl_stages<+1, T, T2, N, true, NL, true, 1, NL, 0>(o, S, Wx, lane, nl, store);
+    });
+}
+
+}  // namespace fts

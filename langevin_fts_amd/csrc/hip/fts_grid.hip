@@ -25,8 +25,10 @@ void KernelTimer::enable(bool on) {
 
 hipEvent_t KernelTimer::take() {
     if (pool_.empty()) {
+        // timing-only events: no system-scope fence (no L2 writeback/invalidate
+        // between the timed kernels)
         hipEvent_t e;
-        FTS_HIP_CHECK(hipEventCreate(&e));
+        FTS_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
         return e;
     }
     hipEvent_t e = pool_.back();
@@ -125,6 +127,14 @@ void allow_lds_ct() {
     allow_lds(&slab_kernel_ct<float, SLAB_INV2, N>);
     allow_lds(&xpass_kernel_ct<double, N>);
     allow_lds(&xpass_kernel_ct<float, N>);
+    allow_lds(&slab_kernel_wl<double, SLAB_FWD, N>);
+    allow_lds(&slab_kernel_wl<double, SLAB_INV1, N>);
+    allow_lds(&slab_kernel_wl<double, SLAB_INV2, N>);
+    allow_lds(&slab_kernel_wl<float, SLAB_FWD, N>);
+    allow_lds(&slab_kernel_wl<float, SLAB_INV1, N>);
+    allow_lds(&slab_kernel_wl<float, SLAB_INV2, N>);
+    allow_lds(&xpass_kernel_wl<double, N>);
+    allow_lds(&xpass_kernel_wl<float, N>);
 }
 
 void set_lds_attributes_ct() {
@@ -140,6 +150,21 @@ void set_lds_attributes_ct() {
 
 template <typename T, int N>
 void launch_slab_ct(const Grid& g, int mode, const SlabBatch<T>& b) {
+    if (g.wave_local) {
+        using G = WSlab<N>;
+        const dim3 grid(N, b.n), block(G::BD);
+        const size_t lds = G::lds_elems * sizeof(typename vec2<T>::type);
+        if (mode == SLAB_FWD)
+            hipLaunchKernelGGL((slab_kernel_wl<T, SLAB_FWD, N>), grid, block, lds, g.stream(), b, g.tw<T>(1),
+                               g.tw<T>(2), g.sg.inv_m);
+        else if (mode == SLAB_INV1)
+            hipLaunchKernelGGL((slab_kernel_wl<T, SLAB_INV1, N>), grid, block, lds, g.stream(), b, g.tw<T>(1),
+                               g.tw<T>(2), g.sg.inv_m);
+        else
+            hipLaunchKernelGGL((slab_kernel_wl<T, SLAB_INV2, N>), grid, block, lds, g.stream(), b, g.tw<T>(1),
+                               g.tw<T>(2), g.sg.inv_m);
+        return;
+    }
     using G = SlabShape<N>;
     const dim3 grid(N, b.n), block(G::BD);
     const size_t lds = G::lds_elems * sizeof(typename vec2<T>::type);
@@ -156,6 +181,13 @@ void launch_slab_ct(const Grid& g, int mode, const SlabBatch<T>& b) {
 
 template <typename T, int N>
 void launch_xpass_ct(const Grid& g, const XBatch<T>& b, const T* bx, const T* by, const T* bz) {
+    if (g.wave_local) {
+        using G = WPencil<N>;
+        const dim3 grid(N * G::TILES, b.n), block(G::BD);
+        hipLaunchKernelGGL((xpass_kernel_wl<T, N>), grid, block, G::lds_elems * sizeof(typename vec2<T>::type),
+                           g.stream(), b, g.tw<T>(0), bx, by, bz);
+        return;
+    }
     using G = PencilShape<N>;
     const dim3 grid(N * G::TILES, b.n), block(G::BD);
     hipLaunchKernelGGL((xpass_kernel_ct<T, N>), grid, block, G::lds_elems * sizeof(typename vec2<T>::type), g.stream(),
@@ -277,6 +309,8 @@ Grid::Grid(fts_context* c, int d, const int* n, const double* l, const double* m
         const int n = tnx[0];
         if (n == 32 || n == 48 || n == 64 || n == 96 || n == 128) ct_n = n;
     }
+    // wave-local line transforms for those grids (env FTS_CT_KERNELS=1: the block-synchronous ones)
+    wave_local = ct_n != 0 && !std::getenv("FTS_CT_KERNELS");
     if (ct_n) set_lds_attributes_ct();
 
     partials.allocate(sizeof(double) * MAX_BATCH * RED_BLOCKS * 3);

@@ -9,6 +9,7 @@
 #include "fts_common.hpp"
 #include "fts_fft.hpp"
 #include "fts_fft_ct.hpp"
+#include "fts_fft_wl.hpp"
 
 namespace fts {
 
@@ -93,6 +94,7 @@ public:
     int slab_maxr;   // real values per thread in the slab kernel (8 or 16)
     bool big;        // some axis needs a radix outside {2, 3, 4, 5, 8}
     int ct_n;        // N of the compile-time specialised kernels for an N^3 grid, 0: runtime kernels
+    bool wave_local; // N^3 grids: one wavefront per line transform
     DeviceBuffer partials;   // reduction scratch: MAX_BATCH * RED_BLOCKS doubles
     DeviceBuffer scalars;    // reduction results: MAX_BATCH doubles
     DeviceBuffer scratch64;  // general FP64 scratch of 2 fields
