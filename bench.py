@@ -12,22 +12,32 @@ Fields start as N(0,1) from a fixed seed; W warm-up steps (and the initial saddl
 point) run untimed, then K steps are timed between stream synchronisations.
 
 Printed: one JSON line. `roofline` is the slab pass (the dominant kernel:
-inverse y/z FFT, exp(-w ds) factor, q store, forward z/y FFT) timed with HIP events
-around every launch on the library's stream during the timed steps; its
-algorithmic bytes are the spectra, fields and w each launch must move once.
+inverse y/z FFT, exp(-w ds) factor, q store, forward z/y FFT), timed during the
+timed steps with fence-free HIP events around one launch in 16 on the library's
+stream; its algorithmic bytes are the spectra, fields and w each launch must move
+once. `traffic` is the slab launch's HBM bytes from two `rocprofv3 --pmc` child
+runs (FETCH_SIZE, WRITE_SIZE) made before this process touches the GPU, each
+scaled by what the same counter reads for a plain device copy of 256 MiB.
 `cpu_baseline` times the CPU oracle (FP64, OpenMP FFT lines) on one saddle
 iteration of the same 64^3 workload and scales by the GPU run's iterations per step.
 """
 import argparse
 import contextlib
+import csv
+import glob
 import json
 import os
+import re
+import shutil
+import subprocess
 import sys
+import tempfile
 import time
 
 import numpy as np
 
 PEAK_HBM_GBPS = 8000.0
+SAMPLE = 16   # one launch in SAMPLE of each kernel family carries timing events
 
 WORKLOADS = {
     "lfts64": dict(nx=[64, 64, 64], lx=[6.0, 6.0, 6.0], model="discrete", ds=1 / 90, f=0.5, chi_n=17.0,
@@ -143,6 +153,66 @@ def plan_dict(mol, an, model):
     return {"model": model, "polymers": pols, "propagators": props, "blocks": blocks}
 
 
+SLAB_KERNEL = re.compile(r"slab_kernel_\w+<double, 1, 64>")
+
+
+def pmc_child(workload):
+    """A short run for the counter passes: one saddle point of two iterations."""
+    from langevin_fts_amd import lfts
+
+    w = dict(WORKLOADS[workload], max_iter=2)
+    M = int(np.prod(w["nx"]))
+    with stdout_to_stderr():
+        sim = lfts.LFTS(lfts_params(w, "/tmp/lfts_bench"), random_seed=1)
+        rng = np.random.Generator(np.random.PCG64(1))
+        sim.saddle_device.set_fields(np.ascontiguousarray(rng.standard_normal((2, M))))
+        sim._find_saddle_point_device()
+
+
+def _counter_means(outdir, counter, name_re):
+    vals = []
+    for path in glob.glob(os.path.join(outdir, "**", "*counter_collection.csv"), recursive=True):
+        with open(path) as f:
+            for row in csv.DictReader(f):
+                if row["Counter_Name"] == counter and name_re.search(row["Kernel_Name"]):
+                    vals.append(float(row["Counter_Value"]))
+    return sum(vals) / len(vals) if vals else None
+
+
+def measure_traffic(workload):
+    """HBM bytes of one slab launch: FETCH_SIZE and WRITE_SIZE (KiB per dispatch)
+    scaled by the same counters on a 256 MiB device copy. None when rocprofv3 is
+    missing or a pass fails."""
+    prof = shutil.which("rocprofv3")
+    if prof is None:
+        return None
+    here = os.path.abspath(__file__)
+    env = dict(os.environ, TMPDIR="/tmp")
+    out = {}
+    try:
+        for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+            for mode, name_re, extra in (("copy", re.compile(r"copyBuffer"), ["--copy-calibration", "256"]),
+                                         ("slab", SLAB_KERNEL, ["--pmc-child", "--workload", workload])):
+                d = tempfile.mkdtemp(prefix="fts_pmc_", dir="/tmp")
+                cmd = [prof, "--pmc", counter, "--output-format", "csv", "-d", d, "-o", mode, "--",
+                       sys.executable, here] + extra
+                r = subprocess.run(cmd, cwd="/tmp", env=env, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL,
+                                   timeout=180)
+                if r.returncode != 0:
+                    return None
+                out[(counter, mode)] = _counter_means(d, counter, name_re)
+                shutil.rmtree(d, ignore_errors=True)
+    except (OSError, subprocess.SubprocessError):
+        return None
+    if any(v is None or v <= 0 for v in out.values()):
+        return None
+    copy_kib = 256 * 1024
+    fetch = out[("FETCH_SIZE", "slab")] * copy_kib / out[("FETCH_SIZE", "copy")]
+    write = out[("WRITE_SIZE", "slab")] * copy_kib / out[("WRITE_SIZE", "copy")]
+    return {"bytes": (fetch + write) * 1024, "fetch_scale": copy_kib / out[("FETCH_SIZE", "copy")],
+            "write_scale": copy_kib / out[("WRITE_SIZE", "copy")]}
+
+
 def copy_calibration(mib):
     """Plain device copies of a known size, for scaling FETCH_SIZE / WRITE_SIZE."""
     import ctypes
@@ -169,17 +239,25 @@ def main():
     ap.add_argument("--workload", default="lfts64", choices=sorted(WORKLOADS))
     ap.add_argument("--seed", type=int, default=12345)
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline budget (0: skip)")
-    ap.add_argument("--no-kernel-timing", action="store_true", help="time the steps without launch events")
+    ap.add_argument("--no-kernel-timing", action="store_true", help="time the steps without sampled launch events")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 8 device-to-device copies of MIB MiB (reference dispatches for HBM counters)")
+    ap.add_argument("--pmc-child", action="store_true", help="only run the short pass the counter runs profile")
+    ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 counter passes")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
+        return
+    if args.pmc_child:
+        pmc_child(args.workload)
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported (the L-FTS path here is a "
               "single-GPU build)", file=sys.stderr)
         sys.exit(2)
+
+    # counter passes first, while this process has not initialised the GPU
+    traffic = None if args.no_traffic or args.workload != "lfts64" else measure_traffic(args.workload)
 
     from langevin_fts_amd import langevinfts as lf
     from langevin_fts_amd import lfts
@@ -197,7 +275,7 @@ def main():
         langevin_steps(sim, args.warmup, warm)
 
         if not args.no_kernel_timing:
-            lf.set_kernel_timing(True)
+            lf.set_kernel_timing(SAMPLE)
         stats = {"iterations": 0, "failures": 0}
         sim.saddle_device.field_moments()   # synchronises the stream
         t0 = time.perf_counter()
@@ -207,7 +285,7 @@ def main():
         kern = {}
         if not args.no_kernel_timing:
             kern = {f: lf.get_kernel_timing(f) for f in ("slab", "pencil")}
-            lf.set_kernel_timing(False)
+            lf.set_kernel_timing(0)
 
     steps_per_s = args.steps / elapsed
     iters = stats["iterations"] / args.steps
@@ -225,16 +303,22 @@ def main():
         sl, pe = kern["slab"], kern["pencil"]
         achieved = sl["bytes"] / (sl["milliseconds"] * 1e-3) / 1e9
         out["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBPS, "unit": "GB/s",
-                           "frac": achieved / PEAK_HBM_GBPS, "traffic": None,
-                           "kernel": "slab pass (slab_kernel_ct, 64^3 FP64)",
-                           "launches": sl["launches"],
+                           "frac": achieved / PEAK_HBM_GBPS,
+                           "traffic": traffic["bytes"] if traffic else None,
+                           "kernel": "slab pass (slab_kernel_wl<double, 1, 64>: 2 propagators per launch)",
+                           "timed_launch_count": sl["launches"],
                            "avg_launch_us": 1e3 * sl["milliseconds"] / max(sl["launches"], 1),
-                           "bytes_per_launch": sl["bytes"] / max(sl["launches"], 1)}
+                           "bytes_per_launch": sl["bytes"] / max(sl["launches"], 1),
+                           "timed_launches": f"1 in {SAMPLE}"}
+        if traffic:
+            out["roofline"]["traffic_counter_scale"] = {"FETCH_SIZE": traffic["fetch_scale"],
+                                                        "WRITE_SIZE": traffic["write_scale"]}
         both_ms = sl["milliseconds"] + pe["milliseconds"]
         out["config"]["propagator_GBps"] = (sl["bytes"] + pe["bytes"]) / (both_ms * 1e-3) / 1e9
         out["config"]["pencil_GBps"] = pe["bytes"] / (pe["milliseconds"] * 1e-3) / 1e9
         out["config"]["pencil_avg_launch_us"] = 1e3 * pe["milliseconds"] / max(pe["launches"], 1)
-        out["config"]["propagator_kernel_share"] = both_ms * 1e-3 / elapsed
+        # the timed launches are one in SAMPLE of each family
+        out["config"]["propagator_kernel_share"] = SAMPLE * both_ms * 1e-3 / elapsed
     if args.cpu_seconds > 0:
         with stdout_to_stderr():
             out["cpu_baseline"] = cpu_baseline(w, iters, args.cpu_seconds)

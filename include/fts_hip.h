@@ -49,9 +49,10 @@ int fts_abi_version(void);
 int fts_context_create(int device, fts_context** out);
 int fts_context_destroy(fts_context* ctx);
 /* Kernel timing (no reference counterpart: measurement of the slab and pencil
- * passes for roofline reports). Enabling clears the totals; get synchronises the
- * stream and returns the totals of one family since the last enable. */
-int fts_context_set_kernel_timing(fts_context* ctx, int enable);
+ * passes for roofline reports). sample_every = k > 0 times one launch in k of each
+ * family with a pair of fence-free HIP events (0: off); the call clears the totals.
+ * get synchronises the stream and returns the totals of the timed launches. */
+int fts_context_set_kernel_timing(fts_context* ctx, int sample_every);
 int fts_context_get_kernel_timing(fts_context* ctx, int family, fts_kernel_stats* out);
 int fts_context_synchronize(fts_context* ctx);
 /* hipStream_t of the context, returned as void* */

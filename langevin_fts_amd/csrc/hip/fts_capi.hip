@@ -60,11 +60,12 @@ int fts_context_destroy(fts_context* ctx) {
     });
 }
 
-int fts_context_set_kernel_timing(fts_context* ctx, int enable) {
+int fts_context_set_kernel_timing(fts_context* ctx, int sample_every) {
     return guarded([&] {
         need(ctx, "ctx");
+        FTS_REQUIRE(sample_every >= 0, "sample_every must be non-negative");
         FTS_HIP_CHECK(hipSetDevice(ctx->device));
-        ctx->timer.enable(enable != 0);
+        ctx->timer.enable(sample_every);
     });
 }
 

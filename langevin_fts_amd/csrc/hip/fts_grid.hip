@@ -17,10 +17,11 @@ KernelTimer::~KernelTimer() {
     for (auto e : pool_) (void)hipEventDestroy(e);
 }
 
-void KernelTimer::enable(bool on) {
+void KernelTimer::enable(int sample_every) {
     drain(pending_.size());
     for (auto& s : stats_) s = fts_kernel_stats{};
-    on_ = on;
+    for (auto& c : count_) c = 0;
+    every_ = sample_every > 0 ? sample_every : 0;
 }
 
 hipEvent_t KernelTimer::take() {
@@ -36,7 +37,8 @@ hipEvent_t KernelTimer::take() {
     return e;
 }
 
-hipEvent_t KernelTimer::begin(hipStream_t st) {
+hipEvent_t KernelTimer::begin(hipStream_t st, int family) {
+    if (every_ == 0 || count_[family]++ % every_ != 0) return nullptr;
     hipEvent_t a = take();
     FTS_HIP_CHECK(hipEventRecord(a, st));
     return a;
@@ -386,7 +388,7 @@ void Grid::launch_slab(int mode, const SlabBatch<T>& b) const {
     require_supported<T>(*this);
     if (b.n == 0) return;
     KernelTimer& timer = ctx->timer;
-    hipEvent_t start = timer.enabled() ? timer.begin(stream()) : nullptr;
+    hipEvent_t start = timer.begin(stream(), FTS_KERNEL_SLAB);
     switch (ct_n) {
         case 32: launch_slab_ct<T, 32>(*this, mode, b); break;
         case 48: launch_slab_ct<T, 48>(*this, mode, b); break;
@@ -410,7 +412,7 @@ void Grid::launch_xpass(const XBatch<T>& b, const T* bx, const T* by, const T* b
     require_supported<T>(*this);
     if (b.n == 0) return;
     KernelTimer& timer = ctx->timer;
-    hipEvent_t start = timer.enabled() ? timer.begin(stream()) : nullptr;
+    hipEvent_t start = timer.begin(stream(), FTS_KERNEL_PENCIL);
     switch (ct_n) {
         case 32: launch_xpass_ct<T, 32>(*this, b, bx, by, bz); break;
         case 48: launch_xpass_ct<T, 48>(*this, b, bx, by, bz); break;

@@ -19,9 +19,9 @@ namespace fts {
 class KernelTimer {
 public:
     ~KernelTimer();
-    void enable(bool on);
-    bool enabled() const { return on_; }
-    hipEvent_t begin(hipStream_t st);
+    void enable(int sample_every);
+    // a start event when this launch of the family is sampled, else nullptr
+    hipEvent_t begin(hipStream_t st, int family);
     void end(hipStream_t st, hipEvent_t start, int family, double bytes);
     fts_kernel_stats get(int family);
 
@@ -33,7 +33,8 @@ private:
     };
     hipEvent_t take();
     void drain(size_t n);
-    bool on_ = false;
+    int every_ = 0;
+    long long count_[FTS_KERNEL_FAMILIES] = {};
     std::vector<Pending> pending_;
     std::vector<hipEvent_t> pool_;
     fts_kernel_stats stats_[FTS_KERNEL_FAMILIES] = {};

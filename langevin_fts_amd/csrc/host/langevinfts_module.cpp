@@ -380,7 +380,8 @@ PYBIND11_MODULE(langevinfts, m) {
 
     // Kernel timing of the slab and pencil passes on the device context (an
     // addition to the reference's surface, for roofline reports).
-    m.def("set_kernel_timing", [](bool enable) { check_fts(fts_context_set_kernel_timing(hip_context(), enable)); });
+    m.def("set_kernel_timing",
+          [](int sample_every) { check_fts(fts_context_set_kernel_timing(hip_context(), sample_every)); });
     m.def("get_kernel_timing", [](const std::string& family) {
         int f;
         if (family == "slab") f = FTS_KERNEL_SLAB;

@@ -1,0 +1,110 @@
+"""SCFT driver on the hip platform against the rows the reference's own example
+recorded (examples/scft/Gyroid.py:91-96: 32^3 gyroid, continuous AB, f=0.36,
+chiN=20, box relaxation), and the non-relaxing path against the oracle."""
+import re
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.conftest import golden
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+def gyroid_fields(nx):
+    """Initial gyroid fields of examples/scft/Gyroid.py (double-gyroid level-set form)."""
+    x = (np.arange(nx[0]) + 1) * 2 * np.pi / nx[0]
+    y = (np.arange(nx[1]) + 1) * 2 * np.pi / nx[1]
+    z = (np.arange(nx[2]) + 1) * 2 * np.pi / nx[2]
+    xx, yy, zz = np.meshgrid(x, y, z, indexing="ij")
+    c1 = np.sqrt(8.0 / 3.0) * (np.cos(xx) * np.sin(yy) * np.sin(2.0 * zz) + np.cos(yy) * np.sin(zz) * np.sin(2.0 * xx)
+                               + np.cos(zz) * np.sin(xx) * np.sin(2.0 * yy))
+    c2 = np.sqrt(4.0 / 3.0) * (np.cos(2.0 * xx) * np.cos(2.0 * yy) + np.cos(2.0 * yy) * np.cos(2.0 * zz)
+                               + np.cos(2.0 * zz) * np.cos(2.0 * xx))
+    return -0.3164 * c1 + 0.1074 * c2, 0.3164 * c1 - 0.1074 * c2
+
+
+def params(nx, lx, box, max_iter, f=0.36, model="continuous", ds=1 / 100, chi_n=20.0):
+    return {"nx": nx, "lx": lx, "box_is_altering": box, "chain_model": model, "ds": ds,
+            "segment_lengths": {"A": 1.0, "B": 1.0}, "chi_n": {"A,B": chi_n},
+            "distinct_polymers": [{"volume_fraction": 1.0,
+                                   "blocks": [{"type": "A", "length": f}, {"type": "B", "length": 1 - f}]}],
+            "optimizer": {"name": "am", "max_hist": 20, "start_error": 1e-2, "mix_min": 0.1, "mix_init": 0.1},
+            "max_iter": max_iter, "tolerance": 1e-8}
+
+
+def test_gyroid_recorded_iterations(capsys):
+    from langevin_fts_amd import scft
+
+    say("scft gyroid recorded rows")
+    rec = golden("drivers")["gyroid_scft"]["rows"]
+    nx = [32, 32, 32]
+    w_a, w_b = gyroid_fields(nx)
+    calc = scft.SCFT(params(nx, [3.3, 3.3, 3.3], True, len(rec)))
+    calc.run(initial_fields={"A": w_a, "B": w_b})
+    out = capsys.readouterr().out
+    num = r"([-+]?\d+\.?\d*(?:[eE][-+]?\d+)?)"
+    rows = []
+    for line in out.split("\n"):
+        m = re.match(r"\s+(\d+)\s+" + num + r"\s+\[\s*" + num + r"\s*\]\s+" + num + r"\s+" + num +
+                     r"\s+\[\s*" + num + r",\s*" + num + r",\s*" + num + r"\s*\]", line)
+        if m:
+            rows.append([int(m.group(1))] + [float(m.group(i)) for i in range(2, 9)])
+    with capsys.disabled():
+        for r, ref in zip(rows, rec):
+            say(f"  {r}\n  recorded {[ref['iter'], ref['mass_error'], ref['Q'], ref['H'], ref['error']] + ref['lx']}")
+    assert len(rows) == len(rec)
+    for r, ref in zip(rows, rec):
+        assert r[0] == ref["iter"]
+        assert abs(r[1]) < 1e-12
+        assert abs(r[2] / ref["Q"] - 1) < 1e-6
+        assert abs(r[3] - ref["H"]) < 2e-9
+        assert abs(r[4] / ref["error"] - 1) < 1e-6
+        np.testing.assert_allclose(r[5:8], ref["lx"], rtol=0, atol=2e-7)
+
+
+def test_scft_iterations_match_oracle(capsys):
+    """Fixed box, discrete chain: each iteration's fields against the oracle solver
+    with the reference's residual, error and Anderson mixing restated in numpy."""
+    from langevin_fts_amd import scft
+
+    nx, lx = [16, 12, 10], [3.2, 2.9, 2.4]
+    say(f"scft vs oracle {nx}")
+    p = params(nx, lx, False, 8, f=0.5, model="discrete", ds=1 / 90, chi_n=15.0)
+    rng = np.random.default_rng(3)
+    w0 = {"A": rng.standard_normal(nx), "B": rng.standard_normal(nx)}
+    calc = scft.SCFT(p)
+    calc.run(initial_fields=w0)
+    capsys.readouterr()
+
+    plan = golden("plans")["diblock_f0.5_ds1_90|discrete|1"]
+    inp = plan["input"]
+    M = int(np.prod(nx))
+    grid = O.Grid(nx, lx)
+    solver = O.Solver(grid, plan, inp["ds"], inp["bonds"])
+    am = O.AndersonMixing(2 * M, 20, 1e-2, 0.1, 0.1)
+    chi = np.array([[0.0, 15.0], [15.0, 0.0]])
+    P = np.identity(2) - np.ones((2, 2)) @ np.linalg.inv(chi) / np.sum(np.linalg.inv(chi))
+    w = np.stack([w0["A"].ravel(), w0["B"].ravel()])
+    V = grid.volume()
+    for i in range(2):
+        w[i] -= grid.integral(w[i]) / V
+    err = 1e20
+    for it in range(8):
+        solver.compute_statistics({"A": w[0], "B": w[1]})
+        phi = np.stack([solver.get_total_concentration("A"), solver.get_total_concentration("B")])
+        wd = chi @ phi - P @ w
+        for i in range(2):
+            wd[i] -= grid.integral(wd[i]) / V
+        old, err = err, np.sqrt(sum(grid.inner_product(wd[i], wd[i]) for i in range(2)) /
+                                (1.0 + sum(grid.inner_product(w[i], w[i]) for i in range(2))))
+        w = np.reshape(am.calculate_new_fields(w.ravel(), wd.ravel(), old, err), (2, M))
+        for i in range(2):
+            w[i] -= grid.integral(w[i]) / V
+    np.testing.assert_allclose(calc.w, w, rtol=0, atol=1e-9)
+    np.testing.assert_allclose(calc.phi["A"], phi[0], rtol=0, atol=1e-9)
