@@ -21,6 +21,7 @@ typedef struct fts_grid fts_grid;
 typedef struct fts_solver fts_solver;
 typedef struct fts_am fts_am;
 typedef struct fts_saddle fts_saddle;
+typedef struct fts_cnn fts_cnn;
 
 enum { FTS_OK = 0, FTS_ERROR = 1 };
 enum { FTS_CONTINUOUS = 0, FTS_DISCRETE = 1 };
@@ -36,6 +37,8 @@ typedef struct fts_kernel_stats {
     double bytes;          /* summed algorithmic bytes: each field, spectrum and w read or written once */
     long long launches;
 } fts_kernel_stats;
+/* hidden channels of the learned-guess network */
+#define FTS_CNN_CHANNELS 32
 /* most monomer types of a field theory handled by fts_saddle */
 #define FTS_MAX_TYPES 8
 
@@ -221,6 +224,31 @@ int fts_saddle_find(fts_saddle* s, double tolerance, int max_iter, int verbose, 
 int fts_saddle_langevin_update(fts_saddle* s, const double* noise, int on_device, double dt);
 /* plain means of w_aux[j] and w_aux[j]^2, S each (lfts.py:250-262 dH/dchiN terms) */
 int fts_saddle_field_moments(fts_saddle* s, double* mean, double* mean_sq);
+
+/* ---- Learned initial guess of the saddle point (no counterpart in the
+ *      reference, whose drivers start each saddle point from the previous step's
+ *      w+, examples/fts/lfts.py:701-717). A periodic 3D CNN maps w- to w+:
+ *      conv(1->32) ReLU, 4 x [conv(32->32) ReLU], conv(32->1), 3x3x3 kernels with
+ *      circular padding; bf16 weights and activations, FP32 accumulation (MFMA);
+ *      FP32 master weights trained with Adam. Inputs are divided by in_scale and
+ *      outputs multiplied by out_scale. The grid size must be a multiple of 32. */
+int fts_cnn_n_params(void);
+int fts_cnn_create(fts_context* ctx, const int* nx, double in_scale, double out_scale, fts_cnn** out);
+int fts_cnn_destroy(fts_cnn* c);
+/* FP32 master parameters, fts_cnn_n_params() floats, host memory; setting resets Adam */
+int fts_cnn_set_params(fts_cnn* c, const float* params);
+int fts_cnn_get_params(fts_cnn* c, float* params);
+int fts_cnn_predict(fts_cnn* c, const double* w_minus, double* w_plus, int on_device);
+/* loss = grid mean of (f(w- / in_scale) - w+ / out_scale)^2; grads (host, n_params floats) may be null */
+int fts_cnn_gradients(fts_cnn* c, const double* w_minus, const double* w_plus_target, int on_device, double* loss,
+                      float* grads);
+/* one Adam step (beta 0.9 / 0.999, eps 1e-8) on one (w-, w+) pair */
+int fts_cnn_train_step(fts_cnn* c, const double* w_minus, const double* w_plus_target, int on_device, double lr,
+                       double* loss);
+/* AB field theories (one real, one imaginary field): w_aux[imag] = f(w_aux[real]) on the device,
+ * and one training step on the current (w_aux[real], w_aux[imag]) pair */
+int fts_saddle_predict_guess(fts_saddle* s, fts_cnn* c);
+int fts_saddle_train_guess(fts_saddle* s, fts_cnn* c, double lr, double* loss);
 
 #ifdef __cplusplus
 }

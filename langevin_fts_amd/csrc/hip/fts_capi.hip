@@ -453,4 +453,94 @@ int fts_saddle_field_moments(fts_saddle* s, double* mean, double* mean_sq) {
     });
 }
 
+int fts_cnn_n_params(void) { return fts::cnn_n_params(); }
+
+int fts_cnn_create(fts_context* ctx, const int* nx, double in_scale, double out_scale, fts_cnn** out) {
+    return guarded([&] {
+        need(ctx, "ctx");
+        need(nx, "nx");
+        need(out, "out");
+        FTS_HIP_CHECK(hipSetDevice(ctx->device));
+        auto* c = new fts_cnn;
+        try {
+            c->c = std::make_unique<fts::Cnn>(ctx, nx[0], nx[1], nx[2], in_scale, out_scale);
+        } catch (...) {
+            delete c;
+            throw;
+        }
+        *out = c;
+    });
+}
+
+int fts_cnn_destroy(fts_cnn* c) {
+    return guarded([&] { delete c; });
+}
+
+int fts_cnn_set_params(fts_cnn* c, const float* params) {
+    return guarded([&] {
+        need(c, "cnn");
+        need(params, "params");
+        c->c->set_params(params);
+    });
+}
+
+int fts_cnn_get_params(fts_cnn* c, float* params) {
+    return guarded([&] {
+        need(c, "cnn");
+        need(params, "params");
+        c->c->get_params(params);
+    });
+}
+
+int fts_cnn_predict(fts_cnn* c, const double* w_minus, double* w_plus, int on_device) {
+    return guarded([&] {
+        need(c, "cnn");
+        need(w_minus, "w_minus");
+        need(w_plus, "w_plus");
+        c->c->predict(w_minus, w_plus, on_device != 0);
+    });
+}
+
+int fts_cnn_gradients(fts_cnn* c, const double* w_minus, const double* w_plus_target, int on_device, double* loss,
+                      float* grads) {
+    return guarded([&] {
+        need(c, "cnn");
+        need(w_minus, "w_minus");
+        need(w_plus_target, "w_plus_target");
+        const double l = c->c->gradients(w_minus, w_plus_target, on_device != 0);
+        if (loss) *loss = l;
+        if (grads) c->c->get_grads(grads);
+    });
+}
+
+int fts_cnn_train_step(fts_cnn* c, const double* w_minus, const double* w_plus_target, int on_device, double lr,
+                       double* loss) {
+    return guarded([&] {
+        need(c, "cnn");
+        need(w_minus, "w_minus");
+        need(w_plus_target, "w_plus_target");
+        FTS_REQUIRE(lr > 0, "learning rate must be positive");
+        const double l = c->c->train_step(w_minus, w_plus_target, on_device != 0, lr);
+        if (loss) *loss = l;
+    });
+}
+
+int fts_saddle_predict_guess(fts_saddle* s, fts_cnn* c) {
+    return guarded([&] {
+        need(s, "saddle");
+        need(c, "cnn");
+        s->s->predict_guess(*c->c);
+    });
+}
+
+int fts_saddle_train_guess(fts_saddle* s, fts_cnn* c, double lr, double* loss) {
+    return guarded([&] {
+        need(s, "saddle");
+        need(c, "cnn");
+        FTS_REQUIRE(lr > 0, "learning rate must be positive");
+        const double l = s->s->train_guess(*c->c, lr);
+        if (loss) *loss = l;
+    });
+}
+
 }  // extern "C"

@@ -1,0 +1,460 @@
+// Periodic 3D CNN that predicts the L-FTS pressure field w+ from the exchange
+// field w- (the learned initial guess of the saddle-point iteration; it has no
+// counterpart in the reference, whose drivers start each saddle point from the
+// previous step's w+, examples/fts/lfts.py:701-717).
+//
+// Network: conv(1->C) ReLU, 4 x [conv(C->C) ReLU], conv(C->1); 3x3x3 kernels with
+// circular padding, C = 32. Hidden activations are bf16, channels-last [voxel][C];
+// every product accumulates in FP32.
+//
+//   hidden layers    implicit GEMM on v_mfma_f32_32x32x16_bf16: one wavefront owns
+//                    32 voxels x 32 output channels and accumulates 27 taps x 2
+//                    input-channel chunks of 16
+//   first / last     FP32 FMA (1 input or 1 output channel)
+//   backward         data gradients: the same MFMA kernel with flipped, transposed
+//                    bf16 weights and the ReLU mask of the layer input; hidden weight
+//                    gradients: MFMA GEMMs over voxels, split over voxel chunks and
+//                    reduced in a fixed order
+//   optimizer        Adam on FP32 master weights, then bf16 copies for the kernels
+//
+// Tap t = 9 (dx + 1) + 3 (dy + 1) + (dz + 1); the flipped tap is 26 - t.
+#include <cmath>
+#include <vector>
+
+#include "fts_internal.hpp"
+#include "fts_reduce.hpp"
+
+namespace fts {
+
+namespace {
+
+constexpr int C = FTS_CNN_CHANNELS;
+constexpr int TAPS = 27;
+constexpr int NHID = 4;                                 // hidden C->C layers
+constexpr int P_W0 = 0;                                 // [27][C]
+constexpr int P_B0 = P_W0 + TAPS * C;                   // [C]
+constexpr int P_WH = P_B0 + C;                          // NHID x ([27][C co][C ci] + [C])
+constexpr int HSZ = TAPS * C * C + C;
+constexpr int P_WL = P_WH + NHID * HSZ;                 // [27][C]
+constexpr int P_BL = P_WL + TAPS * C;                   // [1]
+constexpr int N_PARAMS = P_BL + 1;
+constexpr int WG_CHUNK = 1024;                          // voxels per block in the weight-gradient kernels
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+struct Geo {
+    int nx, ny, nz, M;
+};
+
+__device__ __forceinline__ int neighbour(const Geo& g, int v, int t) {
+    const int z = v % g.nz, y = (v / g.nz) % g.ny, x = v / (g.nz * g.ny);
+    int xx = x + t / 9 - 1, yy = y + (t / 3) % 3 - 1, zz = z + t % 3 - 1;
+    xx = xx < 0 ? xx + g.nx : (xx >= g.nx ? xx - g.nx : xx);
+    yy = yy < 0 ? yy + g.ny : (yy >= g.ny ? yy - g.ny : yy);
+    zz = zz < 0 ? zz + g.nz : (zz >= g.nz ? zz - g.nz : zz);
+    return (xx * g.ny + yy) * g.nz + zz;
+}
+
+// ---------------------------------------------------------------- forward
+__global__ void __launch_bounds__(256) conv_first(Geo g, const float* x, const float* W, const float* b, __bf16* out) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= g.M) return;
+    float acc[C];
+#pragma unroll
+    for (int c = 0; c < C; c++) acc[c] = b[c];
+    for (int t = 0; t < TAPS; t++) {
+        const float xv = x[neighbour(g, v, t)];
+#pragma unroll
+        for (int c = 0; c < C; c++) acc[c] += W[t * C + c] * xv;
+    }
+    bf16x8* o = reinterpret_cast<bf16x8*>(out + (size_t)v * C);
+#pragma unroll
+    for (int q = 0; q < C / 8; q++) {
+        bf16x8 h;
+#pragma unroll
+        for (int e = 0; e < 8; e++) h[e] = (__bf16)fmaxf(acc[q * 8 + e], 0.f);
+        o[q] = h;
+    }
+}
+
+// Y[v][co] = act(bias[co] + sum_t sum_ci W[t][co][ci] X[nbr(v,t)][ci]) (x [X_in[v][co] > 0] when
+// mask is given: the ReLU derivative of a backward pass). M % 32 == 0.
+__global__ void __launch_bounds__(256) conv_hidden(Geo g, const __bf16* X, const __bf16* W, const float* bias,
+                                                   const __bf16* mask, int relu, __bf16* Y) {
+    const int lane = threadIdx.x & 63;
+    const int v0 = (blockIdx.x * 4 + (threadIdx.x >> 6)) * 32;
+    if (v0 >= g.M) return;
+    const int half = lane >> 5, col = lane & 31;
+    const int vi = v0 + col;   // voxel of this lane's A rows
+    f32x16 acc = {};
+    for (int t = 0; t < TAPS; t++) {
+        const __bf16* xa = X + (size_t)neighbour(g, vi, t) * C + half * 8;
+        const __bf16* wb = W + ((size_t)t * C + col) * C + half * 8;
+#pragma unroll
+        for (int ch = 0; ch < 2; ch++) {
+            const bf16x8 a = *reinterpret_cast<const bf16x8*>(xa + ch * 16);
+            const bf16x8 b = *reinterpret_cast<const bf16x8*>(wb + ch * 16);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+        }
+    }
+    // D[i][j]: j = lane % 32 (output channel), i = 8 (r / 4) + 4 (lane / 32) + r % 4 (voxel)
+    const float bj = bias ? bias[col] : 0.f;
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        const int v = v0 + 8 * (r >> 2) + 4 * half + (r & 3);
+        float val = acc[r] + bj;
+        if (relu) val = fmaxf(val, 0.f);
+        if (mask && !((float)mask[(size_t)v * C + col] > 0.f)) val = 0.f;
+        Y[(size_t)v * C + col] = (__bf16)val;
+    }
+}
+
+__global__ void __launch_bounds__(256) conv_last(Geo g, const __bf16* X, const float* W, const float* b, float* y) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= g.M) return;
+    float acc = b[0];
+    for (int t = 0; t < TAPS; t++) {
+        const bf16x8* xv = reinterpret_cast<const bf16x8*>(X + (size_t)neighbour(g, v, t) * C);
+#pragma unroll
+        for (int q = 0; q < C / 8; q++) {
+            const bf16x8 h = xv[q];
+#pragma unroll
+            for (int e = 0; e < 8; e++) acc += W[t * C + q * 8 + e] * (float)h[e];
+        }
+    }
+    y[v] = acc;
+}
+
+// ---------------------------------------------------------------- loss
+// d = 2 (y - target) / M, partial sums of (y - target)^2
+__global__ void __launch_bounds__(256) loss_grad(Geo g, const float* y, const float* target, float* dy,
+                                                 double* partials) {
+    __shared__ double ws[16];
+    double acc = 0.0;
+    for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < g.M; v += gridDim.x * blockDim.x) {
+        const float d = y[v] - target[v];
+        dy[v] = 2.f * d / (float)g.M;
+        acc += (double)d * d;
+    }
+    acc = block_sum(acc, ws);
+    if (threadIdx.x == 0) partials[blockIdx.x] = acc;
+}
+
+__global__ void __launch_bounds__(256) sum_partials(const double* partials, int n, double* out) {
+    __shared__ double ws[16];
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) acc += partials[i];
+    acc = block_sum(acc, ws);
+    if (threadIdx.x == 0) out[0] = acc;
+}
+
+// ---------------------------------------------------------------- backward
+// dX[u][c] = [X[u][c] > 0] sum_t W[t][c] dy[nbr(u, 26 - t)]  (last layer, 1 output channel)
+__global__ void __launch_bounds__(256) back_last(Geo g, const float* dy, const float* W, const __bf16* X, __bf16* dX) {
+    const int u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= g.M) return;
+    float acc[C] = {};
+    for (int t = 0; t < TAPS; t++) {
+        const float d = dy[neighbour(g, u, 26 - t)];
+#pragma unroll
+        for (int c = 0; c < C; c++) acc[c] += W[t * C + c] * d;
+    }
+    const bf16x8* xv = reinterpret_cast<const bf16x8*>(X + (size_t)u * C);
+    bf16x8* o = reinterpret_cast<bf16x8*>(dX + (size_t)u * C);
+#pragma unroll
+    for (int q = 0; q < C / 8; q++) {
+        const bf16x8 h = xv[q];
+        bf16x8 r;
+#pragma unroll
+        for (int e = 0; e < 8; e++) r[e] = (__bf16)((float)h[e] > 0.f ? acc[q * 8 + e] : 0.f);
+        o[q] = r;
+    }
+}
+
+// Hidden weight gradient, per tap and voxel chunk: G[t][co][ci] = sum_v dZ[v][co] X[nbr(v,t)][ci]
+// as D = A B with A = dZ^T (32 x voxels), B = X_t (voxels x 32). One wavefront sums WG_CHUNK / 4
+// voxels; the block's 4 wavefronts meet in LDS; partial[t][chunk][co][ci].
+__global__ void __launch_bounds__(256) wgrad_hidden(Geo g, const __bf16* dZ, const __bf16* X, float* partial) {
+    __shared__ float red[4][C * C];
+    const int t = blockIdx.y;
+    const int chunk = blockIdx.x;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int half = lane >> 5, col = lane & 31;
+    const int vbeg = chunk * WG_CHUNK + wave * (WG_CHUNK / 4);
+    const int vend = min(vbeg + WG_CHUNK / 4, g.M);
+    f32x16 acc = {};
+    for (int v0 = vbeg; v0 < vend; v0 += 16) {
+        bf16x8 a, b;
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            const int v = v0 + half * 8 + e;
+            const bool ok = v < vend;
+            a[e] = ok ? dZ[(size_t)v * C + col] : (__bf16)0.f;
+            b[e] = ok ? X[(size_t)neighbour(g, v, t) * C + col] : (__bf16)0.f;
+        }
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+    }
+    // D[i = co][j = ci]: j = lane % 32, i = 8 (r / 4) + 4 (lane / 32) + r % 4
+#pragma unroll
+    for (int r = 0; r < 16; r++) red[wave][(8 * (r >> 2) + 4 * half + (r & 3)) * C + col] = acc[r];
+    __syncthreads();
+    float* out = partial + ((size_t)t * gridDim.x + chunk) * C * C;
+    for (int i = threadIdx.x; i < C * C; i += blockDim.x) out[i] = ((red[0][i] + red[1][i]) + red[2][i]) + red[3][i];
+}
+
+// First layer: G[t][c] = sum_v dZ[v][c] x[nbr(v,t)]; last layer: G[t][c] = sum_v dy[v] X[nbr(v,t)][c].
+// Thread (t, c) of the block sums the chunk's voxels; partial[chunk][t][c].
+__global__ void __launch_bounds__(TAPS * C) wgrad_edge(Geo g, int first, const __bf16* H, const float* s,
+                                                       float* partial) {
+    const int t = threadIdx.x / C, c = threadIdx.x % C;
+    const int vbeg = blockIdx.x * WG_CHUNK, vend = min(vbeg + WG_CHUNK, g.M);
+    float acc = 0.f;
+    if (first) {
+        for (int v = vbeg; v < vend; v++) acc += (float)H[(size_t)v * C + c] * s[neighbour(g, v, t)];
+    } else {
+        for (int v = vbeg; v < vend; v++) acc += s[v] * (float)H[(size_t)neighbour(g, v, t) * C + c];
+    }
+    partial[(size_t)blockIdx.x * TAPS * C + threadIdx.x] = acc;
+}
+
+// Bias gradients: channel sums of dZ (C channels) or of dy (1 channel); partial[chunk][c]
+__global__ void __launch_bounds__(256) bias_grad(Geo g, const __bf16* dZ, const float* dy, float* partial) {
+    __shared__ float red[256];
+    const int vbeg = blockIdx.x * WG_CHUNK, vend = min(vbeg + WG_CHUNK, g.M);
+    const int c = threadIdx.x % C, lane_group = threadIdx.x / C;   // 8 groups of 32
+    float acc = 0.f;
+    if (dZ) {
+        for (int v = vbeg + lane_group; v < vend; v += 8) acc += (float)dZ[(size_t)v * C + c];
+    } else if (c == 0) {
+        for (int v = vbeg + lane_group; v < vend; v += 8) acc += dy[v];
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    if (threadIdx.x < C) {
+        float s = 0.f;
+        for (int k = 0; k < 8; k++) s += red[k * C + threadIdx.x];
+        partial[(size_t)blockIdx.x * C + threadIdx.x] = s;
+    }
+}
+
+// grad[i] = sum over chunks of partial[chunk * stride + i], i < n (fixed order)
+__global__ void __launch_bounds__(256) reduce_chunks(const float* partial, int n_chunks, int stride, int n,
+                                                     float* grad) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float s = 0.f;
+    for (int k = 0; k < n_chunks; k++) s += partial[(size_t)k * stride + i];
+    grad[i] = s;
+}
+
+// ---------------------------------------------------------------- parameters
+__global__ void __launch_bounds__(256) adam_step(float* p, const float* gr, float* m, float* v, int n, float lr,
+                                                 float b1, float b2, float eps, float c1, float c2) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = gr[i];
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+// bf16 copies of the hidden weights: forward [t][co][ci] and the backward form
+// [t'][ci][co] with t' = 26 - t
+__global__ void __launch_bounds__(256) hidden_weights(const float* p, __bf16* wf, __bf16* wb) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= NHID * TAPS * C * C) return;
+    const int l = i / (TAPS * C * C), r = i % (TAPS * C * C);
+    const int t = r / (C * C), co = (r / C) % C, ci = r % C;
+    const __bf16 w = (__bf16)p[P_WH + l * HSZ + r];
+    wf[i] = w;
+    wb[(size_t)l * TAPS * C * C + ((size_t)(26 - t) * C + ci) * C + co] = w;
+}
+
+__global__ void __launch_bounds__(256) to_float(const double* in, double scale, int n, float* out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = (float)(in[i] * scale);
+}
+
+__global__ void __launch_bounds__(256) to_double(const float* in, double scale, int n, double* out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = (double)in[i] * scale;
+}
+
+int blocks(int n, int b = 256) { return (n + b - 1) / b; }
+
+}  // namespace
+
+int cnn_n_params() { return N_PARAMS; }
+
+Cnn::Cnn(fts_context* ctx, int nx, int ny, int nz, double in_scale, double out_scale)
+    : ctx_(ctx), nx_(nx), ny_(ny), nz_(nz), in_scale_(in_scale), out_scale_(out_scale) {
+    FTS_REQUIRE(nx >= 3 && ny >= 3 && nz >= 3, "the CNN needs at least 3 points per axis");
+    M_ = nx * ny * nz;
+    FTS_REQUIRE(M_ % 32 == 0, "the CNN needs a grid size that is a multiple of 32");
+    FTS_REQUIRE(in_scale > 0 && out_scale > 0, "scales must be positive");
+    n_chunks_ = (M_ + WG_CHUNK - 1) / WG_CHUNK;
+    params_.allocate(sizeof(float) * N_PARAMS);
+    grads_.allocate(sizeof(float) * N_PARAMS);
+    adam_m_.allocate(sizeof(float) * N_PARAMS);
+    adam_v_.allocate(sizeof(float) * N_PARAMS);
+    wf_.allocate(sizeof(__bf16) * NHID * TAPS * C * C);
+    wb_.allocate(sizeof(__bf16) * NHID * TAPS * C * C);
+    x_.allocate(sizeof(float) * M_);
+    target_.allocate(sizeof(float) * M_);
+    y_.allocate(sizeof(float) * M_);
+    dy_.allocate(sizeof(float) * M_);
+    acts_.allocate(sizeof(__bf16) * (size_t)(NHID + 1) * M_ * C);
+    grad_a_.allocate(sizeof(__bf16) * (size_t)M_ * C);
+    grad_b_.allocate(sizeof(__bf16) * (size_t)M_ * C);
+    partial_.allocate(sizeof(float) * (size_t)std::max(TAPS * n_chunks_ * C * C, n_chunks_ * TAPS * C));
+    loss_partials_.allocate(sizeof(double) * 256);
+    loss_.allocate(sizeof(double));
+    stage_in_.allocate(sizeof(double) * M_);
+    stage_out_.allocate(sizeof(double) * M_);
+    std::vector<float> zero(N_PARAMS, 0.f);
+    set_params(zero.data());
+}
+
+void Cnn::set_params(const float* host) {
+    hipStream_t st = ctx_->stream;
+    FTS_HIP_CHECK(hipMemcpyAsync(params_.get(), host, sizeof(float) * N_PARAMS, hipMemcpyHostToDevice, st));
+    FTS_HIP_CHECK(hipMemsetAsync(adam_m_.get(), 0, sizeof(float) * N_PARAMS, st));
+    FTS_HIP_CHECK(hipMemsetAsync(adam_v_.get(), 0, sizeof(float) * N_PARAMS, st));
+    step_ = 0;
+    refresh_weights();
+    FTS_HIP_CHECK(hipStreamSynchronize(st));
+}
+
+void Cnn::get_params(float* host) {
+    FTS_HIP_CHECK(hipMemcpyAsync(host, params_.get(), sizeof(float) * N_PARAMS, hipMemcpyDeviceToHost, ctx_->stream));
+    FTS_HIP_CHECK(hipStreamSynchronize(ctx_->stream));
+}
+
+void Cnn::get_grads(float* host) {
+    FTS_HIP_CHECK(hipMemcpyAsync(host, grads_.get(), sizeof(float) * N_PARAMS, hipMemcpyDeviceToHost, ctx_->stream));
+    FTS_HIP_CHECK(hipStreamSynchronize(ctx_->stream));
+}
+
+void Cnn::refresh_weights() {
+    hipLaunchKernelGGL(hidden_weights, dim3(blocks(NHID * TAPS * C * C)), dim3(256), 0, ctx_->stream,
+                       params_.as<float>(), wf_.as<__bf16>(), wb_.as<__bf16>());
+    FTS_HIP_CHECK(hipGetLastError());
+}
+
+void Cnn::load_input(const double* w_minus, bool on_device) {
+    hipStream_t st = ctx_->stream;
+    const double* src = w_minus;
+    if (!on_device) {
+        FTS_HIP_CHECK(hipMemcpyAsync(stage_in_.get(), w_minus, sizeof(double) * M_, hipMemcpyHostToDevice, st));
+        src = stage_in_.as<double>();
+    }
+    hipLaunchKernelGGL(to_float, dim3(blocks(M_)), dim3(256), 0, st, src, 1.0 / in_scale_, M_, x_.as<float>());
+    FTS_HIP_CHECK(hipGetLastError());
+}
+
+void Cnn::forward() {
+    hipStream_t st = ctx_->stream;
+    const Geo g{nx_, ny_, nz_, M_};
+    const float* p = params_.as<float>();
+    __bf16* a = acts_.as<__bf16>();
+    const size_t AS = (size_t)M_ * C;
+    hipLaunchKernelGGL(conv_first, dim3(blocks(M_)), dim3(256), 0, st, g, x_.as<float>(), p + P_W0, p + P_B0, a);
+    for (int l = 0; l < NHID; l++)
+        hipLaunchKernelGGL(conv_hidden, dim3(blocks(M_ / 32, 4)), dim3(256), 0, st, g, a + l * AS,
+                           wf_.as<__bf16>() + (size_t)l * TAPS * C * C, p + P_WH + l * HSZ + TAPS * C * C,
+                           (const __bf16*)nullptr, 1, a + (l + 1) * AS);
+    hipLaunchKernelGGL(conv_last, dim3(blocks(M_)), dim3(256), 0, st, g, a + NHID * AS, p + P_WL, p + P_BL,
+                       y_.as<float>());
+    FTS_HIP_CHECK(hipGetLastError());
+}
+
+void Cnn::predict(const double* w_minus, double* w_plus, bool on_device) {
+    load_input(w_minus, on_device);
+    forward();
+    hipStream_t st = ctx_->stream;
+    if (on_device) {
+        hipLaunchKernelGGL(to_double, dim3(blocks(M_)), dim3(256), 0, st, y_.as<float>(), out_scale_, M_, w_plus);
+        FTS_HIP_CHECK(hipGetLastError());
+    } else {
+        hipLaunchKernelGGL(to_double, dim3(blocks(M_)), dim3(256), 0, st, y_.as<float>(), out_scale_, M_,
+                           stage_out_.as<double>());
+        FTS_HIP_CHECK(hipGetLastError());
+        FTS_HIP_CHECK(hipMemcpyAsync(w_plus, stage_out_.get(), sizeof(double) * M_, hipMemcpyDeviceToHost, st));
+    }
+    FTS_HIP_CHECK(hipStreamSynchronize(st));
+}
+
+double Cnn::gradients(const double* w_minus, const double* w_plus_target, bool on_device) {
+    hipStream_t st = ctx_->stream;
+    const Geo g{nx_, ny_, nz_, M_};
+    load_input(w_minus, on_device);
+    const double* tsrc = w_plus_target;
+    if (!on_device) {
+        FTS_HIP_CHECK(hipMemcpyAsync(stage_out_.get(), w_plus_target, sizeof(double) * M_, hipMemcpyHostToDevice, st));
+        tsrc = stage_out_.as<double>();
+    }
+    hipLaunchKernelGGL(to_float, dim3(blocks(M_)), dim3(256), 0, st, tsrc, 1.0 / out_scale_, M_, target_.as<float>());
+    forward();
+    const int lb = std::min(256, blocks(M_));
+    hipLaunchKernelGGL(loss_grad, dim3(lb), dim3(256), 0, st, g, y_.as<float>(), target_.as<float>(), dy_.as<float>(),
+                       loss_partials_.as<double>());
+    hipLaunchKernelGGL(sum_partials, dim3(1), dim3(256), 0, st, loss_partials_.as<double>(), lb, loss_.as<double>());
+
+    const float* p = params_.as<float>();
+    float* gr = grads_.as<float>();
+    float* part = partial_.as<float>();
+    __bf16* a = acts_.as<__bf16>();
+    const size_t AS = (size_t)M_ * C;
+    // last layer: weights, bias, then dZ of the last hidden activation
+    hipLaunchKernelGGL(wgrad_edge, dim3(n_chunks_), dim3(TAPS * C), 0, st, g, 0, a + NHID * AS, dy_.as<float>(), part);
+    hipLaunchKernelGGL(reduce_chunks, dim3(blocks(TAPS * C)), dim3(256), 0, st, part, n_chunks_, TAPS * C, TAPS * C,
+                       gr + P_WL);
+    hipLaunchKernelGGL(bias_grad, dim3(n_chunks_), dim3(256), 0, st, g, (const __bf16*)nullptr, dy_.as<float>(), part);
+    hipLaunchKernelGGL(reduce_chunks, dim3(1), dim3(256), 0, st, part, n_chunks_, C, 1, gr + P_BL);
+    hipLaunchKernelGGL(back_last, dim3(blocks(M_)), dim3(256), 0, st, g, dy_.as<float>(), p + P_WL, a + NHID * AS,
+                       grad_a_.as<__bf16>());
+    __bf16* dz = grad_a_.as<__bf16>();
+    __bf16* other = grad_b_.as<__bf16>();
+    for (int l = NHID - 1; l >= 0; l--) {
+        // layer l maps a[l] -> a[l + 1]; dz = dL/d(pre-activation of a[l + 1])
+        hipLaunchKernelGGL(wgrad_hidden, dim3(n_chunks_, TAPS), dim3(256), 0, st, g, dz, a + l * AS, part);
+        hipLaunchKernelGGL(reduce_chunks, dim3(blocks(TAPS * C * C)), dim3(256), 0, st, part, n_chunks_,
+                           TAPS * C * C, TAPS * C * C, gr + P_WH + l * HSZ);
+        hipLaunchKernelGGL(bias_grad, dim3(n_chunks_), dim3(256), 0, st, g, dz, (const float*)nullptr, part);
+        hipLaunchKernelGGL(reduce_chunks, dim3(1), dim3(256), 0, st, part, n_chunks_, C, C,
+                           gr + P_WH + l * HSZ + TAPS * C * C);
+        hipLaunchKernelGGL(conv_hidden, dim3(blocks(M_ / 32, 4)), dim3(256), 0, st, g, dz,
+                           wb_.as<__bf16>() + (size_t)l * TAPS * C * C, (const float*)nullptr, a + l * AS, 0, other);
+        std::swap(dz, other);
+    }
+    // first layer
+    hipLaunchKernelGGL(wgrad_edge, dim3(n_chunks_), dim3(TAPS * C), 0, st, g, 1, dz, x_.as<float>(), part);
+    hipLaunchKernelGGL(reduce_chunks, dim3(blocks(TAPS * C)), dim3(256), 0, st, part, n_chunks_, TAPS * C, TAPS * C,
+                       gr + P_W0);
+    hipLaunchKernelGGL(bias_grad, dim3(n_chunks_), dim3(256), 0, st, g, dz, (const float*)nullptr, part);
+    hipLaunchKernelGGL(reduce_chunks, dim3(1), dim3(256), 0, st, part, n_chunks_, C, C, gr + P_B0);
+    FTS_HIP_CHECK(hipGetLastError());
+    double loss = 0.0;
+    FTS_HIP_CHECK(hipMemcpyAsync(&loss, loss_.get(), sizeof(double), hipMemcpyDeviceToHost, st));
+    FTS_HIP_CHECK(hipStreamSynchronize(st));
+    return loss / M_;
+}
+
+double Cnn::train_step(const double* w_minus, const double* w_plus_target, bool on_device, double lr) {
+    const double loss = gradients(w_minus, w_plus_target, on_device);
+    step_++;
+    const float b1 = 0.9f, b2 = 0.999f;
+    const float c1 = 1.f - std::pow(b1, (float)step_), c2 = 1.f - std::pow(b2, (float)step_);
+    hipLaunchKernelGGL(adam_step, dim3(blocks(N_PARAMS)), dim3(256), 0, ctx_->stream, params_.as<float>(),
+                       grads_.as<float>(), adam_m_.as<float>(), adam_v_.as<float>(), N_PARAMS, (float)lr, b1, b2,
+                       1e-8f, c1, c2);
+    FTS_HIP_CHECK(hipGetLastError());
+    refresh_weights();
+    FTS_HIP_CHECK(hipStreamSynchronize(ctx_->stream));
+    return loss;
+}
+
+}  // namespace fts

@@ -179,6 +179,8 @@ private:
     int dots_start_ = 0, dots_n_ = 0;
 };
 
+class Cnn;
+
 class SaddlePoint {
 public:
     SaddlePoint(fts_context* ctx, SolverBase* solver, AndersonMixingDevice* am, int M, const fts_field_theory& ft);
@@ -193,6 +195,9 @@ public:
               double* hamiltonian);
     void langevin_update(const double* noise, bool on_device, double dt);
     void field_moments(double* mean, double* mean_sq);
+    // learned guess: w_aux[imag] = f(w_aux[real]); one training step on the current pair
+    void predict_guess(Cnn& cnn);
+    double train_guess(Cnn& cnn, double lr);
 
 private:
     void concentrations();
@@ -217,6 +222,33 @@ private:
         partials_, stats_;
 };
 
+int cnn_n_params();
+
+// Periodic 3D CNN of the learned saddle-point guess (fts_cnn.hip)
+class Cnn {
+public:
+    Cnn(fts_context* ctx, int nx, int ny, int nz, double in_scale, double out_scale);
+    void set_params(const float* host);
+    void get_params(float* host);
+    void get_grads(float* host);
+    void predict(const double* w_minus, double* w_plus, bool on_device);
+    // loss = mean (f(w-) - w+)^2 in scaled units; gradients into the device grads
+    double gradients(const double* w_minus, const double* w_plus_target, bool on_device);
+    double train_step(const double* w_minus, const double* w_plus_target, bool on_device, double lr);
+    int n_grid() const { return M_; }
+
+private:
+    void load_input(const double* w_minus, bool on_device);
+    void forward();
+    void refresh_weights();
+    fts_context* ctx_;
+    int nx_, ny_, nz_, M_, n_chunks_;
+    double in_scale_, out_scale_;
+    long long step_ = 0;
+    DeviceBuffer params_, grads_, adam_m_, adam_v_, wf_, wb_, x_, target_, y_, dy_, acts_, grad_a_, grad_b_, partial_,
+        loss_partials_, loss_, stage_in_, stage_out_;
+};
+
 }  // namespace fts
 
 struct fts_grid {
@@ -230,4 +262,7 @@ struct fts_am {
 };
 struct fts_saddle {
     std::unique_ptr<fts::SaddlePoint> s;
+};
+struct fts_cnn {
+    std::unique_ptr<fts::Cnn> c;
 };

@@ -428,5 +428,18 @@ void SaddlePoint::langevin_update(const double* noise, bool on_device, double dt
     FTS_HIP_CHECK(hipStreamSynchronize(st));
 }
 
-}  // namespace fts
+void SaddlePoint::predict_guess(Cnn& cnn) {
+    FTS_REQUIRE(R_ == 1 && I_ == 1, "the learned guess maps one real field to one imaginary field");
+    FTS_REQUIRE(cnn.n_grid() == M_, "the network's grid differs from the simulation grid");
+    cnn.predict(w_aux_.as<double>() + (size_t)real_idx_[0] * M_, w_aux_.as<double>() + (size_t)imag_idx_[0] * M_,
+                true);
+}
 
+double SaddlePoint::train_guess(Cnn& cnn, double lr) {
+    FTS_REQUIRE(R_ == 1 && I_ == 1, "the learned guess maps one real field to one imaginary field");
+    FTS_REQUIRE(cnn.n_grid() == M_, "the network's grid differs from the simulation grid");
+    return cnn.train_step(w_aux_.as<double>() + (size_t)real_idx_[0] * M_,
+                          w_aux_.as<double>() + (size_t)imag_idx_[0] * M_, true, lr);
+}
+
+}  // namespace fts

@@ -675,6 +675,38 @@ void DeviceSaddlePoint::field_moments(double* mean, double* mean_sq) {
     check_fts(fts_saddle_field_moments(saddle_, mean, mean_sq));
 }
 
+void DeviceSaddlePoint::predict_guess(LearnedGuess& g) { check_fts(fts_saddle_predict_guess(saddle_, g.handle())); }
+double DeviceSaddlePoint::train_guess(LearnedGuess& g, double lr) {
+    double loss = 0.0;
+    check_fts(fts_saddle_train_guess(saddle_, g.handle(), lr, &loss));
+    return loss;
+}
+
+// ------------------------------------------------------------------ learned guess
+LearnedGuess::LearnedGuess(std::vector<int> nx, double in_scale, double out_scale) {
+    if (nx.size() != 3) LFTS_THROW("the learned guess needs a 3D grid");
+    M_ = nx[0] * nx[1] * nx[2];
+    check_fts(fts_cnn_create(hip_context(), nx.data(), in_scale, out_scale, &cnn_));
+}
+LearnedGuess::~LearnedGuess() {
+    if (cnn_) fts_cnn_destroy(cnn_);
+}
+void LearnedGuess::set_params(const float* p) { check_fts(fts_cnn_set_params(cnn_, p)); }
+void LearnedGuess::get_params(float* p) { check_fts(fts_cnn_get_params(cnn_, p)); }
+void LearnedGuess::predict(const double* w_minus, double* w_plus) {
+    check_fts(fts_cnn_predict(cnn_, w_minus, w_plus, 0));
+}
+double LearnedGuess::gradients(const double* w_minus, const double* target, float* grads) {
+    double loss = 0.0;
+    check_fts(fts_cnn_gradients(cnn_, w_minus, target, 0, &loss, grads));
+    return loss;
+}
+double LearnedGuess::train_step(const double* w_minus, const double* target, double lr) {
+    double loss = 0.0;
+    check_fts(fts_cnn_train_step(cnn_, w_minus, target, 0, lr, &loss));
+    return loss;
+}
+
 // ------------------------------------------------------------------ factory
 ComputationBox* AbstractFactory::create_computation_box(std::vector<int> nx, std::vector<double> lx,
                                                         std::vector<std::string> bc, const double* mask) {

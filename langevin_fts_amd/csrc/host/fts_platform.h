@@ -145,6 +145,27 @@ struct FieldTheoryTables {
     double h_const = 0.0;
 };
 
+// Periodic 3D CNN of the learned saddle-point guess (w- -> w+); weights on the device.
+class LearnedGuess {
+public:
+    LearnedGuess(std::vector<int> nx, double in_scale, double out_scale);
+    ~LearnedGuess();
+    LearnedGuess(const LearnedGuess&) = delete;
+    LearnedGuess& operator=(const LearnedGuess&) = delete;
+    static int n_params() { return fts_cnn_n_params(); }
+    int n_grid() const { return M_; }
+    void set_params(const float* p);
+    void get_params(float* p);
+    void predict(const double* w_minus, double* w_plus);
+    double gradients(const double* w_minus, const double* target, float* grads);
+    double train_step(const double* w_minus, const double* target, double lr);
+    fts_cnn* handle() const { return cnn_; }
+
+private:
+    int M_;
+    fts_cnn* cnn_ = nullptr;
+};
+
 class DeviceSaddlePoint {
 public:
     DeviceSaddlePoint(PropagatorComputation* solver, AndersonMixing* am, const FieldTheoryTables& t);
@@ -167,6 +188,8 @@ public:
     void find(double tolerance, int max_iter, int verbose, double* log, int* n_iter, double* error, double* ham);
     void langevin_update(const double* noise, double dt);
     void field_moments(double* mean, double* mean_sq);
+    void predict_guess(LearnedGuess& g);
+    double train_guess(LearnedGuess& g, double lr);
 
 private:
     int S_, R_, I_, P_, M_;

@@ -339,10 +339,51 @@ PYBIND11_MODULE(langevinfts, m) {
                                 std::to_string(n));
                  if (n > 0) o.langevin_update(noise.data(), dt);
              })
-        .def("field_moments", [](DeviceSaddlePoint& o) {
-            std::vector<double> mean(o.n_types()), mean_sq(o.n_types());
-            o.field_moments(mean.data(), mean_sq.data());
-            return py::make_tuple(mean, mean_sq);
+        .def("field_moments",
+             [](DeviceSaddlePoint& o) {
+                 std::vector<double> mean(o.n_types()), mean_sq(o.n_types());
+                 o.field_moments(mean.data(), mean_sq.data());
+                 return py::make_tuple(mean, mean_sq);
+             })
+        .def("predict_guess", &DeviceSaddlePoint::predict_guess)
+        .def("train_guess", &DeviceSaddlePoint::train_guess, py::arg("guess"), py::arg("lr"));
+
+    // Learned initial guess of the saddle point: periodic 3D CNN w- -> w+ (an addition to
+    // the reference's surface).
+    using farr = py::array_t<float, py::array::c_style | py::array::forcecast>;
+    py::class_<LearnedGuess>(m, "LearnedGuess")
+        .def(py::init<std::vector<int>, double, double>(), py::arg("nx"), py::arg("in_scale") = 1.0,
+             py::arg("out_scale") = 1.0)
+        .def_static("n_params", &LearnedGuess::n_params)
+        .def("set_params",
+             [](LearnedGuess& o, const farr& p) {
+                 if (p.size() != LearnedGuess::n_params())
+                     LFTS_THROW("expected " + std::to_string(LearnedGuess::n_params()) + " parameters");
+                 o.set_params(p.data());
+             })
+        .def("get_params",
+             [](LearnedGuess& o) {
+                 farr p(LearnedGuess::n_params());
+                 o.get_params(p.mutable_data());
+                 return p;
+             })
+        .def("predict",
+             [](LearnedGuess& o, const darr& w) {
+                 if (w.size() != o.n_grid()) LFTS_THROW("w_minus must have n_grid values");
+                 py::array_t<double> out(o.n_grid());
+                 o.predict(w.data(), out.mutable_data());
+                 return out;
+             })
+        .def("gradients",
+             [](LearnedGuess& o, const darr& w, const darr& t) {
+                 if (w.size() != o.n_grid() || t.size() != o.n_grid()) LFTS_THROW("fields must have n_grid values");
+                 farr g(LearnedGuess::n_params());
+                 const double loss = o.gradients(w.data(), t.data(), g.mutable_data());
+                 return py::make_tuple(loss, g);
+             })
+        .def("train_step", [](LearnedGuess& o, const darr& w, const darr& t, double lr) {
+            if (w.size() != o.n_grid() || t.size() != o.n_grid()) LFTS_THROW("fields must have n_grid values");
+            return o.train_step(w.data(), t.data(), lr);
         });
 
     py::class_<AbstractFactory>(m, "AbstractFactory")

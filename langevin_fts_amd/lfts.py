@@ -204,8 +204,25 @@ class SymmetricPolymerTheory:
         return dH
 
 
+def guess_initial_params(seed):
+    """He-normal weights and zero biases for the learned-guess network (layout of
+    LearnedGuess: W0[27][32], b0[32], 4 x (W[27][32][32], b[32]), WL[27][32], bL)."""
+    C, T = 32, 27
+    rng = np.random.default_rng(seed)
+    parts = [rng.normal(0.0, np.sqrt(2.0 / T), T * C), np.zeros(C)]
+    for _ in range(4):
+        parts += [rng.normal(0.0, np.sqrt(2.0 / (T * C)), T * C * C), np.zeros(C)]
+    parts += [rng.normal(0.0, np.sqrt(1.0 / (T * C)), T * C), np.zeros(1)]
+    p = np.concatenate(parts).astype(np.float32)
+    assert p.size == lf.LearnedGuess.n_params()
+    return p
+
+
 class LFTS:
-    """L-FTS driver; `params` has the keys of the reference examples (fts/Lamella.py)."""
+    """L-FTS driver; `params` has the keys of the reference examples (fts/Lamella.py).
+    An optional "learned_guess" entry {"train_steps", "lr", "train_iters", "seed"}
+    trains a periodic 3D CNN on the (w-, w+) saddle points of the first train_steps
+    accepted Langevin steps and then starts every saddle point from its prediction."""
 
     def __init__(self, params, random_seed=None):
         self.monomer_types = sorted(list(params["segment_lengths"].keys()))
@@ -336,6 +353,17 @@ class LFTS:
         self.langevin = params["langevin"].copy()
         self.langevin.update({"sigma": langevin_sigma})
         self.verbose_level = params["verbose_level"]
+        # learned initial guess of w+ (optional): {"train_steps", "lr", "train_iters", "seed"}
+        self.learned = dict(params["learned_guess"]) if params.get("learned_guess") else None
+        if self.learned is not None:
+            assert len(self.mpt.aux_fields_real_idx) == 1 and len(self.mpt.aux_fields_imag_idx) == 1, \
+                "the learned guess maps one real field to one imaginary field"
+            assert cb.get_dim() == 3, "the learned guess needs a 3D grid"
+            self.learned.setdefault("train_steps", 1000)
+            self.learned.setdefault("lr", 1e-3)
+            self.learned.setdefault("train_iters", 1)
+            self.learned.setdefault("seed", 0)
+        self.guess = None
         self.saddle = params["saddle"].copy()
         self.recording = params["recording"].copy()
         self.cb = cb
@@ -374,6 +402,31 @@ class LFTS:
         n_iter, error_level, hamiltonian = self._find_saddle_point_device()
         w_aux[...] = self.saddle_device.get_fields()
         return self._phi_from_device(), hamiltonian, n_iter, error_level
+
+    # ---------------------------------------------------------------- learned guess
+    def init_guess(self):
+        """Network for w+ = f(w-), scaled by the current fields' standard deviations,
+        with He-normal weights from the configured seed."""
+        w_aux = self.saddle_device.get_fields()
+        S = len(self.monomer_types)
+        ri, ii = self.mpt.aux_fields_real_idx[0], self.mpt.aux_fields_imag_idx[0]
+        in_scale = float(np.std(w_aux[ri])) or 1.0
+        out_scale = float(np.std(w_aux[ii])) or 1.0
+        self.guess = lf.LearnedGuess(self.cb.get_nx(), in_scale, out_scale)
+        self.guess.set_params(guess_initial_params(self.learned["seed"]))
+        self.guess_trained_steps = 0
+        return in_scale, out_scale
+
+    def train_guess(self):
+        """Adam step(s) on the current (w-, w+*) pair of the device state; returns the loss."""
+        loss = None
+        for _ in range(self.learned["train_iters"]):
+            loss = self.saddle_device.train_guess(self.guess, self.learned["lr"])
+        self.guess_trained_steps += 1
+        return loss
+
+    def guess_ready(self):
+        return self.guess is not None and self.guess_trained_steps >= self.learned["train_steps"]
 
     # ---------------------------------------------------------------- device-resident run
     def _phi_from_device(self):
@@ -457,6 +510,8 @@ class LFTS:
 
         print("iterations, mass error, total partitions, Hamiltonian, incompressibility error (or saddle point error)")
         self._find_saddle_point_device()
+        if self.learned is not None and self.guess is None:
+            self.init_guess()
 
         H_history = []
         dH_history = {key: [] for key in self.chi_n}
@@ -482,6 +537,8 @@ class LFTS:
             dev.backup()
             noise = self.random.normal(0.0, self.langevin["sigma"], [R, M])
             dev.langevin_update(noise, self.langevin["dt"])
+            if self.guess_ready():
+                dev.predict_guess(self.guess)
             saddle_iter, error_level, hamiltonian = self._find_saddle_point_device()
             total_saddle_iter += saddle_iter
             total_error_level += error_level
@@ -497,6 +554,8 @@ class LFTS:
                       % (successive_fail_count))
                 break
             successive_fail_count = 0
+            if self.guess is not None and not self.guess_ready():
+                self.train_guess()
 
             computing = langevin_step % self.recording["sf_computing_period"] == 0
             recording_sf = langevin_step % self.recording["sf_recording_period"] == 0

@@ -1,0 +1,112 @@
+"""The learned-guess network on the GPU (bf16 MFMA kernels, FP32 accumulation)
+against the numpy oracle with the same bf16 rounding points."""
+import numpy as np
+import pytest
+
+from oracle import cnn_oracle as K
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+@pytest.fixture(scope="module")
+def lf():
+    from langevin_fts_amd import langevinfts
+    return langevinfts
+
+
+def params(seed):
+    rng = np.random.default_rng(seed + 100)
+    p = K.init_params(seed)
+    p[K.P_B0:K.P_WH] = rng.normal(0, 0.1, K.C)
+    for l in range(K.NHID):
+        b = K.P_WH + l * K.HSZ + K.TAPS * K.C * K.C
+        p[b:b + K.C] = rng.normal(0, 0.1, K.C)
+    p[K.P_BL] = 0.05
+    return p
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(np.asarray(a) - b) / max(np.linalg.norm(b), 1e-30))
+
+
+@pytest.mark.parametrize("nx", [[16, 16, 16], [8, 12, 16]])
+def test_forward_matches_oracle(lf, nx):
+    say(f"cnn forward {nx}")
+    p = params(1)
+    net = lf.LearnedGuess(nx, 1.0, 1.0)
+    net.set_params(p)
+    np.testing.assert_array_equal(net.get_params(), p)
+    x = np.random.default_rng(2).standard_normal(nx)
+    y = net.predict(x.ravel())
+    y_ref, _ = K.forward(p, x)
+    assert rel_l2(y, y_ref.ravel()) < 3e-3
+
+
+def test_scales(lf):
+    say("cnn scales")
+    nx = [8, 8, 8]
+    p = params(3)
+    net = lf.LearnedGuess(nx, 2.0, 0.5)
+    net.set_params(p)
+    x = np.random.default_rng(4).standard_normal(nx)
+    y_ref, _ = K.forward(p, x / 2.0)
+    assert rel_l2(net.predict(x.ravel()), 0.5 * y_ref.ravel()) < 3e-3
+
+
+@pytest.mark.parametrize("nx", [[16, 16, 16], [8, 12, 16]])
+def test_gradients_match_oracle(lf, nx):
+    say(f"cnn gradients {nx}")
+    p = params(5)
+    rng = np.random.default_rng(6)
+    x = rng.standard_normal(nx)
+    t = 0.5 * np.roll(x, 1, axis=0) - 0.3 * x + 0.1 * rng.standard_normal(nx)
+    net = lf.LearnedGuess(nx, 1.0, 1.0)
+    net.set_params(p)
+    loss, g = net.gradients(x.ravel(), t.ravel())
+    loss_ref, g_ref = K.loss_and_gradients(p, x, t)
+    assert abs(loss / loss_ref - 1) < 2e-3
+    blocks = {"W0": (K.P_W0, K.P_B0), "b0": (K.P_B0, K.P_WH), "WL": (K.P_WL, K.P_BL), "bL": (K.P_BL, K.N_PARAMS)}
+    for l in range(K.NHID):
+        b = K.P_WH + l * K.HSZ
+        blocks[f"W{l + 1}"] = (b, b + K.TAPS * K.C * K.C)
+        blocks[f"b{l + 1}"] = (b + K.TAPS * K.C * K.C, b + K.HSZ)
+    for name, (lo, hi) in blocks.items():
+        err = rel_l2(g[lo:hi], g_ref[lo:hi])
+        say(f"  {name}: relative L2 {err:.2e}")
+        assert err < 1e-2, name
+
+
+def test_adam_step_matches_oracle(lf):
+    say("cnn adam")
+    nx = [8, 8, 16]
+    p = params(7)
+    rng = np.random.default_rng(8)
+    x = rng.standard_normal(nx)
+    t = np.roll(x, -1, axis=2)
+    net = lf.LearnedGuess(nx, 1.0, 1.0)
+    net.set_params(p)
+    _, g = net.gradients(x.ravel(), t.ravel())
+    net.train_step(x.ravel(), t.ravel(), 1e-3)
+    p_ref, _, _ = K.adam(p.astype(np.float64), g.astype(np.float64), 0.0, 0.0, 1, 1e-3)
+    np.testing.assert_allclose(net.get_params(), p_ref, rtol=0, atol=2e-6)
+
+
+def test_training_fits_a_periodic_filter(lf):
+    """A target that is a fixed 3x3x3 periodic filter of the input is learnable:
+    100 Adam steps cut the loss by an order of magnitude."""
+    say("cnn training")
+    nx = [16, 16, 16]
+    rng = np.random.default_rng(9)
+    net = lf.LearnedGuess(nx, 1.0, 1.0)
+    net.set_params(K.init_params(10))
+    losses = []
+    for it in range(100):
+        x = rng.standard_normal(nx)
+        t = 0.5 * x - 0.2 * (np.roll(x, 1, 0) + np.roll(x, -1, 1)) + 0.1 * np.roll(x, 1, 2)
+        losses.append(net.train_step(x.ravel(), t.ravel(), 2e-3))
+    say(f"  loss {np.mean(losses[:5]):.4f} -> {np.mean(losses[-5:]):.4f}")
+    assert np.mean(losses[-5:]) < 0.1 * np.mean(losses[:5])
