@@ -245,10 +245,16 @@ int fts_cnn_gradients(fts_cnn* c, const double* w_minus, const double* w_plus_ta
 /* one Adam step (beta 0.9 / 0.999, eps 1e-8) on one (w-, w+) pair */
 int fts_cnn_train_step(fts_cnn* c, const double* w_minus, const double* w_plus_target, int on_device, double lr,
                        double* loss);
-/* AB field theories (one real, one imaginary field): w_aux[imag] = f(w_aux[real]) on the device,
- * and one training step on the current (w_aux[real], w_aux[imag]) pair */
-int fts_saddle_predict_guess(fts_saddle* s, fts_cnn* c);
-int fts_saddle_train_guess(fts_saddle* s, fts_cnn* c, double lr, double* loss);
+/* Field theories with one real and one imaginary field (AB melts), on the device:
+ * prepare: remember f(w_aux[real]) before the Langevin update;
+ * predict: w_aux[imag] = f(w_aux[real]) (delta = 0), or w_aux[imag] += f(w_aux[real]) - the
+ *          prepared prediction (delta = 1: the change of w+ the network expects from the update);
+ * train:   push the current (w_aux[real], w_aux[imag]) pair into a ring of `replay` pairs, then
+ *          `iters` Adam steps, the first on the newest pair and the rest on pairs drawn from the
+ *          ring; loss is the newest pair's loss before its step. */
+int fts_saddle_prepare_guess(fts_saddle* s, fts_cnn* c);
+int fts_saddle_predict_guess(fts_saddle* s, fts_cnn* c, int delta);
+int fts_saddle_train_guess(fts_saddle* s, fts_cnn* c, double lr, int iters, int replay, double* loss);
 
 #ifdef __cplusplus
 }

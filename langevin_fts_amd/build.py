@@ -27,7 +27,7 @@ HIP_FLAGS = [
     # keep the compiler from emitting packed FP32 VALU ops
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-I", os.path.join(REPO, "include"),
-]
+] + os.environ.get("FTS_HIP_DEFINES", "").split()   # tuning defines, e.g. -DFTS_STREAM_STORES=1
 
 
 def _newer(target, deps):

@@ -525,20 +525,28 @@ int fts_cnn_train_step(fts_cnn* c, const double* w_minus, const double* w_plus_t
     });
 }
 
-int fts_saddle_predict_guess(fts_saddle* s, fts_cnn* c) {
+int fts_saddle_prepare_guess(fts_saddle* s, fts_cnn* c) {
     return guarded([&] {
         need(s, "saddle");
         need(c, "cnn");
-        s->s->predict_guess(*c->c);
+        s->s->prepare_guess(*c->c);
     });
 }
 
-int fts_saddle_train_guess(fts_saddle* s, fts_cnn* c, double lr, double* loss) {
+int fts_saddle_predict_guess(fts_saddle* s, fts_cnn* c, int delta) {
+    return guarded([&] {
+        need(s, "saddle");
+        need(c, "cnn");
+        s->s->predict_guess(*c->c, delta != 0);
+    });
+}
+
+int fts_saddle_train_guess(fts_saddle* s, fts_cnn* c, double lr, int iters, int replay, double* loss) {
     return guarded([&] {
         need(s, "saddle");
         need(c, "cnn");
         FTS_REQUIRE(lr > 0, "learning rate must be positive");
-        const double l = s->s->train_guess(*c->c, lr);
+        const double l = s->s->train_guess(*c->c, lr, iters, replay);
         if (loss) *loss = l;
     });
 }

@@ -174,7 +174,7 @@ __global__ void __launch_bounds__(256) back_last(Geo g, const float* dy, const f
 
 // Hidden weight gradient, per tap and voxel chunk: G[t][co][ci] = sum_v dZ[v][co] X[nbr(v,t)][ci]
 // as D = A B with A = dZ^T (32 x voxels), B = X_t (voxels x 32). One wavefront sums WG_CHUNK / 4
-// voxels; the block's 4 wavefronts meet in LDS; partial[t][chunk][co][ci].
+// voxels; the block's 4 wavefronts meet in LDS; partial[chunk][t][co][ci].
 __global__ void __launch_bounds__(256) wgrad_hidden(Geo g, const __bf16* dZ, const __bf16* X, float* partial) {
     __shared__ float red[4][C * C];
     const int t = blockIdx.y;
@@ -199,7 +199,7 @@ __global__ void __launch_bounds__(256) wgrad_hidden(Geo g, const __bf16* dZ, con
 #pragma unroll
     for (int r = 0; r < 16; r++) red[wave][(8 * (r >> 2) + 4 * half + (r & 3)) * C + col] = acc[r];
     __syncthreads();
-    float* out = partial + ((size_t)t * gridDim.x + chunk) * C * C;
+    float* out = partial + ((size_t)chunk * TAPS + t) * C * C;
     for (int i = threadIdx.x; i < C * C; i += blockDim.x) out[i] = ((red[0][i] + red[1][i]) + red[2][i]) + red[3][i];
 }
 

@@ -19,6 +19,26 @@
 
 namespace fts {
 
+// Stores of results that the next pass reads: plain, or streaming (nontemporal) when
+// FTS_STREAM_STORES is 1.
+#ifndef FTS_STREAM_STORES
+#define FTS_STREAM_STORES 0
+#endif
+template <typename T>
+__device__ __forceinline__ void put(T* p, T v) {
+    if constexpr (FTS_STREAM_STORES) __builtin_nontemporal_store(v, p);
+    else *p = v;
+}
+template <typename T2>
+__device__ __forceinline__ void put2(T2* p, T2 v) {
+    if constexpr (FTS_STREAM_STORES) {
+        __builtin_nontemporal_store(v.x, &p->x);
+        __builtin_nontemporal_store(v.y, &p->y);
+    } else {
+        *p = v;
+    }
+}
+
 __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -115,12 +135,15 @@ __device__ __forceinline__ void wl_stages(typename WRegs<T2, L, REV, S, NLW, LF>
 }
 
 // ------------------------------------------------------------------ slab pass
+#ifndef FTS_SLAB_THREADS_64
+#define FTS_SLAB_THREADS_64 512   // workgroup size of the 64^3 slab pass
+#endif
 template <int N>
 struct WSlab {
     static constexpr int NZC = N / 2 + 1;
     static constexpr int P = NZC;   // plane row pitch (complex); packed z line p spans rows 2p, 2p+1
     static constexpr int NP = N / 2;
-    static constexpr int BD = N <= 32 ? 256 : (N <= 48 ? 384 : (N <= 64 ? 512 : 1024));
+    static constexpr int BD = N <= 32 ? 256 : (N <= 48 ? 384 : (N <= 64 ? FTS_SLAB_THREADS_64 : 1024));
     static constexpr int NW = BD / 64;
     static constexpr int NLY = (NZC + NW - 1) / NW;   // y columns per wavefront
     static constexpr int NLZ = (NP + NW - 1) / NW;    // packed z rows per wavefront
@@ -319,7 +342,7 @@ __global__ void __launch_bounds__(WSlab<N>::BD) slab_kernel_wl(SlabBatch<T> batc
                 if (MO::ok(g, ncol)) {
                     const int c = col0 + MO::line(g), j = MO::idx(g);
 #pragma unroll
-                    for (int r = 0; r < G::YF::r(G::YF::n - 1); r++) spec[(j + r * MO::B) * NZC + c] = o[t][r];
+                    for (int r = 0; r < G::YF::r(G::YF::n - 1); r++) put2(&spec[(j + r * MO::B) * NZC + c], o[t][r]);
                 }
             }
         });
@@ -368,8 +391,8 @@ __global__ void __launch_bounds__(WSlab<N>::BD) slab_kernel_wl(SlabBatch<T> batc
                         qo[t][r] *= it.mask[io];
                     }
                     if (it.qout) {
-                        it.qout[ie] = qe[t][r];
-                        it.qout[io] = qo[t][r];
+                        put(&it.qout[ie], qe[t][r]);
+                        put(&it.qout[io], qo[t][r]);
                     }
                 }
             }
@@ -425,7 +448,7 @@ __global__ void __launch_bounds__(64) xpass_kernel_wl(XBatch<T> batch, const typ
             if (MO::ok(g, nl)) {
                 const int l = MO::line(g), i0 = MO::idx(g);
 #pragma unroll
-                for (int r = 0; r < I::r(I::n - 1); r++) c[(i0 + r * MO::B) * ROW + l] = o[t][r];
+                for (int r = 0; r < I::r(I::n - 1); r++) put2(&c[(i0 + r * MO::B) * ROW + l], o[t][r]);
             }
         }
     };
@@ -467,7 +490,7 @@ __global__ void __launch_bounds__(64) xpass_kernel_wl(XBatch<T> batch, const typ
                 if (MF::ok(g, nl)) {
                     const int l = MF::line(g), i0 = MF::idx(g);
 #pragma unroll
-                    for (int r = 0; r < RF; r++) c[(i0 + r * MF::B) * ROW + l] = o[t][r];
+                    for (int r = 0; r < RF; r++) put2(&c[(i0 + r * MF::B) * ROW + l], o[t][r]);
                 }
             }
             return;

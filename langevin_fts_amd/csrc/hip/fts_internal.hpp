@@ -195,9 +195,10 @@ public:
               double* hamiltonian);
     void langevin_update(const double* noise, bool on_device, double dt);
     void field_moments(double* mean, double* mean_sq);
-    // learned guess: w_aux[imag] = f(w_aux[real]); one training step on the current pair
-    void predict_guess(Cnn& cnn);
-    double train_guess(Cnn& cnn, double lr);
+    // learned guess of the imaginary field from the real one
+    void prepare_guess(Cnn& cnn);               // f(w_real) before the Langevin update (delta mode)
+    void predict_guess(Cnn& cnn, bool delta);   // w_imag = f(w_real), or w_imag += f(w_real) - prepared
+    double train_guess(Cnn& cnn, double lr, int iters, int replay);
 
 private:
     void concentrations();
@@ -220,6 +221,11 @@ private:
     double h_const_;
     DeviceBuffer w_aux_, phi_, w_solver_, h_, lam_, am_w_, am_d_, noise_prev_, noise_cur_, w_backup_, phi_backup_,
         partials_, stats_;
+    // learned guess: replay ring of (w_real, w_imag) pairs and the prepared prediction
+    DeviceBuffer replay_, guess_old_, guess_new_;
+    int replay_cap_ = 0, replay_n_ = 0, replay_head_ = 0;
+    unsigned long long replay_rng_ = 0x9E3779B97F4A7C15ull;
+    bool prepared_ = false;
 };
 
 int cnn_n_params();

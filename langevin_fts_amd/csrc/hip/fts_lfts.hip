@@ -134,6 +134,10 @@ __global__ void lm_kernel(LmArgs a, int M) {
 
 int grid_blocks(int M) { return std::min(2048, ceil_div(M, 256)); }
 
+__global__ void add_difference_kernel(double* x, const double* a, const double* b, int M) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < M; i += gridDim.x * blockDim.x) x[i] += a[i] - b[i];
+}
+
 }  // namespace
 
 SaddlePoint::SaddlePoint(fts_context* ctx, SolverBase* solver, AndersonMixingDevice* am, int M,
@@ -428,18 +432,66 @@ void SaddlePoint::langevin_update(const double* noise, bool on_device, double dt
     FTS_HIP_CHECK(hipStreamSynchronize(st));
 }
 
-void SaddlePoint::predict_guess(Cnn& cnn) {
+void SaddlePoint::prepare_guess(Cnn& cnn) {
     FTS_REQUIRE(R_ == 1 && I_ == 1, "the learned guess maps one real field to one imaginary field");
     FTS_REQUIRE(cnn.n_grid() == M_, "the network's grid differs from the simulation grid");
-    cnn.predict(w_aux_.as<double>() + (size_t)real_idx_[0] * M_, w_aux_.as<double>() + (size_t)imag_idx_[0] * M_,
-                true);
+    if (guess_old_.bytes() == 0) guess_old_.allocate(sizeof(double) * M_);
+    cnn.predict(w_aux_.as<double>() + (size_t)real_idx_[0] * M_, guess_old_.as<double>(), true);
+    prepared_ = true;
 }
 
-double SaddlePoint::train_guess(Cnn& cnn, double lr) {
+void SaddlePoint::predict_guess(Cnn& cnn, bool delta) {
     FTS_REQUIRE(R_ == 1 && I_ == 1, "the learned guess maps one real field to one imaginary field");
     FTS_REQUIRE(cnn.n_grid() == M_, "the network's grid differs from the simulation grid");
-    return cnn.train_step(w_aux_.as<double>() + (size_t)real_idx_[0] * M_,
-                          w_aux_.as<double>() + (size_t)imag_idx_[0] * M_, true, lr);
+    double* w_real = w_aux_.as<double>() + (size_t)real_idx_[0] * M_;
+    double* w_imag = w_aux_.as<double>() + (size_t)imag_idx_[0] * M_;
+    if (!delta) {
+        cnn.predict(w_real, w_imag, true);
+        return;
+    }
+    FTS_REQUIRE(prepared_, "delta guess needs prepare_guess before the Langevin update");
+    if (guess_new_.bytes() == 0) guess_new_.allocate(sizeof(double) * M_);
+    cnn.predict(w_real, guess_new_.as<double>(), true);
+    hipLaunchKernelGGL(add_difference_kernel, dim3(grid_blocks(M_)), dim3(256), 0, ctx_->stream, w_imag,
+                       guess_new_.as<const double>(), guess_old_.as<const double>(), M_);
+    FTS_HIP_CHECK(hipGetLastError());
+    prepared_ = false;
+}
+
+double SaddlePoint::train_guess(Cnn& cnn, double lr, int iters, int replay) {
+    FTS_REQUIRE(R_ == 1 && I_ == 1, "the learned guess maps one real field to one imaginary field");
+    FTS_REQUIRE(cnn.n_grid() == M_, "the network's grid differs from the simulation grid");
+    FTS_REQUIRE(iters >= 1 && replay >= 1, "iterations and replay size must be positive");
+    hipStream_t st = ctx_->stream;
+    if (replay != replay_cap_) {
+        replay_.allocate(sizeof(double) * 2 * (size_t)M_ * replay);
+        replay_cap_ = replay;
+        replay_n_ = replay_head_ = 0;
+    }
+    // newest pair into the ring
+    double* slot = replay_.as<double>() + (size_t)replay_head_ * 2 * M_;
+    FTS_HIP_CHECK(hipMemcpyAsync(slot, w_aux_.as<double>() + (size_t)real_idx_[0] * M_, sizeof(double) * M_,
+                                 hipMemcpyDeviceToDevice, st));
+    FTS_HIP_CHECK(hipMemcpyAsync(slot + M_, w_aux_.as<double>() + (size_t)imag_idx_[0] * M_, sizeof(double) * M_,
+                                 hipMemcpyDeviceToDevice, st));
+    const int newest = replay_head_;
+    replay_head_ = (replay_head_ + 1) % replay_cap_;
+    replay_n_ = std::min(replay_n_ + 1, replay_cap_);
+    double loss = 0.0;
+    for (int i = 0; i < iters; i++) {
+        int k = newest;
+        if (i > 0) {
+            // xorshift64 pick among the stored pairs
+            replay_rng_ ^= replay_rng_ << 13;
+            replay_rng_ ^= replay_rng_ >> 7;
+            replay_rng_ ^= replay_rng_ << 17;
+            k = (int)(replay_rng_ % (unsigned long long)replay_n_);
+        }
+        const double* pair = replay_.as<double>() + (size_t)k * 2 * M_;
+        const double l = cnn.train_step(pair, pair + M_, true, lr);
+        if (i == 0) loss = l;
+    }
+    return loss;
 }
 
 }  // namespace fts

@@ -675,10 +675,13 @@ void DeviceSaddlePoint::field_moments(double* mean, double* mean_sq) {
     check_fts(fts_saddle_field_moments(saddle_, mean, mean_sq));
 }
 
-void DeviceSaddlePoint::predict_guess(LearnedGuess& g) { check_fts(fts_saddle_predict_guess(saddle_, g.handle())); }
-double DeviceSaddlePoint::train_guess(LearnedGuess& g, double lr) {
+void DeviceSaddlePoint::prepare_guess(LearnedGuess& g) { check_fts(fts_saddle_prepare_guess(saddle_, g.handle())); }
+void DeviceSaddlePoint::predict_guess(LearnedGuess& g, bool delta) {
+    check_fts(fts_saddle_predict_guess(saddle_, g.handle(), delta ? 1 : 0));
+}
+double DeviceSaddlePoint::train_guess(LearnedGuess& g, double lr, int iters, int replay) {
     double loss = 0.0;
-    check_fts(fts_saddle_train_guess(saddle_, g.handle(), lr, &loss));
+    check_fts(fts_saddle_train_guess(saddle_, g.handle(), lr, iters, replay, &loss));
     return loss;
 }
 

@@ -188,8 +188,9 @@ public:
     void find(double tolerance, int max_iter, int verbose, double* log, int* n_iter, double* error, double* ham);
     void langevin_update(const double* noise, double dt);
     void field_moments(double* mean, double* mean_sq);
-    void predict_guess(LearnedGuess& g);
-    double train_guess(LearnedGuess& g, double lr);
+    void prepare_guess(LearnedGuess& g);
+    void predict_guess(LearnedGuess& g, bool delta);
+    double train_guess(LearnedGuess& g, double lr, int iters, int replay);
 
 private:
     int S_, R_, I_, P_, M_;

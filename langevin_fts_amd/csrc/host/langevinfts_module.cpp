@@ -345,8 +345,10 @@ PYBIND11_MODULE(langevinfts, m) {
                  o.field_moments(mean.data(), mean_sq.data());
                  return py::make_tuple(mean, mean_sq);
              })
-        .def("predict_guess", &DeviceSaddlePoint::predict_guess)
-        .def("train_guess", &DeviceSaddlePoint::train_guess, py::arg("guess"), py::arg("lr"));
+        .def("prepare_guess", &DeviceSaddlePoint::prepare_guess)
+        .def("predict_guess", &DeviceSaddlePoint::predict_guess, py::arg("guess"), py::arg("delta") = false)
+        .def("train_guess", &DeviceSaddlePoint::train_guess, py::arg("guess"), py::arg("lr"), py::arg("iters") = 1,
+             py::arg("replay") = 1);
 
     // Learned initial guess of the saddle point: periodic 3D CNN w- -> w+ (an addition to
     // the reference's surface).

@@ -353,7 +353,8 @@ class LFTS:
         self.langevin = params["langevin"].copy()
         self.langevin.update({"sigma": langevin_sigma})
         self.verbose_level = params["verbose_level"]
-        # learned initial guess of w+ (optional): {"train_steps", "lr", "train_iters", "seed"}
+        # learned initial guess of w+ (optional):
+        # {"train_steps", "lr", "train_iters", "replay", "delta", "seed"}
         self.learned = dict(params["learned_guess"]) if params.get("learned_guess") else None
         if self.learned is not None:
             assert len(self.mpt.aux_fields_real_idx) == 1 and len(self.mpt.aux_fields_imag_idx) == 1, \
@@ -361,7 +362,9 @@ class LFTS:
             assert cb.get_dim() == 3, "the learned guess needs a 3D grid"
             self.learned.setdefault("train_steps", 1000)
             self.learned.setdefault("lr", 1e-3)
-            self.learned.setdefault("train_iters", 1)
+            self.learned.setdefault("train_iters", 4)
+            self.learned.setdefault("replay", 32)
+            self.learned.setdefault("delta", True)
             self.learned.setdefault("seed", 0)
         self.guess = None
         self.saddle = params["saddle"].copy()
@@ -418,15 +421,37 @@ class LFTS:
         return in_scale, out_scale
 
     def train_guess(self):
-        """Adam step(s) on the current (w-, w+*) pair of the device state; returns the loss."""
-        loss = None
-        for _ in range(self.learned["train_iters"]):
-            loss = self.saddle_device.train_guess(self.guess, self.learned["lr"])
+        """Adam steps on the current (w-, w+*) pair and on pairs replayed from a device
+        ring of recent saddle points; the network and its scales are created at the
+        first call. Returns the newest pair's loss."""
+        if self.guess is None:
+            self.init_guess()
+        loss = self.saddle_device.train_guess(self.guess, self.learned["lr"], self.learned["train_iters"],
+                                              self.learned["replay"])
         self.guess_trained_steps += 1
         return loss
 
     def guess_ready(self):
         return self.guess is not None and self.guess_trained_steps >= self.learned["train_steps"]
+
+    def learned_step(self, noise):
+        """One Langevin step with the learned guess when it is trained, else with training
+        (the body of run's loop without recording); returns (iterations, error, accepted)."""
+        dev = self.saddle_device
+        dev.backup()
+        ready = self.guess_ready()
+        if ready and self.learned["delta"]:
+            dev.prepare_guess(self.guess)
+        dev.langevin_update(noise, self.langevin["dt"])
+        if ready:
+            dev.predict_guess(self.guess, self.learned["delta"])
+        n, err, _ = self._find_saddle_point_device()
+        ok = not (np.isnan(err) or err >= self.saddle["tolerance"])
+        if not ok:
+            dev.restore()
+        elif not ready:
+            self.train_guess()
+        return n, err, ok
 
     # ---------------------------------------------------------------- device-resident run
     def _phi_from_device(self):
@@ -510,8 +535,6 @@ class LFTS:
 
         print("iterations, mass error, total partitions, Hamiltonian, incompressibility error (or saddle point error)")
         self._find_saddle_point_device()
-        if self.learned is not None and self.guess is None:
-            self.init_guess()
 
         H_history = []
         dH_history = {key: [] for key in self.chi_n}
@@ -536,9 +559,12 @@ class LFTS:
             print("Langevin step: ", langevin_step)
             dev.backup()
             noise = self.random.normal(0.0, self.langevin["sigma"], [R, M])
+            ready = self.guess_ready()
+            if ready and self.learned["delta"]:
+                dev.prepare_guess(self.guess)
             dev.langevin_update(noise, self.langevin["dt"])
-            if self.guess_ready():
-                dev.predict_guess(self.guess)
+            if ready:
+                dev.predict_guess(self.guess, self.learned["delta"])
             saddle_iter, error_level, hamiltonian = self._find_saddle_point_device()
             total_saddle_iter += saddle_iter
             total_error_level += error_level
@@ -554,7 +580,7 @@ class LFTS:
                       % (successive_fail_count))
                 break
             successive_fail_count = 0
-            if self.guess is not None and not self.guess_ready():
+            if self.learned is not None and not self.guess_ready():
                 self.train_guess()
 
             computing = langevin_step % self.recording["sf_computing_period"] == 0

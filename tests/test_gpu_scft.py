@@ -91,7 +91,7 @@ def test_scft_iterations_match_oracle(capsys):
     chi = np.array([[0.0, 15.0], [15.0, 0.0]])
     P = np.identity(2) - np.ones((2, 2)) @ np.linalg.inv(chi) / np.sum(np.linalg.inv(chi))
     w = np.stack([w0["A"].ravel(), w0["B"].ravel()])
-    V = grid.volume()
+    V = grid.volume
     for i in range(2):
         w[i] -= grid.integral(w[i]) / V
     err = 1e20
