@@ -8,8 +8,19 @@ saddle point to 1e-4 by Anderson mixing (history 20, start 0.5, mix 0.01/0.01),
 everything FP64. A "step" is one Langevin step: the Leimkuhler-Matthews update of
 w- and the saddle-point iteration of w+ (each iteration = all propagators,
 concentrations and one Anderson mixing), as examples/fts/lfts.py:687-731 does it.
-Fields start as N(0,1) from a fixed seed; W warm-up steps (and the initial saddle
-point) run untimed, then K steps are timed between stream synchronisations.
+Fields start as N(0,1) from a fixed seed; the initial saddle point and W warm-up
+steps run untimed. Then, with the same tolerance:
+
+  training     --guess-train-steps steps (default 1000) that train the CNN guess on
+               the run's own saddle points (timed separately, reported, not in value);
+  plain arm    K timed steps, each saddle point starting from the previous w+
+               (the reference's method);
+  learned arm  K timed steps from the same state whose saddle points start from the
+               CNN's prediction.
+
+`value` is the learned arm's Langevin steps/s (the plain arm's with --guess plain);
+config carries the plain rate, their ratio, the iterations per step of each arm and
+the rate of a 10,000-step run that trains on its first steps.
 
 Printed: one JSON line. `roofline` is the slab pass (the dominant kernel:
 inverse y/z FFT, exp(-w ds) factor, q store, forward z/y FFT), timed during the
@@ -84,21 +95,26 @@ def lfts_params(w, rec_dir):
     }
 
 
-def langevin_steps(sim, n, stats):
-    """n Langevin steps on the device state (lfts.py:687-731)."""
+def langevin_steps(sim, n, stats, learned=False):
+    """n Langevin steps on the device state (lfts.py:687-731); with learned=True the
+    driver's learned-guess step (training until the network is ready, then prediction)."""
     dev = sim.saddle_device
     M = sim.cb.get_n_grid()
     R = len(sim.mpt.aux_fields_real_idx)
     tol = sim.saddle["tolerance"]
     for _ in range(n):
-        dev.backup()
         noise = sim.random.normal(0.0, sim.langevin["sigma"], [R, M])
-        dev.langevin_update(noise, sim.langevin["dt"])
-        it, err, _ = sim._find_saddle_point_device()
+        if learned:
+            it, err, ok = sim.learned_step(noise)
+        else:
+            dev.backup()
+            dev.langevin_update(noise, sim.langevin["dt"])
+            it, err, _ = sim._find_saddle_point_device()
+            ok = not (np.isnan(err) or err >= tol)
+            if not ok:
+                dev.restore()
         stats["iterations"] += it
-        if np.isnan(err) or err >= tol:
-            dev.restore()
-            stats["failures"] += 1
+        stats["failures"] += 0 if ok else 1
 
 
 def cpu_baseline(w, iterations_per_step, budget_s):
@@ -244,6 +260,9 @@ def main():
                     help="only run 8 device-to-device copies of MIB MiB (reference dispatches for HBM counters)")
     ap.add_argument("--pmc-child", action="store_true", help="only run the short pass the counter runs profile")
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 counter passes")
+    ap.add_argument("--guess", default="learned", choices=["learned", "plain"],
+                    help="learned: after the plain arm, train the CNN guess and time the steps that use it")
+    ap.add_argument("--guess-train-steps", type=int, default=1000, help="Langevin steps the CNN trains on")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
@@ -264,8 +283,11 @@ def main():
 
     w = WORKLOADS[args.workload]
     M = int(np.prod(w["nx"]))
+    params = lfts_params(w, "/tmp/lfts_bench")
+    params["learned_guess"] = {"train_steps": args.guess_train_steps, "lr": 1e-3, "train_iters": 4, "replay": 64,
+                               "delta": True, "seed": 0}
     with stdout_to_stderr():
-        sim = lfts.LFTS(lfts_params(w, "/tmp/lfts_bench"), random_seed=args.seed)
+        sim = lfts.LFTS(params, random_seed=args.seed)
         rng = np.random.Generator(np.random.PCG64(args.seed))
         w_mono = np.stack([rng.standard_normal(M), rng.standard_normal(M)])
         sim.saddle_device.set_fields(np.ascontiguousarray(sim.mpt.to_aux_fields(w_mono)))
@@ -274,31 +296,64 @@ def main():
         warm = {"iterations": 0, "failures": 0}
         langevin_steps(sim, args.warmup, warm)
 
+        if args.guess == "learned":
+            # online training on the run's own saddle points (plain saddle points, then Adam steps)
+            train = {"iterations": 0, "failures": 0}
+            t0 = time.perf_counter()
+            langevin_steps(sim, args.guess_train_steps, train, learned=True)
+            sim.saddle_device.field_moments()
+            train_s = time.perf_counter() - t0
+
+        # plain arm: the reference's method (each saddle point starts from the previous w+)
         if not args.no_kernel_timing:
             lf.set_kernel_timing(SAMPLE)
-        stats = {"iterations": 0, "failures": 0}
+        plain = {"iterations": 0, "failures": 0}
         sim.saddle_device.field_moments()   # synchronises the stream
         t0 = time.perf_counter()
-        langevin_steps(sim, args.steps, stats)
+        langevin_steps(sim, args.steps, plain)
         sim.saddle_device.field_moments()
-        elapsed = time.perf_counter() - t0
+        plain_s = time.perf_counter() - t0
         kern = {}
         if not args.no_kernel_timing:
             kern = {f: lf.get_kernel_timing(f) for f in ("slab", "pencil")}
             lf.set_kernel_timing(0)
 
+        learned = None
+        if args.guess == "learned":
+            # the same state and tolerance, saddle points starting from the CNN's prediction
+            learned = {"iterations": 0, "failures": 0}
+            t0 = time.perf_counter()
+            langevin_steps(sim, args.steps, learned, learned=True)
+            sim.saddle_device.field_moments()
+            learned_s = time.perf_counter() - t0
+
+    stats = learned if learned is not None else plain
+    elapsed = learned_s if learned is not None else plain_s
     steps_per_s = args.steps / elapsed
     iters = stats["iterations"] / args.steps
+    plain_iters = plain["iterations"] / args.steps
     out = {
         "metric": "L-FTS steps/s @64³ (learned vs plain AM guess, same tol); propagator HBM GB/s",
         "value": steps_per_s, "unit": "steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": 1e3 * elapsed / args.steps, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64",
         "data": "synthetic: w_A, w_B ~ N(0,1) from PCG64(seed); Langevin noise from numpy PCG64",
-        "config": {"workload": describe(args.workload, w), "guess": "plain Anderson mixing",
+        "config": {"workload": describe(args.workload, w),
+                   "guess": "learned CNN (delta form) + Anderson mixing" if learned else "plain Anderson mixing",
                    "saddle_iterations_per_step": iters, "saddle_failures": stats["failures"],
-                   "propagator_evaluations_per_s": stats["iterations"] / elapsed},
+                   "plain_steps_per_s": args.steps / plain_s, "plain_saddle_iterations_per_step": plain_iters,
+                   "plain_saddle_failures": plain["failures"],
+                   "propagator_evaluations_per_s": plain["iterations"] / plain_s},
     }
+    if learned is not None:
+        n_train, n_run = args.guess_train_steps, 10000
+        train_step_s = train_s / max(n_train, 1)
+        out["config"].update({
+            "learned_vs_plain": (args.steps / learned_s) / (args.steps / plain_s),
+            "guess_train_steps": n_train, "train_step_ms": 1e3 * train_step_s,
+            "train_saddle_iterations_per_step": train["iterations"] / max(n_train, 1),
+            # a 10,000-step run that trains on its first n_train steps, at the measured rates
+            "run10k_steps_per_s": n_run / (n_train * train_step_s + (n_run - n_train) * learned_s / args.steps)})
     if kern:
         sl, pe = kern["slab"], kern["pencil"]
         achieved = sl["bytes"] / (sl["milliseconds"] * 1e-3) / 1e9
@@ -317,11 +372,11 @@ def main():
         out["config"]["propagator_GBps"] = (sl["bytes"] + pe["bytes"]) / (both_ms * 1e-3) / 1e9
         out["config"]["pencil_GBps"] = pe["bytes"] / (pe["milliseconds"] * 1e-3) / 1e9
         out["config"]["pencil_avg_launch_us"] = 1e3 * pe["milliseconds"] / max(pe["launches"], 1)
-        # the timed launches are one in SAMPLE of each family
-        out["config"]["propagator_kernel_share"] = SAMPLE * both_ms * 1e-3 / elapsed
+        # kernel time of the plain arm; the timed launches are one in SAMPLE of each family
+        out["config"]["propagator_kernel_share"] = SAMPLE * both_ms * 1e-3 / plain_s
     if args.cpu_seconds > 0:
         with stdout_to_stderr():
-            out["cpu_baseline"] = cpu_baseline(w, iters, args.cpu_seconds)
+            out["cpu_baseline"] = cpu_baseline(w, plain_iters, args.cpu_seconds)
     print(json.dumps(out), flush=True)
 
 
