@@ -55,6 +55,9 @@ WORKLOADS = {
                    nbar=1e4, dt=0.8, tol=1e-4, am=(20, 5e-1, 0.01, 0.01), max_iter=200),
     "lfts32": dict(nx=[32, 32, 32], lx=[4.0, 4.0, 4.0], model="discrete", ds=1 / 90, f=0.5, chi_n=17.0,
                    nbar=1e4, dt=0.8, tol=1e-4, am=(20, 5e-1, 0.01, 0.01), max_iter=200),
+    # batching form: 32 independent replicas advanced together
+    "replicas48": dict(nx=[48, 48, 48], lx=[4.0, 4.0, 4.0], model="discrete", ds=1 / 64, f=0.5, chi_n=17.0,
+                       nbar=1e4, dt=0.8, tol=1e-4, am=(20, 5e-1, 0.01, 0.01), max_iter=200, replicas=32),
 }
 
 
@@ -75,7 +78,8 @@ def stdout_to_stderr():
 
 def describe(name, w):
     n = round(1 / w["ds"])
-    return (f"L-FTS AB diblock, {w['model']} N={n}, f={w['f']}, chiN={w['chi_n']}, "
+    rep = f"{w['replicas']} replicas of " if w.get("replicas", 1) > 1 else ""
+    return (f"{rep}L-FTS AB diblock, {w['model']} N={n}, f={w['f']}, chiN={w['chi_n']}, "
             f"{'x'.join(map(str, w['nx']))} grid, L={w['lx'][0]}, nbar={w['nbar']:g}, dt={w['dt']}, "
             f"saddle tol {w['tol']:g}")
 
@@ -247,6 +251,61 @@ def copy_calibration(mib):
     print(json.dumps({"copy_bytes_per_dispatch": nbytes, "dispatches": 8}), flush=True)
 
 
+def kernel_roofline(kern, label):
+    sl = kern["slab"]
+    achieved = sl["bytes"] / (sl["milliseconds"] * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBPS, "unit": "GB/s",
+            "frac": achieved / PEAK_HBM_GBPS, "traffic": None, "kernel": label,
+            "timed_launch_count": sl["launches"], "avg_launch_us": 1e3 * sl["milliseconds"] / max(sl["launches"], 1),
+            "bytes_per_launch": sl["bytes"] / max(sl["launches"], 1), "timed_launches": f"1 in {SAMPLE}"}
+
+
+def replica_bench(args, w, lf, lfts):
+    """Batching form: w["replicas"] independent runs whose propagators share launches;
+    value = replica Langevin steps per second over all replicas."""
+    R = w["replicas"]
+    M = int(np.prod(w["nx"]))
+    with stdout_to_stderr():
+        sim = lfts.LFTS(lfts_params(w, "/tmp/lfts_bench"), random_seed=args.seed, n_replicas=R)
+        rng = np.random.Generator(np.random.PCG64(args.seed))
+        w_aux = np.stack([sim.mpt.to_aux_fields(rng.standard_normal((2, M))) for _ in range(R)])
+        dev = sim.saddle_device
+        dev.set_fields(np.ascontiguousarray(w_aux))
+        dev.set_noise_prev(np.zeros((R, 1, M)))
+        dev.find_saddle_point(sim.saddle["tolerance"], sim.saddle["max_iter"], 0)
+        for _ in range(args.warmup):
+            sim.replica_step()
+        if not args.no_kernel_timing:
+            lf.set_kernel_timing(SAMPLE)
+        its, fails = [], 0
+        dev.field_moments()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            n, _, ok = sim.replica_step()
+            its += list(n)
+            fails += sum(1 for a in ok if not a)
+        dev.field_moments()
+        elapsed = time.perf_counter() - t0
+        kern = {}
+        if not args.no_kernel_timing:
+            kern = {f: lf.get_kernel_timing(f) for f in ("slab", "pencil")}
+            lf.set_kernel_timing(0)
+    out = {
+        "metric": f"L-FTS replica steps/s @{w['nx'][0]}³ ({R} replicas batched)",
+        "value": R * args.steps / elapsed, "unit": "replica steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * elapsed / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic: w_A, w_B ~ N(0,1) from PCG64(seed); replica r's noise from PCG64(seed + r)",
+        "config": {"workload": describe(args.workload, w), "replicas": R,
+                   "saddle_iterations_per_step": float(np.mean(its)), "saddle_failures": fails,
+                   "max_saddle_iterations": int(np.max(its))},
+    }
+    if kern:
+        out["roofline"] = kernel_roofline(
+            kern, f"slab pass (slab_kernel_wl<double, 1, {w['nx'][0]}>: up to 16 propagators per launch)")
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -283,6 +342,9 @@ def main():
 
     w = WORKLOADS[args.workload]
     M = int(np.prod(w["nx"]))
+    if w.get("replicas", 1) > 1:
+        replica_bench(args, w, lf, lfts)
+        return
     params = lfts_params(w, "/tmp/lfts_bench")
     params["learned_guess"] = {"train_steps": args.guess_train_steps, "lr": 1e-3, "train_iters": 4, "replay": 64,
                                "delta": True, "seed": 0}

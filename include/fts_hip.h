@@ -131,6 +131,12 @@ typedef struct {
  * precision FTS_FP64 or FTS_FP32 (propagators and FFTs; Q, concentrations and
  * stress accumulate in FP64 either way). The plan arrays are copied. */
 int fts_solver_create(fts_grid* g, const fts_plan* plan, int precision, fts_solver** out);
+/* n_replicas independent copies of the plan in one solver (no reference counterpart:
+ * the replica-batched L-FTS form). Replica r has its own monomer fields (index
+ * r n_monomers + m in the w array of compute_propagators), blocks (r n_blocks + b),
+ * polymers (r n_polymers + p) and solvents; every schedule slice advances all
+ * replicas' jobs, so the same launches carry all of them. */
+int fts_solver_create_replicas(fts_grid* g, const fts_plan* plan, int precision, int n_replicas, fts_solver** out);
 int fts_solver_destroy(fts_solver* s);
 /* PropagatorComputation::update_laplacian_operator after fts_grid_set_lx */
 int fts_solver_update_laplacian(fts_solver* s);
@@ -201,6 +207,16 @@ typedef struct fts_field_theory {
 
 int fts_saddle_create(fts_context* ctx, fts_solver* solver, fts_am* am, int n_grid, const fts_field_theory* ft,
                       fts_saddle** out);
+/* n_replicas independent saddle points over a solver made by fts_solver_create_replicas
+ * (n_blocks, n_solvents: per replica; one Anderson mixing per replica with the given
+ * parameters). Fields, concentrations and noise then hold n_replicas consecutive
+ * copies ([R][S][n_grid], [R][S][n_grid], [R][n_real][n_grid]); find iterates until
+ * every replica converged or max_iter (a converged replica keeps its fields) and
+ * writes n_iter, error_level and hamiltonian per replica; field_moments returns R x S
+ * values. The learned-guess calls need a single replica. */
+int fts_saddle_create_replicas(fts_context* ctx, fts_solver* solver, int n_replicas, int n_blocks, int n_solvents,
+                               int max_hist, double start_error, double mix_min, double mix_init, int n_grid,
+                               const fts_field_theory* ft, fts_saddle** out);
 int fts_saddle_destroy(fts_saddle* s);
 /* w_aux [S][n_grid] in and out */
 int fts_saddle_set_fields(fts_saddle* s, const double* w_aux, int on_device);
@@ -214,6 +230,8 @@ int fts_saddle_get_noise(fts_saddle* s, double* noise_prev, int on_device);
  * point fails (lfts.py:696-698, :727-729) */
 int fts_saddle_backup(fts_saddle* s);
 int fts_saddle_restore(fts_saddle* s);
+/* put back only the replicas whose flag is nonzero (n_replicas flags) */
+int fts_saddle_restore_replicas(fts_saddle* s, const int* replicas);
 /* find_saddle_point (lfts.py:806-867). verbose 2: log row per iteration; verbose 1:
  * one row for the final iteration; row = [iter, mass error, H, Q_0..Q_{P-1},
  * error_0..error_{I-1}] (3 + P + I doubles). hamiltonian: H at the final iteration. */

@@ -1,6 +1,7 @@
 // extern "C" entry points of include/fts_hip.h: argument checks, exception to
 // status-code conversion, thread-local error text.
 #include <string>
+#include <vector>
 
 #include "fts_internal.hpp"
 
@@ -210,6 +211,126 @@ int fts_solver_create(fts_grid* g, const fts_plan* plan, int precision, fts_solv
     });
 }
 
+namespace {
+
+// R copies of a plan with their own monomer fields: replica r's propagator p is
+// r n_props + p, block b is r n_blocks + b, polymer q is r n_polymers + q, solvent
+// s is r n_solvents + s, monomer m is r n_monomers + m, q_init k is r n_qinit + k;
+// every schedule slice holds the jobs of all replicas.
+struct ReplicatedPlan {
+    fts_plan plan{};
+    std::vector<double> seg, vf, alpha, svf;
+    std::vector<int> pm, pmax, pk, pq, pdo, pnd, dp, dn, dr, pjo, pnj, je, bp, bl, br, bnl, bnr, brep, bm, bnvu, smon,
+        soff, jp, jf, jt;
+
+    ReplicatedPlan(const fts_plan& p, int R) {
+        for (int r = 0; r < R; r++) {
+            seg.insert(seg.end(), p.segment_length, p.segment_length + p.n_monomers);
+            for (int i = 0; i < p.n_props; i++) {
+                pm.push_back(p.prop_monomer[i] + r * p.n_monomers);
+                pmax.push_back(p.prop_max_n[i]);
+                pk.push_back(p.prop_kind[i]);
+                pq.push_back(p.prop_qinit && p.prop_qinit[i] >= 0 ? p.prop_qinit[i] + r * p.n_qinit : -1);
+                pdo.push_back((int)dp.size());
+                pnd.push_back(p.prop_n_deps[i]);
+                for (int d = 0; d < p.prop_n_deps[i]; d++) {
+                    const int k = p.prop_dep_off[i] + d;
+                    dp.push_back(p.dep_prop[k] + r * p.n_props);
+                    dn.push_back(p.dep_n[k]);
+                    dr.push_back(p.dep_rep[k]);
+                }
+                pjo.push_back((int)je.size());
+                pnj.push_back(p.prop_n_je[i]);
+                for (int d = 0; d < p.prop_n_je[i]; d++) je.push_back(p.je[p.prop_je_off[i] + d]);
+            }
+            for (int b = 0; b < p.n_blocks; b++) {
+                bp.push_back(p.blk_polymer[b] + r * p.n_polymers);
+                bl.push_back(p.blk_left[b] + r * p.n_props);
+                br.push_back(p.blk_right[b] + r * p.n_props);
+                bnl.push_back(p.blk_n_left[b]);
+                bnr.push_back(p.blk_n_right[b]);
+                brep.push_back(p.blk_rep[b]);
+                bm.push_back(p.blk_monomer[b] + r * p.n_monomers);
+                bnvu.push_back(p.blk_n_vu[b]);
+            }
+            vf.insert(vf.end(), p.pol_vf, p.pol_vf + p.n_polymers);
+            alpha.insert(alpha.end(), p.pol_alpha, p.pol_alpha + p.n_polymers);
+            for (int k = 0; k < p.n_solvents; k++) {
+                svf.push_back(p.sol_vf[k]);
+                smon.push_back(p.sol_monomer[k] + r * p.n_monomers);
+            }
+        }
+        for (int sl = 0; sl < p.n_slices; sl++) {
+            soff.push_back((int)jp.size());
+            for (int r = 0; r < R; r++)
+                for (int j = p.slice_off[sl]; j < p.slice_off[sl + 1]; j++) {
+                    jp.push_back(p.job_prop[j] + r * p.n_props);
+                    jf.push_back(p.job_from[j]);
+                    jt.push_back(p.job_to[j]);
+                }
+        }
+        soff.push_back((int)jp.size());
+        plan.model = p.model;
+        plan.ds = p.ds;
+        plan.n_monomers = p.n_monomers * R;
+        plan.segment_length = seg.data();
+        plan.n_props = p.n_props * R;
+        plan.prop_monomer = pm.data();
+        plan.prop_max_n = pmax.data();
+        plan.prop_kind = pk.data();
+        plan.prop_qinit = pq.data();
+        plan.prop_dep_off = pdo.data();
+        plan.prop_n_deps = pnd.data();
+        plan.dep_prop = dp.empty() ? nullptr : dp.data();
+        plan.dep_n = dn.empty() ? nullptr : dn.data();
+        plan.dep_rep = dr.empty() ? nullptr : dr.data();
+        plan.prop_je_off = pjo.data();
+        plan.prop_n_je = pnj.data();
+        plan.je = je.empty() ? nullptr : je.data();
+        plan.n_blocks = p.n_blocks * R;
+        plan.blk_polymer = bp.data();
+        plan.blk_left = bl.data();
+        plan.blk_right = br.data();
+        plan.blk_n_left = bnl.data();
+        plan.blk_n_right = bnr.data();
+        plan.blk_rep = brep.data();
+        plan.blk_monomer = bm.data();
+        plan.blk_n_vu = bnvu.data();
+        plan.n_polymers = p.n_polymers * R;
+        plan.pol_vf = vf.data();
+        plan.pol_alpha = alpha.data();
+        plan.n_solvents = p.n_solvents * R;
+        plan.sol_vf = svf.empty() ? nullptr : svf.data();
+        plan.sol_monomer = smon.empty() ? nullptr : smon.data();
+        plan.n_qinit = p.n_qinit * R;
+        plan.n_slices = p.n_slices;
+        plan.slice_off = soff.data();
+        plan.job_prop = jp.data();
+        plan.job_from = jf.data();
+        plan.job_to = jt.data();
+    }
+};
+
+}  // namespace
+
+int fts_solver_create_replicas(fts_grid* g, const fts_plan* plan, int precision, int n_replicas, fts_solver** out) {
+    return guarded([&] {
+        need(g, "grid");
+        need(plan, "plan");
+        need(out, "out");
+        FTS_REQUIRE(n_replicas >= 1, "n_replicas must be positive");
+        ReplicatedPlan rp(*plan, n_replicas);
+        auto* s = new fts_solver;
+        try {
+            s->s = fts::make_solver(g->g.get(), &rp.plan, precision);
+        } catch (...) {
+            delete s;
+            throw;
+        }
+        *out = s;
+    });
+}
+
 int fts_solver_destroy(fts_solver* s) {
     return guarded([&] { delete s; });
 }
@@ -361,7 +482,39 @@ int fts_saddle_create(fts_context* ctx, fts_solver* solver, fts_am* am, int n_gr
         need(ft->polymer_weight, "polymer_weight");
         auto* s = new fts_saddle;
         try {
-            s->s = std::make_unique<fts::SaddlePoint>(ctx, solver->s.get(), am->a.get(), n_grid, *ft);
+            s->s = std::make_unique<fts::SaddlePoint>(ctx, solver->s.get(),
+                                                      std::vector<fts::AndersonMixingDevice*>{am->a.get()}, n_grid,
+                                                      *ft);
+        } catch (...) {
+            delete s;
+            throw;
+        }
+        *out = s;
+    });
+}
+
+int fts_saddle_create_replicas(fts_context* ctx, fts_solver* solver, int n_replicas, int n_blocks, int n_solvents,
+                               int max_hist, double start_error, double mix_min, double mix_init, int n_grid,
+                               const fts_field_theory* ft, fts_saddle** out) {
+    return guarded([&] {
+        need(ctx, "ctx");
+        need(solver, "solver");
+        need(ft, "field theory");
+        need(out, "out");
+        FTS_REQUIRE(n_replicas >= 1, "n_replicas must be positive");
+        FTS_REQUIRE(ft->n_types >= 1 && ft->n_types <= FTS_MAX_TYPES, "n_types must be in [1, 8]");
+        need(ft->imag_idx, "imag_idx");
+        need(ft->w_map, "w_map");
+        need(ft->phi_term_off, "phi_term_off");
+        auto* s = new fts_saddle;
+        try {
+            std::vector<fts::AndersonMixingDevice*> ams;
+            for (int r = 0; r < n_replicas; r++) {
+                s->owned_ams.push_back(std::make_unique<fts::AndersonMixingDevice>(
+                    ctx, ft->n_imag * n_grid, max_hist, start_error, mix_min, mix_init));
+                ams.push_back(s->owned_ams.back().get());
+            }
+            s->s = std::make_unique<fts::SaddlePoint>(ctx, solver->s.get(), ams, n_grid, *ft, n_blocks, n_solvents);
         } catch (...) {
             delete s;
             throw;
@@ -425,6 +578,14 @@ int fts_saddle_restore(fts_saddle* s) {
     return guarded([&] {
         need(s, "saddle");
         s->s->restore();
+    });
+}
+
+int fts_saddle_restore_replicas(fts_saddle* s, const int* replicas) {
+    return guarded([&] {
+        need(s, "saddle");
+        need(replicas, "replicas");
+        s->s->restore(replicas);
     });
 }
 

@@ -183,34 +183,40 @@ class Cnn;
 
 class SaddlePoint {
 public:
-    SaddlePoint(fts_context* ctx, SolverBase* solver, AndersonMixingDevice* am, int M, const fts_field_theory& ft);
+    // one Anderson mixing per replica; n_blocks / n_solvents: per replica (replicated plans)
+    SaddlePoint(fts_context* ctx, SolverBase* solver, std::vector<AndersonMixingDevice*> ams, int M,
+                const fts_field_theory& ft, int n_blocks = 0, int n_solvents = 0);
     void set_fields(const double* w, bool on_device);
     void get_fields(double* w, bool on_device);
     void get_concentration(int type, double* out, bool on_device);
     void set_noise(const double* noise, bool on_device);
     void get_noise(double* noise, bool on_device);
     void backup();
-    void restore();
+    void restore(const int* replicas = nullptr);
+    // n_iter, error and hamiltonian hold one entry per replica
     void find(double tolerance, int max_iter, int verbose, double* log, int* n_iter, double* error,
               double* hamiltonian);
     void langevin_update(const double* noise, bool on_device, double dt);
     void field_moments(double* mean, double* mean_sq);
-    // learned guess of the imaginary field from the real one
+    int n_replicas() const { return NR_; }
+    // learned guess of the imaginary field from the real one (single replica)
     void prepare_guess(Cnn& cnn);               // f(w_real) before the Langevin update (delta mode)
     void predict_guess(Cnn& cnn, bool delta);   // w_imag = f(w_real), or w_imag += f(w_real) - prepared
     double train_guess(Cnn& cnn, double lr, int iters, int replay);
 
 private:
+    double* wa(int r) const { return w_aux_.as<double>() + (size_t)r * S_ * M_; }
+    double* ph(int r) const { return phi_.as<double>() + (size_t)r * S_ * M_; }
     void concentrations();
-    void run_lincomb(const std::vector<double>& cw, const std::vector<double>& cp, const std::vector<double>& c0,
-                     int n_out, double* out);
+    void run_lincomb(int rep, const std::vector<double>& cw, const std::vector<double>& cp,
+                     const std::vector<double>& c0, int n_out, double* out);
     void row_sums(const std::vector<const double*>& rows, const double* centre, bool square, double* out_dev);
-    double hamiltonian();
+    double hamiltonian(int rep);
 
     fts_context* ctx_;
     SolverBase* solver_;
-    AndersonMixingDevice* am_;
-    int M_, S_, R_, I_, P_, n_solver_w_;
+    std::vector<AndersonMixingDevice*> ams_;
+    int M_, NR_, n_blocks_, n_solvents_, S_, R_, I_, P_, n_solver_w_;
     std::vector<int> real_idx_, imag_idx_;
     std::vector<double> w_map_;   // n_solver_w x S
     std::vector<int> term_off_, term_kind_, term_index_;
@@ -267,6 +273,7 @@ struct fts_am {
     std::unique_ptr<fts::AndersonMixingDevice> a;
 };
 struct fts_saddle {
+    std::vector<std::unique_ptr<fts::AndersonMixingDevice>> owned_ams;   // replica saddles own theirs
     std::unique_ptr<fts::SaddlePoint> s;
 };
 struct fts_cnn {

@@ -140,19 +140,23 @@ __global__ void add_difference_kernel(double* x, const double* a, const double* 
 
 }  // namespace
 
-SaddlePoint::SaddlePoint(fts_context* ctx, SolverBase* solver, AndersonMixingDevice* am, int M,
-                         const fts_field_theory& ft)
-    : ctx_(ctx), solver_(solver), am_(am), M_(M) {
+SaddlePoint::SaddlePoint(fts_context* ctx, SolverBase* solver, std::vector<AndersonMixingDevice*> ams, int M,
+                         const fts_field_theory& ft, int n_blocks, int n_solvents)
+    : ctx_(ctx), solver_(solver), ams_(std::move(ams)), M_(M), NR_((int)ams_.size()), n_blocks_(n_blocks),
+      n_solvents_(n_solvents) {
     S_ = ft.n_types;
     R_ = ft.n_real;
     I_ = ft.n_imag;
     P_ = ft.n_polymers;
     n_solver_w_ = ft.n_solver_fields;
+    FTS_REQUIRE(NR_ >= 1, "at least one replica is required");
+    for (auto* a : ams_) FTS_REQUIRE(a != nullptr, "Anderson mixing is required for every replica");
     FTS_REQUIRE(M > 0, "n_grid must be positive");
     FTS_REQUIRE(S_ >= 1 && S_ <= FTS_MAX_TYPES, "n_types must be in [1, 8]");
     FTS_REQUIRE(R_ >= 0 && I_ >= 1 && R_ + I_ <= S_, "invalid numbers of real and imaginary fields");
     FTS_REQUIRE(n_solver_w_ >= 1 && n_solver_w_ <= LC_MAX_OUT, "n_solver_fields must be in [1, 8]");
     FTS_REQUIRE(P_ >= 1, "n_polymers must be positive");
+    FTS_REQUIRE(NR_ == 1 || (n_blocks > 0 && n_solvents >= 0), "replicas need the per-replica block count");
     for (int c = 0; c < R_; c++) {
         FTS_REQUIRE(ft.real_idx[c] >= 0 && ft.real_idx[c] < S_, "real_idx out of range");
         real_idx_.push_back(ft.real_idx[c]);
@@ -183,7 +187,7 @@ SaddlePoint::SaddlePoint(fts_context* ctx, SolverBase* solver, AndersonMixingDev
     for (int i : real_idx_) FTS_REQUIRE(dt_scaling_[i] > 0.0, "dt_scaling of the real fields must be positive");
     for (int i : imag_idx_) FTS_REQUIRE(dt_scaling_[i] > 0.0, "dt_scaling of the imaginary fields must be positive");
 
-    const size_t f = sizeof(double) * M;
+    const size_t f = sizeof(double) * M * NR_;
     w_aux_.allocate(f * S_);
     phi_.allocate(f * S_);
     w_solver_.allocate(f * n_solver_w_);
@@ -196,7 +200,7 @@ SaddlePoint::SaddlePoint(fts_context* ctx, SolverBase* solver, AndersonMixingDev
     w_backup_.allocate(f * S_);
     phi_backup_.allocate(f * S_);
     partials_.allocate(sizeof(double) * 2 * FTS_MAX_TYPES * STAT_BLOCKS);
-    stats_.allocate(sizeof(double) * 4 * FTS_MAX_TYPES);
+    stats_.allocate(sizeof(double) * 4 * FTS_MAX_TYPES * NR_);
     FTS_HIP_CHECK(hipMemsetAsync(w_aux_.get(), 0, f * S_, ctx_->stream));
     FTS_HIP_CHECK(hipMemsetAsync(phi_.get(), 0, f * S_, ctx_->stream));
     FTS_HIP_CHECK(hipMemsetAsync(noise_prev_.get(), 0, f * std::max(R_, 1), ctx_->stream));
@@ -204,19 +208,19 @@ SaddlePoint::SaddlePoint(fts_context* ctx, SolverBase* solver, AndersonMixingDev
 }
 
 void SaddlePoint::set_fields(const double* w, bool on_device) {
-    FTS_HIP_CHECK(hipMemcpyAsync(w_aux_.get(), w, sizeof(double) * M_ * S_,
+    FTS_HIP_CHECK(hipMemcpyAsync(w_aux_.get(), w, sizeof(double) * M_ * S_ * NR_,
                                  on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, ctx_->stream));
     FTS_HIP_CHECK(hipStreamSynchronize(ctx_->stream));
 }
 
 void SaddlePoint::get_fields(double* w, bool on_device) {
-    FTS_HIP_CHECK(hipMemcpyAsync(w, w_aux_.get(), sizeof(double) * M_ * S_,
+    FTS_HIP_CHECK(hipMemcpyAsync(w, w_aux_.get(), sizeof(double) * M_ * S_ * NR_,
                                  on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, ctx_->stream));
     FTS_HIP_CHECK(hipStreamSynchronize(ctx_->stream));
 }
 
 void SaddlePoint::get_concentration(int type, double* out, bool on_device) {
-    FTS_REQUIRE(type >= 0 && type < S_, "type index out of range");
+    FTS_REQUIRE(type >= 0 && type < S_ * NR_, "type index out of range");
     FTS_HIP_CHECK(hipMemcpyAsync(out, phi_.as<double>() + (size_t)type * M_, sizeof(double) * M_,
                                  on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, ctx_->stream));
     FTS_HIP_CHECK(hipStreamSynchronize(ctx_->stream));
@@ -224,40 +228,45 @@ void SaddlePoint::get_concentration(int type, double* out, bool on_device) {
 
 void SaddlePoint::set_noise(const double* noise, bool on_device) {
     if (R_ == 0) return;
-    FTS_HIP_CHECK(hipMemcpyAsync(noise_prev_.get(), noise, sizeof(double) * M_ * R_,
+    FTS_HIP_CHECK(hipMemcpyAsync(noise_prev_.get(), noise, sizeof(double) * M_ * R_ * NR_,
                                  on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, ctx_->stream));
     FTS_HIP_CHECK(hipStreamSynchronize(ctx_->stream));
 }
 
 void SaddlePoint::get_noise(double* noise, bool on_device) {
     if (R_ == 0) return;
-    FTS_HIP_CHECK(hipMemcpyAsync(noise, noise_prev_.get(), sizeof(double) * M_ * R_,
+    FTS_HIP_CHECK(hipMemcpyAsync(noise, noise_prev_.get(), sizeof(double) * M_ * R_ * NR_,
                                  on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, ctx_->stream));
     FTS_HIP_CHECK(hipStreamSynchronize(ctx_->stream));
 }
 
 void SaddlePoint::backup() {
-    const size_t b = sizeof(double) * M_ * S_;
+    const size_t b = sizeof(double) * M_ * S_ * NR_;
     FTS_HIP_CHECK(hipMemcpyAsync(w_backup_.get(), w_aux_.get(), b, hipMemcpyDeviceToDevice, ctx_->stream));
     FTS_HIP_CHECK(hipMemcpyAsync(phi_backup_.get(), phi_.get(), b, hipMemcpyDeviceToDevice, ctx_->stream));
 }
 
-void SaddlePoint::restore() {
+void SaddlePoint::restore(const int* replicas) {
     const size_t b = sizeof(double) * M_ * S_;
-    FTS_HIP_CHECK(hipMemcpyAsync(w_aux_.get(), w_backup_.get(), b, hipMemcpyDeviceToDevice, ctx_->stream));
-    FTS_HIP_CHECK(hipMemcpyAsync(phi_.get(), phi_backup_.get(), b, hipMemcpyDeviceToDevice, ctx_->stream));
+    for (int r = 0; r < NR_; r++) {
+        if (replicas && !replicas[r]) continue;
+        FTS_HIP_CHECK(hipMemcpyAsync(wa(r), w_backup_.as<double>() + (size_t)r * S_ * M_, b, hipMemcpyDeviceToDevice,
+                                     ctx_->stream));
+        FTS_HIP_CHECK(hipMemcpyAsync(ph(r), phi_backup_.as<double>() + (size_t)r * S_ * M_, b,
+                                     hipMemcpyDeviceToDevice, ctx_->stream));
+    }
     FTS_HIP_CHECK(hipStreamSynchronize(ctx_->stream));
 }
 
-void SaddlePoint::run_lincomb(const std::vector<double>& cw, const std::vector<double>& cp,
+void SaddlePoint::run_lincomb(int rep, const std::vector<double>& cw, const std::vector<double>& cp,
                               const std::vector<double>& c0, int n_out, double* out) {
-    // inputs: w_aux rows 0..S-1, then phi rows 0..S-1 (cp empty: w_aux only)
+    // inputs: w_aux rows 0..S-1 of the replica, then its phi rows (cp empty: w_aux only)
     LinComb a{};
     a.n_out = n_out;
     a.n_in = cp.empty() ? S_ : 2 * S_;
     for (int j = 0; j < S_; j++) {
-        a.in[j] = w_aux_.as<double>() + (size_t)j * M_;
-        if (!cp.empty()) a.in[S_ + j] = phi_.as<double>() + (size_t)j * M_;
+        a.in[j] = wa(rep) + (size_t)j * M_;
+        if (!cp.empty()) a.in[S_ + j] = ph(rep) + (size_t)j * M_;
     }
     for (int r = 0; r < n_out; r++) {
         a.out[r] = out + (size_t)r * M_;
@@ -273,55 +282,72 @@ void SaddlePoint::run_lincomb(const std::vector<double>& cw, const std::vector<d
 
 void SaddlePoint::row_sums(const std::vector<const double*>& rows, const double* centre, bool square,
                            double* out_dev) {
-    Rows r{};
+    // fixed-order sums of any number of rows, 16 rows per launch
     const int n = (int)rows.size();
-    FTS_REQUIRE(n <= 2 * FTS_MAX_TYPES, "too many rows");
-    for (int i = 0; i < n; i++) r.row[i] = rows[i];
-    hipLaunchKernelGGL(row_sum_stage1, dim3(STAT_BLOCKS, n), dim3(STAT_THREADS), 0, ctx_->stream, r, M_, centre,
-                       square ? 1 : 0, partials_.as<double>());
-    hipLaunchKernelGGL(row_sum_stage2, dim3(n), dim3(STAT_BLOCKS), 0, ctx_->stream, partials_.as<double>(),
-                       out_dev);
-    FTS_HIP_CHECK(hipGetLastError());
+    for (int r0 = 0; r0 < n; r0 += 2 * FTS_MAX_TYPES) {
+        Rows r{};
+        const int k = std::min(2 * FTS_MAX_TYPES, n - r0);
+        for (int i = 0; i < k; i++) r.row[i] = rows[r0 + i];
+        hipLaunchKernelGGL(row_sum_stage1, dim3(STAT_BLOCKS, k), dim3(STAT_THREADS), 0, ctx_->stream, r, M_,
+                           centre ? centre + r0 : nullptr, square ? 1 : 0, partials_.as<double>());
+        hipLaunchKernelGGL(row_sum_stage2, dim3(k), dim3(STAT_BLOCKS), 0, ctx_->stream, partials_.as<double>(),
+                           out_dev + r0);
+        FTS_HIP_CHECK(hipGetLastError());
+    }
 }
 
 void SaddlePoint::concentrations() {
-    // monomer fields of the solver (lfts.py:534-545)
-    run_lincomb(w_map_, {}, {}, n_solver_w_, w_solver_.as<double>());
-    std::vector<const double*> w(n_solver_w_);
-    for (int m = 0; m < n_solver_w_; m++) w[m] = w_solver_.as<double>() + (size_t)m * M_;
+    // monomer fields of the solver (lfts.py:534-545), replica r's at r n_solver_w
+    for (int r = 0; r < NR_; r++)
+        run_lincomb(r, w_map_, {}, {}, n_solver_w_, w_solver_.as<double>() + (size_t)r * n_solver_w_ * M_);
+    std::vector<const double*> w(n_solver_w_ * NR_);
+    for (size_t m = 0; m < w.size(); m++) w[m] = w_solver_.as<double>() + m * M_;
     solver_->compute_propagators(w.data(), nullptr, true);
     solver_->compute_concentrations();
-    for (int i = 0; i < S_; i++) {
-        const int o = term_off_[i], n = term_off_[i + 1] - o;
-        solver_->sum_concentrations(n, term_kind_.data() + o, term_index_.data() + o, term_scale_.data() + o,
-                                    phi_.as<double>() + (size_t)i * M_, true);
+    std::vector<int> idx(term_index_.size());
+    for (int r = 0; r < NR_; r++) {
+        for (size_t t = 0; t < idx.size(); t++)
+            idx[t] = term_index_[t] + r * (term_kind_[t] == FTS_SRC_BLOCK ? n_blocks_ : n_solvents_);
+        for (int i = 0; i < S_; i++) {
+            const int o = term_off_[i], n = term_off_[i + 1] - o;
+            solver_->sum_concentrations(n, term_kind_.data() + o, idx.data() + o, term_scale_.data() + o,
+                                        ph(r) + (size_t)i * M_, true);
+        }
     }
 }
 
 void SaddlePoint::field_moments(double* mean, double* mean_sq) {
     std::vector<const double*> rows;
-    for (int j = 0; j < S_; j++) rows.push_back(w_aux_.as<double>() + (size_t)j * M_);
+    for (int r = 0; r < NR_; r++)
+        for (int j = 0; j < S_; j++) rows.push_back(wa(r) + (size_t)j * M_);
+    const int n = (int)rows.size();
     double* st = stats_.as<double>();
     row_sums(rows, nullptr, false, st);
-    row_sums(rows, nullptr, true, st + S_);
-    std::vector<double> h(2 * S_);
-    FTS_HIP_CHECK(hipMemcpyAsync(h.data(), st, sizeof(double) * 2 * S_, hipMemcpyDeviceToHost, ctx_->stream));
+    row_sums(rows, nullptr, true, st + n);
+    std::vector<double> h(2 * n);
+    FTS_HIP_CHECK(hipMemcpyAsync(h.data(), st, sizeof(double) * 2 * n, hipMemcpyDeviceToHost, ctx_->stream));
     FTS_HIP_CHECK(hipStreamSynchronize(ctx_->stream));
-    for (int j = 0; j < S_; j++) {
-        mean[j] = h[j] / M_;
-        mean_sq[j] = h[S_ + j] / M_;
+    for (int k = 0; k < n; k++) {
+        mean[k] = h[k] / M_;
+        mean_sq[k] = h[n + k] / M_;
     }
 }
 
 // lfts.py:197-216: H = sum_j (h_sq_j <w_j^2> + h_lin_j <w_j>) - sum_p phi_p/alpha_p ln Q_p + h_const
-double SaddlePoint::hamiltonian() {
-    std::vector<double> mean(S_), mean_sq(S_);
-    field_moments(mean.data(), mean_sq.data());
+double SaddlePoint::hamiltonian(int rep) {
+    std::vector<const double*> rows;
+    for (int j = 0; j < S_; j++) rows.push_back(wa(rep) + (size_t)j * M_);
+    double* st = stats_.as<double>();
+    row_sums(rows, nullptr, false, st);
+    row_sums(rows, nullptr, true, st + S_);
+    std::vector<double> m(2 * S_);
+    FTS_HIP_CHECK(hipMemcpyAsync(m.data(), st, sizeof(double) * 2 * S_, hipMemcpyDeviceToHost, ctx_->stream));
+    FTS_HIP_CHECK(hipStreamSynchronize(ctx_->stream));
     double h = 0.0;
-    for (int j = 0; j < S_; j++) h += h_lin_[j] * mean[j];
-    for (int j = 0; j < S_; j++) h += h_sq_[j] * mean_sq[j];
+    for (int j = 0; j < S_; j++) h += h_lin_[j] * (m[j] / M_);
+    for (int j = 0; j < S_; j++) h += h_sq_[j] * (m[S_ + j] / M_);
     double hp = 0.0;
-    for (int p = 0; p < P_; p++) hp -= pol_weight_[p] * std::log(solver_->partition(p));
+    for (int p = 0; p < P_; p++) hp -= pol_weight_[p] * std::log(solver_->partition(p + rep * P_));
     return hp + h + h_const_;
 }
 
@@ -329,110 +355,134 @@ void SaddlePoint::find(double tolerance, int max_iter, int verbose, double* log,
                        double* hamiltonian_out) {
     FTS_REQUIRE(max_iter >= 1, "max_iter must be positive");
     hipStream_t st = ctx_->stream;
-    double err = 1e20;
-    double ham = std::nan("");
-    int it = 1;
-    am_->reset();
+    std::vector<double> err(NR_, 1e20), ham(NR_, std::nan(""));
+    std::vector<int> done(NR_, 0);   // iteration at which the replica converged, 0: not yet
+    for (auto* a : ams_) a->reset();
     const int row = 3 + P_ + I_;
-    std::vector<double> stats(2 * I_);
-    for (; it <= max_iter; it++) {
+    const int nrows = NR_ * I_;
+    std::vector<double> stats(2 * nrows);
+    bool contiguous = true;
+    for (int c = 1; c < I_; c++) contiguous = contiguous && imag_idx_[c] == imag_idx_[0] + c;
+    for (int it = 1; it <= max_iter; it++) {
         concentrations();
         // dH/dw of the imaginary fields, sign already flipped (lfts.py:219-245)
-        run_lincomb(h_cw_, h_cp_, h_c0_, I_, h_.as<double>());
         std::vector<const double*> rows;
-        for (int c = 0; c < I_; c++) rows.push_back(h_.as<double>() + (size_t)c * M_);
+        for (int r = 0; r < NR_; r++) {
+            double* hr = h_.as<double>() + (size_t)r * I_ * M_;
+            run_lincomb(r, h_cw_, h_cp_, h_c0_, I_, hr);
+            for (int c = 0; c < I_; c++) rows.push_back(hr + (size_t)c * M_);
+        }
         double* sd = stats_.as<double>();
         row_sums(rows, nullptr, false, sd);
-        row_sums(rows, sd, true, sd + I_);
-        FTS_HIP_CHECK(hipMemcpyAsync(stats.data(), sd, sizeof(double) * 2 * I_, hipMemcpyDeviceToHost, st));
+        row_sums(rows, sd, true, sd + nrows);
+        FTS_HIP_CHECK(hipMemcpyAsync(stats.data(), sd, sizeof(double) * 2 * nrows, hipMemcpyDeviceToHost, st));
         FTS_HIP_CHECK(hipStreamSynchronize(st));
-        const double old_err = err;
-        err = 0.0;
-        std::vector<double> err_i(I_);
-        for (int c = 0; c < I_; c++) {
-            err_i[c] = std::sqrt(stats[I_ + c] / M_);
-            if (c == 0 || err_i[c] > err || std::isnan(err_i[c])) err = err_i[c];
-        }
-        // the Hamiltonian and the printed row of lfts.py:834-849, at every iteration
-        // (verbose 2) or at the final one
-        const bool last = err < tolerance || it == max_iter;
-        if (verbose == 2 || last) {
-            ham = hamiltonian();
-            if (log && verbose >= 1 && (verbose == 2 || last)) {
-                double* r = log + (size_t)(verbose == 2 ? it - 1 : 0) * row;
-                r[0] = it;
-                r[1] = stats[I_ - 1] / M_;   // mass error: mean of the pressure-field derivative
-                r[2] = ham;
-                for (int p = 0; p < P_; p++) r[3 + p] = solver_->partition(p);
-                for (int c = 0; c < I_; c++) r[3 + P_ + c] = err_i[c];
-            }
-        }
-        if (err < tolerance) break;
-        // Anderson mixing of the imaginary fields with d = -dt_scaling * dH/dw (lfts.py:854-860)
-        Gather g{};
-        g.n = I_;
-        bool contiguous = true;
-        for (int c = 1; c < I_; c++) contiguous = contiguous && imag_idx_[c] == imag_idx_[0] + c;
-        double* w_im = contiguous ? w_aux_.as<double>() + (size_t)imag_idx_[0] * M_ : am_w_.as<double>();
-        for (int c = 0; c < I_; c++) {
-            g.src[c] = w_aux_.as<double>() + (size_t)imag_idx_[c] * M_;
-            g.h[c] = h_.as<double>() + (size_t)c * M_;
-            g.w_out[c] = contiguous ? nullptr : am_w_.as<double>() + (size_t)c * M_;
-            g.d_out[c] = am_d_.as<double>() + (size_t)c * M_;
-            g.scale[c] = dt_scaling_[imag_idx_[c]];
-        }
-        hipLaunchKernelGGL(gather_kernel, dim3(grid_blocks(M_)), dim3(256), 0, st, g, M_);
-        FTS_HIP_CHECK(hipGetLastError());
-        am_->calculate_new_fields(w_im, w_im, am_d_.as<double>(), old_err, err, true);
-        if (!contiguous) {
-            Scatter s{};
-            s.n = I_;
+        int active = 0;
+        for (int r = 0; r < NR_; r++) {
+            if (done[r]) continue;
+            const double old_err = err[r];
+            double e = 0.0;
+            std::vector<double> err_i(I_);
             for (int c = 0; c < I_; c++) {
-                s.src[c] = am_w_.as<double>() + (size_t)c * M_;
-                s.dst[c] = w_aux_.as<double>() + (size_t)imag_idx_[c] * M_;
+                err_i[c] = std::sqrt(stats[nrows + r * I_ + c] / M_);
+                if (c == 0 || err_i[c] > e || std::isnan(err_i[c])) e = err_i[c];
             }
-            hipLaunchKernelGGL(scatter_kernel, dim3(grid_blocks(M_)), dim3(256), 0, st, s, M_);
+            err[r] = e;
+            // the Hamiltonian and the printed row of lfts.py:834-849, at every iteration
+            // (verbose 2) or at the final one
+            const bool last = e < tolerance || it == max_iter;
+            if ((NR_ == 1 && verbose == 2) || last) {
+                ham[r] = hamiltonian(r);
+                if (NR_ == 1 && log && verbose >= 1) {
+                    double* lr = log + (size_t)(verbose == 2 ? it - 1 : 0) * row;
+                    lr[0] = it;
+                    lr[1] = stats[I_ - 1] / M_;   // mass error: mean of the pressure-field derivative
+                    lr[2] = ham[r];
+                    for (int p = 0; p < P_; p++) lr[3 + p] = solver_->partition(p);
+                    for (int c = 0; c < I_; c++) lr[3 + P_ + c] = err_i[c];
+                }
+            }
+            if (e < tolerance) {
+                done[r] = it;
+                continue;
+            }
+            active++;
+            // Anderson mixing of the imaginary fields with d = -dt_scaling * dH/dw (lfts.py:854-860)
+            double* am_w = am_w_.as<double>() + (size_t)r * I_ * M_;
+            double* am_d = am_d_.as<double>() + (size_t)r * I_ * M_;
+            const double* hr = h_.as<double>() + (size_t)r * I_ * M_;
+            Gather g{};
+            g.n = I_;
+            double* w_im = contiguous ? wa(r) + (size_t)imag_idx_[0] * M_ : am_w;
+            for (int c = 0; c < I_; c++) {
+                g.src[c] = wa(r) + (size_t)imag_idx_[c] * M_;
+                g.h[c] = hr + (size_t)c * M_;
+                g.w_out[c] = contiguous ? nullptr : am_w + (size_t)c * M_;
+                g.d_out[c] = am_d + (size_t)c * M_;
+                g.scale[c] = dt_scaling_[imag_idx_[c]];
+            }
+            hipLaunchKernelGGL(gather_kernel, dim3(grid_blocks(M_)), dim3(256), 0, st, g, M_);
             FTS_HIP_CHECK(hipGetLastError());
+            ams_[r]->calculate_new_fields(w_im, w_im, am_d, old_err, e, true);
+            if (!contiguous) {
+                Scatter sc{};
+                sc.n = I_;
+                for (int c = 0; c < I_; c++) {
+                    sc.src[c] = am_w + (size_t)c * M_;
+                    sc.dst[c] = wa(r) + (size_t)imag_idx_[c] * M_;
+                }
+                hipLaunchKernelGGL(scatter_kernel, dim3(grid_blocks(M_)), dim3(256), 0, st, sc, M_);
+                FTS_HIP_CHECK(hipGetLastError());
+            }
         }
+        if (active == 0) break;
     }
     // zero mean of the pressure field (lfts.py:865)
-    double* last_row = w_aux_.as<double>() + (size_t)(S_ - 1) * M_;
-    row_sums({last_row}, nullptr, false, stats_.as<double>());
-    hipLaunchKernelGGL(subtract_mean_kernel, dim3(grid_blocks(M_)), dim3(256), 0, st, last_row,
-                       stats_.as<const double>(), M_);
+    std::vector<const double*> last_rows;
+    for (int r = 0; r < NR_; r++) last_rows.push_back(wa(r) + (size_t)(S_ - 1) * M_);
+    row_sums(last_rows, nullptr, false, stats_.as<double>());
+    for (int r = 0; r < NR_; r++)
+        hipLaunchKernelGGL(subtract_mean_kernel, dim3(grid_blocks(M_)), dim3(256), 0, st,
+                           wa(r) + (size_t)(S_ - 1) * M_, stats_.as<const double>() + r, M_);
     FTS_HIP_CHECK(hipGetLastError());
     FTS_HIP_CHECK(hipStreamSynchronize(st));
-    if (n_iter) *n_iter = std::min(it, max_iter);
-    if (error) *error = err;
-    if (hamiltonian_out) *hamiltonian_out = ham;
+    for (int r = 0; r < NR_; r++) {
+        if (n_iter) n_iter[r] = done[r] ? done[r] : max_iter;
+        if (error) error[r] = err[r];
+        if (hamiltonian_out) hamiltonian_out[r] = ham[r];
+    }
 }
 
 void SaddlePoint::langevin_update(const double* noise, bool on_device, double dt) {
     if (R_ == 0) return;
     hipStream_t st = ctx_->stream;
-    FTS_HIP_CHECK(hipMemcpyAsync(noise_cur_.get(), noise, sizeof(double) * M_ * R_,
+    FTS_HIP_CHECK(hipMemcpyAsync(noise_cur_.get(), noise, sizeof(double) * M_ * R_ * NR_,
                                  on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st));
-    // dH/dw of the real fields (no sign flip) from the current w_aux and phi
-    run_lincomb(lam_cw_, lam_cp_, lam_c0_, R_, lam_.as<double>());
-    LmArgs a{};
-    a.n = R_;
-    for (int c = 0; c < R_; c++) {
-        const double s = dt_scaling_[real_idx_[c]];
-        a.w[c] = w_aux_.as<double>() + (size_t)real_idx_[c] * M_;
-        a.lambda[c] = lam_.as<double>() + (size_t)c * M_;
-        a.prev[c] = noise_prev_.as<double>() + (size_t)c * M_;
-        a.cur[c] = noise_cur_.as<double>() + (size_t)c * M_;
-        a.dt_s[c] = dt * s;
-        a.sqrt_s[c] = std::sqrt(s);
+    for (int rep = 0; rep < NR_; rep++) {
+        // dH/dw of the real fields (no sign flip) from the current w_aux and phi
+        double* lam = lam_.as<double>() + (size_t)rep * R_ * M_;
+        run_lincomb(rep, lam_cw_, lam_cp_, lam_c0_, R_, lam);
+        LmArgs a{};
+        a.n = R_;
+        for (int c = 0; c < R_; c++) {
+            const double s = dt_scaling_[real_idx_[c]];
+            a.w[c] = wa(rep) + (size_t)real_idx_[c] * M_;
+            a.lambda[c] = lam + (size_t)c * M_;
+            a.prev[c] = noise_prev_.as<double>() + ((size_t)rep * R_ + c) * M_;
+            a.cur[c] = noise_cur_.as<double>() + ((size_t)rep * R_ + c) * M_;
+            a.dt_s[c] = dt * s;
+            a.sqrt_s[c] = std::sqrt(s);
+        }
+        hipLaunchKernelGGL(lm_kernel, dim3(grid_blocks(M_)), dim3(256), 0, st, a, M_);
+        FTS_HIP_CHECK(hipGetLastError());
     }
-    hipLaunchKernelGGL(lm_kernel, dim3(grid_blocks(M_)), dim3(256), 0, st, a, M_);
-    FTS_HIP_CHECK(hipGetLastError());
     // the current noise becomes the previous one (lfts.py:712)
     std::swap(noise_prev_, noise_cur_);
     FTS_HIP_CHECK(hipStreamSynchronize(st));
 }
 
 void SaddlePoint::prepare_guess(Cnn& cnn) {
+    FTS_REQUIRE(NR_ == 1, "the learned guess runs on a single replica");
     FTS_REQUIRE(R_ == 1 && I_ == 1, "the learned guess maps one real field to one imaginary field");
     FTS_REQUIRE(cnn.n_grid() == M_, "the network's grid differs from the simulation grid");
     if (guess_old_.bytes() == 0) guess_old_.allocate(sizeof(double) * M_);
@@ -441,6 +491,7 @@ void SaddlePoint::prepare_guess(Cnn& cnn) {
 }
 
 void SaddlePoint::predict_guess(Cnn& cnn, bool delta) {
+    FTS_REQUIRE(NR_ == 1, "the learned guess runs on a single replica");
     FTS_REQUIRE(R_ == 1 && I_ == 1, "the learned guess maps one real field to one imaginary field");
     FTS_REQUIRE(cnn.n_grid() == M_, "the network's grid differs from the simulation grid");
     double* w_real = w_aux_.as<double>() + (size_t)real_idx_[0] * M_;
@@ -459,6 +510,7 @@ void SaddlePoint::predict_guess(Cnn& cnn, bool delta) {
 }
 
 double SaddlePoint::train_guess(Cnn& cnn, double lr, int iters, int replay) {
+    FTS_REQUIRE(NR_ == 1, "the learned guess runs on a single replica");
     FTS_REQUIRE(R_ == 1 && I_ == 1, "the learned guess maps one real field to one imaginary field");
     FTS_REQUIRE(cnn.n_grid() == M_, "the network's grid differs from the simulation grid");
     FTS_REQUIRE(iters >= 1 && replay >= 1, "iterations and replay size must be positive");

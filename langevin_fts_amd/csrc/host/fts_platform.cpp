@@ -137,8 +137,10 @@ void ComputationBox::zero_mean(double* g) const {
 
 // ------------------------------------------------------------------ PropagatorComputation
 PropagatorComputation::PropagatorComputation(ComputationBox* cb, Molecules* molecules, PropagatorAnalyzer* analyzer,
-                                             int precision, int n_streams)
-    : cb_(cb), molecules_(molecules), analyzer_(analyzer), precision_(precision), n_streams_(n_streams) {
+                                             int precision, int n_streams, int n_replicas)
+    : cb_(cb), molecules_(molecules), analyzer_(analyzer), precision_(precision), n_streams_(n_streams),
+      n_replicas_(n_replicas) {
+    if (n_replicas < 1) LFTS_THROW("n_replicas must be positive");
     if (!cb) LFTS_THROW("ComputationBox *cb is a null pointer");
     if (!molecules) LFTS_THROW("Molecules *molecules is a null pointer");
     if (!analyzer) LFTS_THROW("PropagatorAnalyzer *propagator_analyzer is a null pointer");
@@ -279,7 +281,8 @@ PropagatorComputation::PropagatorComputation(ComputationBox* cb, Molecules* mole
     pl.job_prop = ip(j_prop);
     pl.job_from = ip(j_from);
     pl.job_to = ip(j_to);
-    check_fts(fts_solver_create(cb->handle(), &pl, precision_, &solver_));
+    if (n_replicas_ == 1) check_fts(fts_solver_create(cb->handle(), &pl, precision_, &solver_));
+    else check_fts(fts_solver_create_replicas(cb->handle(), &pl, precision_, n_replicas_, &solver_));
 }
 
 PropagatorComputation::~PropagatorComputation() {
@@ -558,8 +561,13 @@ void AndersonMixing::calculate_new_fields(double* w_new, const double* w_current
 }
 
 // ------------------------------------------------------------------ L-FTS saddle point
-DeviceSaddlePoint::DeviceSaddlePoint(PropagatorComputation* solver, AndersonMixing* am, const FieldTheoryTables& t) {
-    if (!solver || !am) LFTS_THROW("solver and am are required");
+DeviceSaddlePoint::DeviceSaddlePoint(PropagatorComputation* solver, AndersonMixing* am, const FieldTheoryTables& t,
+                                     const std::vector<double>& am_params) {
+    if (!solver) LFTS_THROW("solver is required");
+    NR_ = solver->n_replicas();
+    if (NR_ == 1 && !am) LFTS_THROW("am is required");
+    if (NR_ > 1 && am_params.size() != 4)
+        LFTS_THROW("a replica solver needs am_params = [max_hist, start_error, mix_min, mix_init]");
     S_ = (int)t.monomer_types.size();
     R_ = (int)t.real_idx.size();
     I_ = (int)t.imag_idx.size();
@@ -581,7 +589,7 @@ DeviceSaddlePoint::DeviceSaddlePoint(PropagatorComputation* solver, AndersonMixi
     need(t.dt_scaling, (size_t)S_, "dt_scaling");
     need(t.h_lin, (size_t)S_, "h_lin");
     need(t.h_sq, (size_t)S_, "h_sq");
-    if (I_ * M_ != am->get_n_var())
+    if (NR_ == 1 && I_ * M_ != am->get_n_var())
         LFTS_THROW("Anderson mixing has n_var " + std::to_string(am->get_n_var()) + ", expected " +
                    std::to_string(I_ * M_));
     std::map<std::string, int> type_index;
@@ -648,8 +656,15 @@ DeviceSaddlePoint::DeviceSaddlePoint(PropagatorComputation* solver, AndersonMixi
     ft.h_const = t.h_const;
     ft.n_polymers = P_;
     ft.polymer_weight = weight.data();
-    check_fts(fts_saddle_create(hip_context(), solver->handle(), am->handle(), M_, &ft, &saddle_));
+    if (NR_ == 1)
+        check_fts(fts_saddle_create(hip_context(), solver->handle(), am->handle(), M_, &ft, &saddle_));
+    else
+        check_fts(fts_saddle_create_replicas(hip_context(), solver->handle(), NR_, solver->n_blocks_per_replica(),
+                                             mol->get_n_solvent_types(), (int)am_params[0], am_params[1],
+                                             am_params[2], am_params[3], M_, &ft, &saddle_));
 }
+
+void DeviceSaddlePoint::restore_replicas(const int* flags) { check_fts(fts_saddle_restore_replicas(saddle_, flags)); }
 
 DeviceSaddlePoint::~DeviceSaddlePoint() {
     if (saddle_) fts_saddle_destroy(saddle_);
@@ -730,6 +745,11 @@ static int lanes() {
     const char* env = std::getenv("LFTS_HIP_LANES");
     const int n = env ? std::atoi(env) : 8;
     return n >= 1 ? n : 8;
+}
+
+PropagatorComputation* AbstractFactory::create_replica_solver(ComputationBox* cb, Molecules* molecules,
+                                                              PropagatorAnalyzer* analyzer, int n_replicas) {
+    return new PropagatorComputation(cb, molecules, analyzer, precision_, lanes(), n_replicas);
 }
 
 PropagatorComputation* AbstractFactory::create_pseudospectral_solver(ComputationBox* cb, Molecules* molecules,

@@ -55,8 +55,10 @@ private:
 
 class PropagatorComputation {
 public:
+    // n_replicas > 1: that many independent copies of the chain set in one solver (replica L-FTS);
+    // the concentration and partition getters then report replica 0
     PropagatorComputation(ComputationBox* cb, Molecules* molecules, PropagatorAnalyzer* analyzer, int precision,
-                          int n_streams);
+                          int n_streams, int n_replicas = 1);
     ~PropagatorComputation();
     PropagatorComputation(const PropagatorComputation&) = delete;
     PropagatorComputation& operator=(const PropagatorComputation&) = delete;
@@ -91,6 +93,8 @@ public:
     void concentration_terms(const std::string& monomer, double scale, std::vector<int>& kinds,
                              std::vector<int>& idx, std::vector<double>& sc) const;
     fts_solver* handle() const { return solver_; }
+    int n_replicas() const { return n_replicas_; }
+    int n_blocks_per_replica() const { return (int)block_keys_.size(); }
     Molecules* molecules() const { return molecules_; }
     const std::vector<std::vector<Scheduler::Job>>& schedule() const { return schedule_; }
     int precision() const { return precision_; }
@@ -104,7 +108,7 @@ private:
     ComputationBox* cb_;
     Molecules* molecules_;
     PropagatorAnalyzer* analyzer_;
-    int precision_, n_streams_;
+    int precision_, n_streams_, n_replicas_ = 1;
     std::vector<std::string> monomers_;
     std::map<std::string, int> prop_index_;
     std::vector<std::string> qinit_names_;
@@ -168,7 +172,10 @@ private:
 
 class DeviceSaddlePoint {
 public:
-    DeviceSaddlePoint(PropagatorComputation* solver, AndersonMixing* am, const FieldTheoryTables& t);
+    // am: the Anderson mixing of a single replica; for a replica solver (n_replicas > 1) am is
+    // null and am_params = {max_hist, start_error, mix_min, mix_init} configure one per replica
+    DeviceSaddlePoint(PropagatorComputation* solver, AndersonMixing* am, const FieldTheoryTables& t,
+                      const std::vector<double>& am_params = {});
     ~DeviceSaddlePoint();
     DeviceSaddlePoint(const DeviceSaddlePoint&) = delete;
     DeviceSaddlePoint& operator=(const DeviceSaddlePoint&) = delete;
@@ -177,6 +184,7 @@ public:
     int n_imag() const { return I_; }
     int n_polymers() const { return P_; }
     int n_grid() const { return M_; }
+    int n_replicas() const { return NR_; }
     void set_fields(const double* w_aux);
     void get_fields(double* w_aux);
     void get_concentration(int type, double* phi);
@@ -184,7 +192,8 @@ public:
     void get_noise_prev(double* noise);
     void backup();
     void restore();
-    // returns (iterations, error level, hamiltonian); log rows as fts_saddle_find
+    void restore_replicas(const int* flags);
+    // returns (iterations, error level, hamiltonian) per replica; log rows as fts_saddle_find
     void find(double tolerance, int max_iter, int verbose, double* log, int* n_iter, double* error, double* ham);
     void langevin_update(const double* noise, double dt);
     void field_moments(double* mean, double* mean_sq);
@@ -193,7 +202,7 @@ public:
     double train_guess(LearnedGuess& g, double lr, int iters, int replay);
 
 private:
-    int S_, R_, I_, P_, M_;
+    int S_, R_, I_, P_, M_, NR_ = 1;
     fts_saddle* saddle_ = nullptr;
 };
 
@@ -206,6 +215,8 @@ public:
     Molecules* create_molecules_information(const std::string& chain_model, double ds,
                                             const std::map<std::string, double>& bond_lengths);
     PropagatorAnalyzer* create_propagator_analyzer(Molecules* molecules, bool aggregate);
+    PropagatorComputation* create_replica_solver(ComputationBox* cb, Molecules* molecules,
+                                                 PropagatorAnalyzer* analyzer, int n_replicas);
     PropagatorComputation* create_pseudospectral_solver(ComputationBox* cb, Molecules* molecules,
                                                         PropagatorAnalyzer* analyzer);
     PropagatorComputation* create_realspace_solver(ComputationBox* cb, Molecules* molecules,

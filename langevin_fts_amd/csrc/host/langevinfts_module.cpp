@@ -259,7 +259,7 @@ PYBIND11_MODULE(langevinfts, m) {
                          std::vector<double> real_cw, std::vector<double> real_cp, std::vector<double> real_c0,
                          std::vector<double> imag_cw, std::vector<double> imag_cp, std::vector<double> imag_c0,
                          std::vector<double> dt_scaling, std::vector<double> h_lin, std::vector<double> h_sq,
-                         double h_const) {
+                         double h_const, std::vector<double> am_params) {
                  FieldTheoryTables t;
                  t.monomer_types = std::move(monomer_types);
                  t.random_fraction = std::move(random_fraction);
@@ -276,35 +276,43 @@ PYBIND11_MODULE(langevinfts, m) {
                  t.h_lin = std::move(h_lin);
                  t.h_sq = std::move(h_sq);
                  t.h_const = h_const;
-                 return new DeviceSaddlePoint(solver, am, t);
+                 return new DeviceSaddlePoint(solver, am, t, am_params);
              }),
+             py::arg("solver"), py::arg("am"), py::arg("monomer_types"), py::arg("random_fraction"),
+             py::arg("matrix_a"), py::arg("real_idx"), py::arg("imag_idx"), py::arg("real_cw"), py::arg("real_cp"),
+             py::arg("real_c0"), py::arg("imag_cw"), py::arg("imag_cp"), py::arg("imag_c0"), py::arg("dt_scaling"),
+             py::arg("h_lin"), py::arg("h_sq"), py::arg("h_const"), py::arg("am_params") = std::vector<double>{},
              py::keep_alive<1, 2>(), py::keep_alive<1, 3>())
+        .def("get_n_replicas", &DeviceSaddlePoint::n_replicas)
         .def("get_n_types", &DeviceSaddlePoint::n_types)
         .def("get_n_real", &DeviceSaddlePoint::n_real)
         .def("get_n_imag", &DeviceSaddlePoint::n_imag)
         .def("set_fields",
              [](DeviceSaddlePoint& o, const darr& w) {
-                 const ssize_t n = (ssize_t)o.n_types() * o.n_grid();
+                 const ssize_t n = (ssize_t)o.n_replicas() * o.n_types() * o.n_grid();
                  if (w.size() != n)
                      LFTS_THROW("Size of w_aux (" + std::to_string(w.size()) + ") must be " + std::to_string(n));
                  o.set_fields(w.data());
              })
         .def("get_fields",
              [](DeviceSaddlePoint& o) {
-                 py::array_t<double> w({(ssize_t)o.n_types(), (ssize_t)o.n_grid()});
+                 std::vector<ssize_t> shape{(ssize_t)o.n_types(), (ssize_t)o.n_grid()};
+                 if (o.n_replicas() > 1) shape.insert(shape.begin(), (ssize_t)o.n_replicas());
+                 py::array_t<double> w(shape);
                  o.get_fields(w.mutable_data());
                  return w;
              })
         .def("get_concentration",
              [](DeviceSaddlePoint& o, int type) {
-                 if (type < 0 || type >= o.n_types()) LFTS_THROW("type index out of range");
+                 // type r n_types + i: monomer type i of replica r
+                 if (type < 0 || type >= o.n_replicas() * o.n_types()) LFTS_THROW("type index out of range");
                  py::array_t<double> phi(o.n_grid());
                  o.get_concentration(type, phi.mutable_data());
                  return phi;
              })
         .def("set_noise_prev",
              [](DeviceSaddlePoint& o, const darr& noise) {
-                 const ssize_t n = (ssize_t)o.n_real() * o.n_grid();
+                 const ssize_t n = (ssize_t)o.n_replicas() * o.n_real() * o.n_grid();
                  if (noise.size() != n)
                      LFTS_THROW("Size of normal_noise_prev (" + std::to_string(noise.size()) + ") must be " +
                                 std::to_string(n));
@@ -312,12 +320,19 @@ PYBIND11_MODULE(langevinfts, m) {
              })
         .def("get_noise_prev",
              [](DeviceSaddlePoint& o) {
-                 py::array_t<double> w({(ssize_t)o.n_real(), (ssize_t)o.n_grid()});
+                 std::vector<ssize_t> shape{(ssize_t)o.n_real(), (ssize_t)o.n_grid()};
+                 if (o.n_replicas() > 1) shape.insert(shape.begin(), (ssize_t)o.n_replicas());
+                 py::array_t<double> w(shape);
                  if (o.n_real() > 0) o.get_noise_prev(w.mutable_data());
                  return w;
              })
         .def("backup", &DeviceSaddlePoint::backup)
         .def("restore", &DeviceSaddlePoint::restore)
+        .def("restore_replicas",
+             [](DeviceSaddlePoint& o, std::vector<int> flags) {
+                 if ((int)flags.size() != o.n_replicas()) LFTS_THROW("one flag per replica is required");
+                 o.restore_replicas(flags.data());
+             })
         .def(
             "find_saddle_point",
             [](DeviceSaddlePoint& o, double tolerance, int max_iter, int verbose) {
@@ -325,15 +340,18 @@ PYBIND11_MODULE(langevinfts, m) {
                 const int rows = verbose == 2 ? std::max(max_iter, 1) : 1;
                 py::array_t<double> log({(ssize_t)rows, (ssize_t)row});
                 std::fill(log.mutable_data(), log.mutable_data() + (size_t)rows * row, 0.0);
-                int n_iter = 0;
-                double err = 0.0, ham = 0.0;
-                o.find(tolerance, max_iter, verbose, log.mutable_data(), &n_iter, &err, &ham);
-                return py::make_tuple(n_iter, err, ham, log);
+                const int R = o.n_replicas();
+                std::vector<int> n_iter(R, 0);
+                std::vector<double> err(R, 0.0), ham(R, 0.0);
+                o.find(tolerance, max_iter, R == 1 ? verbose : 0, R == 1 ? log.mutable_data() : nullptr,
+                       n_iter.data(), err.data(), ham.data());
+                if (R == 1) return py::tuple(py::make_tuple(n_iter[0], err[0], ham[0], log));
+                return py::tuple(py::make_tuple(n_iter, err, ham, py::none()));
             },
             py::arg("tolerance"), py::arg("max_iter"), py::arg("verbose") = 1)
         .def("langevin_update",
              [](DeviceSaddlePoint& o, const darr& noise, double dt) {
-                 const ssize_t n = (ssize_t)o.n_real() * o.n_grid();
+                 const ssize_t n = (ssize_t)o.n_replicas() * o.n_real() * o.n_grid();
                  if (noise.size() != n)
                      LFTS_THROW("Size of the noise (" + std::to_string(noise.size()) + ") must be " +
                                 std::to_string(n));
@@ -341,7 +359,8 @@ PYBIND11_MODULE(langevinfts, m) {
              })
         .def("field_moments",
              [](DeviceSaddlePoint& o) {
-                 std::vector<double> mean(o.n_types()), mean_sq(o.n_types());
+                 const int n = o.n_replicas() * o.n_types();
+                 std::vector<double> mean(n), mean_sq(n);
                  o.field_moments(mean.data(), mean_sq.data());
                  return py::make_tuple(mean, mean_sq);
              })
@@ -411,6 +430,9 @@ PYBIND11_MODULE(langevinfts, m) {
         .def("create_pseudospectral_solver", &AbstractFactory::create_pseudospectral_solver, py::keep_alive<0, 2>(),
              py::keep_alive<0, 3>(), py::keep_alive<0, 4>())
         .def("create_realspace_solver", &AbstractFactory::create_realspace_solver)
+        // n_replicas independent copies of the chains in one solver (replica L-FTS; an addition)
+        .def("create_replica_solver", &AbstractFactory::create_replica_solver, py::keep_alive<0, 2>(),
+             py::keep_alive<0, 3>(), py::keep_alive<0, 4>())
         .def("create_anderson_mixing", &AbstractFactory::create_anderson_mixing)
         .def("display_info", &AbstractFactory::display_info)
         .def("get_model_name", &AbstractFactory::get_model_name);

@@ -224,7 +224,12 @@ class LFTS:
     trains a periodic 3D CNN on the (w-, w+) saddle points of the first train_steps
     accepted Langevin steps and then starts every saddle point from its prediction."""
 
-    def __init__(self, params, random_seed=None):
+    def __init__(self, params, random_seed=None, n_replicas=1):
+        """n_replicas > 1: that many independent runs of the same system advance together,
+        their propagators batched into the same launches; replica r draws its noise from
+        PCG64(random_seed + r) (replica_step)."""
+        self.n_replicas = int(n_replicas)
+        assert self.n_replicas >= 1, "n_replicas must be positive"
         self.monomer_types = sorted(list(params["segment_lengths"].keys()))
         self.segment_lengths = copy.deepcopy(params["segment_lengths"])
         self.distinct_polymers = copy.deepcopy(params["distinct_polymers"])
@@ -289,11 +294,15 @@ class LFTS:
         for polymer in self.distinct_polymers:
             molecules.add_polymer(polymer["volume_fraction"], polymer["blocks_input"])
         analyzer = factory.create_propagator_analyzer(molecules, params.get("aggregate_propagator_computation", True))
-        solver = factory.create_pseudospectral_solver(cb, molecules, analyzer)
         n_grid = int(np.prod(params["nx"]))
-        am = factory.create_anderson_mixing(
-            len(self.mpt.aux_fields_imag_idx) * n_grid, params["am"]["max_hist"], params["am"]["start_error"],
-            params["am"]["mix_min"], params["am"]["mix_init"])
+        am_params = [params["am"]["max_hist"], params["am"]["start_error"], params["am"]["mix_min"],
+                     params["am"]["mix_init"]]
+        if self.n_replicas == 1:
+            solver = factory.create_pseudospectral_solver(cb, molecules, analyzer)
+            am = factory.create_anderson_mixing(len(self.mpt.aux_fields_imag_idx) * n_grid, *am_params)
+        else:
+            solver = factory.create_replica_solver(cb, molecules, analyzer, self.n_replicas)
+            am = None
         langevin_sigma = calculate_sigma(params["langevin"]["nbar"], params["langevin"]["dt"], n_grid,
                                          np.prod(params["lx"]))
 
@@ -304,6 +313,9 @@ class LFTS:
 
         self.random_bg = np.random.PCG64() if random_seed is None else np.random.PCG64(random_seed)
         self.random = np.random.Generator(self.random_bg)
+        if self.n_replicas > 1:
+            base = 0 if random_seed is None else random_seed
+            self.replica_random = [np.random.Generator(np.random.PCG64(base + r)) for r in range(self.n_replicas)]
 
         # the saddle point on the device
         mpt = self.mpt
@@ -318,7 +330,8 @@ class LFTS:
             solver, am, self.monomer_types, {k: dict(v) for k, v in self.random_fraction.items()},
             mpt.matrix_a.ravel().tolist(), mpt.aux_fields_real_idx, mpt.aux_fields_imag_idx,
             rcw.ravel().tolist(), rcp.ravel().tolist(), rc0.tolist(), icw.ravel().tolist(), icp.ravel().tolist(),
-            ic0.tolist(), self.dt_scaling.tolist(), h_lin.tolist(), h_sq.tolist(), float(mpt.h_const))
+            ic0.tolist(), self.dt_scaling.tolist(), h_lin.tolist(), h_sq.tolist(), float(mpt.h_const),
+            am_params if self.n_replicas > 1 else [])
 
         print("---------- Simulation Parameters ----------")
         print("Platform :", platform)
@@ -433,6 +446,24 @@ class LFTS:
 
     def guess_ready(self):
         return self.guess is not None and self.guess_trained_steps >= self.learned["train_steps"]
+
+    def replica_step(self):
+        """One Langevin step of every replica (n_replicas > 1): per-replica noise, one
+        batched saddle-point iteration over all replicas; replicas whose saddle point
+        missed the tolerance are put back to their previous state (their step is
+        skipped). Returns (iterations, error levels, accepted flags) per replica."""
+        assert self.n_replicas > 1, "replica_step needs n_replicas > 1"
+        R = len(self.mpt.aux_fields_real_idx)
+        M = self.cb.get_n_grid()
+        dev = self.saddle_device
+        dev.backup()
+        noise = np.stack([g.normal(0.0, self.langevin["sigma"], [R, M]) for g in self.replica_random])
+        dev.langevin_update(noise, self.langevin["dt"])
+        its, errs, _, _ = dev.find_saddle_point(self.saddle["tolerance"], self.saddle["max_iter"], 0)
+        ok = [not (np.isnan(e) or e >= self.saddle["tolerance"]) for e in errs]
+        if not all(ok):
+            dev.restore_replicas([0 if a else 1 for a in ok])
+        return its, errs, ok
 
     def learned_step(self, noise):
         """One Langevin step with the learned guess when it is trained, else with training
