@@ -302,7 +302,7 @@ def replica_bench(args, w, lf, lfts):
     }
     if kern:
         out["roofline"] = kernel_roofline(
-            kern, f"slab pass (slab_kernel_wl<double, 1, {w['nx'][0]}>: up to 16 propagators per launch)")
+            kern, f"slab pass (slab_kernel_wl<double, 1, {w['nx'][0]}>: up to 32 propagators per launch)")
     print(json.dumps(out), flush=True)
 
 

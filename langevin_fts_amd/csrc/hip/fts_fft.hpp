@@ -23,7 +23,7 @@
 namespace fts {
 
 constexpr int FFT_MAX_STAGES = 12;
-constexpr int MAX_BATCH = 16;   // work items per launch (passed by value)
+constexpr int MAX_BATCH = 32;   // work items per launch (passed by value, kernel arguments < 4 KiB)
 constexpr int FFT_MAXE = 8;     // complex values held per thread per FFT stage
 constexpr int SPLIT_MAXE = 4;   // complex pairs held per thread in the real-row split/merge
 

@@ -70,6 +70,11 @@ struct DotBatch {
     DotItem<T> it[MAX_BATCH];
 };
 
+// batches travel as kernel arguments
+static_assert(sizeof(SlabBatch<double>) <= 4000, "slab batch exceeds the kernel-argument budget");
+static_assert(sizeof(XBatch<double>) <= 4000, "pencil batch exceeds the kernel-argument budget");
+static_assert(sizeof(DotBatch<double>) <= 4000, "dot batch exceeds the kernel-argument budget");
+
 class Grid {
 public:
     Grid(fts_context* ctx, int dim, const int* nx, const double* lx, const double* mask);

@@ -266,4 +266,8 @@ static __global__ void __launch_bounds__(RED_BLOCKS) stress_stage2(const double*
     }
 }
 
+static_assert(sizeof(PhiBatch<double>) <= 4000, "phi batch exceeds the kernel-argument budget");
+static_assert(sizeof(StressBatch<double>) <= 4000, "stress batch exceeds the kernel-argument budget");
+static_assert(sizeof(SumBatch) <= 4000, "sum batch exceeds the kernel-argument budget");
+
 }  // namespace fts
