@@ -131,6 +131,15 @@ typedef struct {
  * precision FTS_FP64 or FTS_FP32 (propagators and FFTs; Q, concentrations and
  * stress accumulate in FP64 either way). The plan arrays are copied. */
 int fts_solver_create(fts_grid* g, const fts_plan* plan, int precision, fts_solver** out);
+/* boundary conditions of fts_solver_create_realspace (ComputationBox.h:15-20) */
+enum { FTS_BC_PERIODIC = 0, FTS_BC_REFLECTING = 1, FTS_BC_ABSORBING = 2 };
+/* CpuComputationContinuous constructed with method "realspace" (CpuComputationContinuous.cpp:25-26,
+ * CpuSolverReal.cpp:9-45; AbstractFactory::create_realspace_solver, AbstractFactory.h:53):
+ * continuous chains only, FP64, one Crank-Nicolson ADI step per contour step.
+ * bc: 2 dim codes FTS_BC_*, (low, high) per axis in axis order; both sides of an
+ * axis are periodic or neither is. fts_solver_compute_stress fails on such a solver,
+ * as CpuSolverReal::compute_single_segment_stress_continuous does. */
+int fts_solver_create_realspace(fts_grid* g, const fts_plan* plan, const int* bc, fts_solver** out);
 /* n_replicas independent copies of the plan in one solver (no reference counterpart:
  * the replica-batched L-FTS form). Replica r has its own monomer fields (index
  * r n_monomers + m in the w array of compute_propagators), blocks (r n_blocks + b),

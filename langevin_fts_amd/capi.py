@@ -65,6 +65,7 @@ def lib():
             "fts_grid_fft_forward": (i, [vp, i, vp, vp, i]),
             "fts_grid_fft_backward": (i, [vp, i, vp, vp, i]),
             "fts_solver_create": (i, [vp, ctypes.POINTER(FtsPlan), i, VPP]),
+            "fts_solver_create_realspace": (i, [vp, ctypes.POINTER(FtsPlan), _IP, VPP]),
             "fts_solver_destroy": (i, [vp]),
             "fts_solver_update_laplacian": (i, [vp]),
             "fts_solver_compute_propagators": (i, [vp, ctypes.POINTER(_DP), ctypes.POINTER(_DP), i]),
@@ -331,12 +332,21 @@ class Plan:
         self.solvents = list(solvents)
 
 
+BC_CODES = {"periodic": 0, "reflecting": 1, "absorbing": 2}
+
+
 class Solver:
-    def __init__(self, grid, plan, precision=FTS_FP64):
+    def __init__(self, grid, plan, precision=FTS_FP64, realspace_bc=None):
+        """realspace_bc: None for pseudo-spectral steps, else 2 per axis of
+        "periodic", "reflecting", "absorbing" (real-space steps, FP64)."""
         self.grid = grid
         self.plan = plan
         h = ctypes.c_void_p()
-        check(lib().fts_solver_create(grid.h, ctypes.byref(plan.struct), precision, ctypes.byref(h)))
+        if realspace_bc is None:
+            check(lib().fts_solver_create(grid.h, ctypes.byref(plan.struct), precision, ctypes.byref(h)))
+        else:
+            bc = np.array([BC_CODES[b] for b in realspace_bc], dtype=np.int32)
+            check(lib().fts_solver_create_realspace(grid.h, ctypes.byref(plan.struct), _ip(bc), ctypes.byref(h)))
         self.h = h
 
     def __del__(self):

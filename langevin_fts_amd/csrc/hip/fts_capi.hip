@@ -211,6 +211,23 @@ int fts_solver_create(fts_grid* g, const fts_plan* plan, int precision, fts_solv
     });
 }
 
+int fts_solver_create_realspace(fts_grid* g, const fts_plan* plan, const int* bc, fts_solver** out) {
+    return guarded([&] {
+        need(g, "grid");
+        need(plan, "plan");
+        need(bc, "bc");
+        need(out, "out");
+        auto* s = new fts_solver;
+        try {
+            s->s = fts::make_solver(g->g.get(), plan, FTS_FP64, bc);
+        } catch (...) {
+            delete s;
+            throw;
+        }
+        *out = s;
+    });
+}
+
 namespace {
 
 // R copies of a plan with their own monomer fields: replica r's propagator p is

@@ -258,6 +258,20 @@ Grid::Grid(fts_context* c, int d, const int* n, const double* l, const double* m
         FTS_HIP_CHECK(hipMemcpy(mask32.get(), m32.data(), sizeof(float) * M, hipMemcpyHostToDevice));
     }
 
+    partials.allocate(sizeof(double) * MAX_BATCH * RED_BLOCKS * 3);
+    scalars.allocate(sizeof(double) * MAX_BATCH);
+    scratch64.allocate(sizeof(double) * 2 * (size_t)M);
+    // the FFT plans serve the pseudo-spectral solvers only; a grid they cannot
+    // transform still serves the real-space solver, so the failure is kept for
+    // the first FFT use
+    try {
+        setup_fft();
+    } catch (const Error& e) {
+        fft_error = e.what();
+    }
+}
+
+void Grid::setup_fft() {
     ax = plan_axis(tnx[0]);
     ay = plan_axis(tnx[1]);
     az = plan_axis(tnx[2]);
@@ -314,10 +328,6 @@ Grid::Grid(fts_context* c, int d, const int* n, const double* l, const double* m
     // wave-local line transforms for those grids (env FTS_CT_KERNELS=1: the block-synchronous ones)
     wave_local = ct_n != 0 && !std::getenv("FTS_CT_KERNELS");
     if (ct_n) set_lds_attributes_ct();
-
-    partials.allocate(sizeof(double) * MAX_BATCH * RED_BLOCKS * 3);
-    scalars.allocate(sizeof(double) * MAX_BATCH);
-    scratch64.allocate(sizeof(double) * 2 * (size_t)M);
 }
 
 // ComputationBox.cpp:229-262
@@ -482,6 +492,7 @@ double Grid::integral(const double* a, bool on_device) {
 
 template <typename T>
 void Grid::fft_forward(const T* in, typename vec2<T>::type* out, int batch) {
+    require_fft();
     for (int b0 = 0; b0 < batch; b0 += MAX_BATCH) {
         const int nb = std::min(MAX_BATCH, batch - b0);
         SlabBatch<T> sb{};
@@ -501,6 +512,7 @@ void Grid::fft_forward(const T* in, typename vec2<T>::type* out, int batch) {
 
 template <typename T>
 void Grid::fft_backward(const typename vec2<T>::type* in, T* out, int batch) {
+    require_fft();
     // the pencil pass transforms in place, so the input is first copied to out's
     // companion workspace: use a temporary spectrum buffer
     using T2 = typename vec2<T>::type;

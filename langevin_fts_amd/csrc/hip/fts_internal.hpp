@@ -96,14 +96,20 @@ public:
     DeviceBuffer tw64[3], tw32[3];
     SlabGeom sg;
     XGeom xg;
-    int slab_threads, x_threads;
-    int slab_maxr;   // real values per thread in the slab kernel (8 or 16)
-    bool big;        // some axis needs a radix outside {2, 3, 4, 5, 8}
-    int ct_n;        // N of the compile-time specialised kernels for an N^3 grid, 0: runtime kernels
-    bool wave_local; // N^3 grids: one wavefront per line transform
+    int slab_threads = 0, x_threads = 0;
+    int slab_maxr = 8;   // real values per thread in the slab kernel (8 or 16)
+    bool big = false;        // some axis needs a radix outside {2, 3, 4, 5, 8}
+    int ct_n = 0;        // N of the compile-time specialised kernels for an N^3 grid, 0: runtime kernels
+    bool wave_local = false; // N^3 grids: one wavefront per line transform
     DeviceBuffer partials;   // reduction scratch: MAX_BATCH * RED_BLOCKS doubles
     DeviceBuffer scalars;    // reduction results: MAX_BATCH doubles
     DeviceBuffer scratch64;  // general FP64 scratch of 2 fields
+
+    std::string fft_error;   // why this grid has no FFT plans ("" when it has them)
+    void setup_fft();
+    void require_fft() const {
+        if (!fft_error.empty()) throw Error(fft_error);
+    }
 
     hipStream_t stream() const { return ctx->stream; }
     template <typename T>
@@ -163,7 +169,8 @@ public:
     virtual void propagator(int prop, int n, int half, double* out, bool on_device) = 0;
 };
 
-std::unique_ptr<SolverBase> make_solver(Grid* g, const fts_plan* plan, int precision);
+// bc: NULL for pseudo-spectral steps, else 2 dim FTS_BC_* codes (real-space steps, FP64 continuous)
+std::unique_ptr<SolverBase> make_solver(Grid* g, const fts_plan* plan, int precision, const int* bc = nullptr);
 
 class AndersonMixingDevice {
 public:

@@ -5,11 +5,14 @@
 // for what is computed; the batching is this file's own: all jobs of one schedule
 // slice (Scheduler.cpp:137-188) advance together, one launch per pass for the
 // whole slice, and consecutive steps of a propagator share a slab pass.
+// With boundary conditions given, continuous chains take real-space
+// Crank-Nicolson steps instead (CpuComputationContinuous.cpp:25-26).
 #include <algorithm>
 #include <cmath>
 #include <map>
 
 #include "fts_internal.hpp"
+#include "fts_real.hpp"
 #include "fts_reduce.hpp"
 
 namespace fts {
@@ -21,8 +24,11 @@ class Solver final : public SolverBase {
     using T2 = typename vec2<T>::type;
 
 public:
-    Solver(Grid* grid, const fts_plan* pl);
-    void update_laplacian() override { bt_.build(*g_, bsq_, ds_); }
+    Solver(Grid* grid, const fts_plan* pl, const int* bc);
+    void update_laplacian() override {
+        if (real_) build_real_tables();
+        else bt_.build(*g_, bsq_, ds_);
+    }
     void compute_propagators(const double* const* w, const double* const* q_init, bool on_device) override;
     void compute_concentrations() override;
     void compute_stress() override;
@@ -47,7 +53,7 @@ private:
         double bsq, weight;
         int table, block;
     };
-    enum { L_INIT, L_SLAB_FWD, L_SLAB_INV1, L_SLAB_INV2, L_XPASS };
+    enum { L_INIT, L_SLAB_FWD, L_SLAB_INV1, L_SLAB_INV2, L_XPASS, L_REAL };
     struct Launch {
         int type, idx;
     };
@@ -68,6 +74,9 @@ private:
     void emit_slab(int mode, const std::vector<SlabItem<T>>& items);
     void emit_x(const std::vector<XItem<T>>& items);
     void emit_init(const std::vector<InitItem<T>>& items);
+    void emit_real(const std::vector<RealItem>& items);
+    void build_real_tables();
+    void launch_real(const RealBatch& b);
     void build_slice(const std::vector<std::array<int, 3>>& jobs);
     void build_stress_terms();
     void run_launches();
@@ -95,6 +104,12 @@ private:
     std::vector<InitBatch<T>> inits_;
     std::vector<SlabBatch<T>> slabs_;
     std::vector<XBatch<T>> xs_;
+    // real-space Crank-Nicolson form
+    bool real_ = false;
+    std::vector<int> bc_;
+    RealGeom rg_{};
+    DeviceBuffer rtab_;
+    std::vector<RealBatch> reals_;
     std::vector<int> part_block_;   // per polymer: block whose slices give Q, -1 none
     std::vector<STerm> sterms_;
     std::vector<double> dqdl_;
@@ -103,8 +118,24 @@ private:
 };
 
 template <typename T>
-Solver<T>::Solver(Grid* grid, const fts_plan* pl) : g_(grid) {
+Solver<T>::Solver(Grid* grid, const fts_plan* pl, const int* bc) : g_(grid) {
     model_ = pl->model;
+    if (bc) {
+        // CpuSolverReal.cpp:16-17
+        FTS_REQUIRE(model_ == FTS_CONTINUOUS, "Real-space method only support 'continuous' chain model.");
+        FTS_REQUIRE((std::is_same<T, double>::value), "the real-space method runs in FP64");
+        real_ = true;
+        for (int i = 0; i < 2 * g_->dim; i++) {
+            FTS_REQUIRE(bc[i] >= BC_PERIODIC && bc[i] <= BC_ABSORBING, "invalid boundary condition code");
+            bc_.push_back(bc[i]);
+        }
+        for (int a = 0; a < g_->dim; a++)
+            FTS_REQUIRE((bc_[2 * a] == BC_PERIODIC) == (bc_[2 * a + 1] == BC_PERIODIC),
+                        "If one side imposes a periodic boundary condition, the other side must also be a periodic "
+                        "boundary condition.");
+    } else {
+        g_->require_fft();
+    }
     ds_ = pl->ds;
     M_ = g_->M;
     Mc_ = g_->Mc;
@@ -182,7 +213,28 @@ Solver<T>::Solver(Grid* grid, const fts_plan* pl) : g_(grid) {
     Qd_.allocate(sizeof(double) * (vf_.size() + svf_.size() + 1));
     Qh_.assign(vf_.size() + svf_.size(), 0.0);
     dqdl_.assign(vf_.size() * 3, 0.0);
-    bt_.build(*g_, bsq_, ds_);
+    if (real_) {
+        rg_.D = g_->dim;
+        rg_.M = M_;
+        for (int a = 0; a < 3; a++) rg_.tn[a] = g_->tnx[a];
+        rg_.st[2] = 1;
+        rg_.st[1] = rg_.tn[2];
+        rg_.st[0] = (long long)rg_.tn[1] * rg_.tn[2];
+        int off = 0;
+        for (int a = 0; a < rg_.D; a++) {
+            rg_.ax[a].n = g_->nx[a];
+            rg_.ax[a].iax = a + (3 - rg_.D);
+            rg_.ax[a].periodic = bc_[2 * a] == BC_PERIODIC;
+            rg_.ax[a].off = off;
+            off += RT_ROWS * g_->nx[a] + 2;
+        }
+        rg_.tab_stride = off;
+        rg_.half_ds = 0.5 * ds_;
+        rtab_.allocate(sizeof(double) * (size_t)off * n_mon_);
+        build_real_tables();
+    } else {
+        bt_.build(*g_, bsq_, ds_);
+    }
 
     // partition segment: first block of each polymer in block order (discrete: with
     // n_left != 0), CpuComputationContinuous.cpp:67-91, CpuComputationDiscrete.cpp:85-115
@@ -229,6 +281,98 @@ void Solver<T>::emit_init(const std::vector<InitItem<T>>& items) {
         for (int k = 0; k < b.n; k++) b.it[k] = items[i + k];
         launches_.push_back({L_INIT, (int)inits_.size()});
         inits_.push_back(b);
+    }
+}
+
+template <typename T>
+void Solver<T>::emit_real(const std::vector<RealItem>& items) {
+    for (size_t i = 0; i < items.size(); i += MAX_BATCH) {
+        RealBatch b{};
+        b.n = (int)std::min<size_t>(MAX_BATCH, items.size() - i);
+        for (int k = 0; k < b.n; k++) b.it[k] = items[i + k];
+        launches_.push_back({L_REAL, (int)reals_.size()});
+        reals_.push_back(b);
+    }
+}
+
+// Per monomer and axis: FiniteDifference.cpp:6-101 rows (l, d, h), then the
+// factors of the system that is solved along the axis. Periodic axes: the cyclic
+// system A = T + u v^T with gamma = -d[0], T = A with d[0] - gamma and
+// d[n-1] - h[n-1] l[0] / gamma, u = (gamma, 0, .., h[n-1]), v = (1, 0, .., l[0] / gamma);
+// z = T^-1 u and 1 / (1 + v.z) are tabulated.
+template <typename T>
+void Solver<T>::build_real_tables() {
+    std::vector<double> tab((size_t)rg_.tab_stride * n_mon_, 0.0);
+    for (int m = 0; m < n_mon_; m++) {
+        for (int a = 0; a < rg_.D; a++) {
+            const RealAxis& A = rg_.ax[a];
+            const int n = A.n;
+            double* t = tab.data() + (size_t)m * rg_.tab_stride + A.off;
+            double *l = t + RT_L * n, *d = t + RT_D * n, *h = t + RT_H * n;
+            double *inv = t + RT_INV * n, *c = t + RT_C * n, *z = t + RT_Z * n;
+            const double f = bsq_[m] * ds_ / (g_->dx[a] * g_->dx[a] * 6.0);
+            for (int i = 0; i < n; i++) {
+                l[i] = -0.5 * f;
+                d[i] = 1.0 + f;
+                h[i] = -0.5 * f;
+            }
+            const int lo = bc_[2 * a], hi = bc_[2 * a + 1];
+            if (lo == BC_REFLECTING) {
+                d[0] += l[0];
+                l[0] = 0.0;
+            } else if (lo == BC_ABSORBING) {
+                l[0] = 0.0;
+            }
+            if (hi == BC_REFLECTING) {
+                d[n - 1] += h[n - 1];
+                h[n - 1] = 0.0;
+            } else if (hi == BC_ABSORBING) {
+                h[n - 1] = 0.0;
+            }
+            std::vector<double> dt(d, d + n), u(n, 0.0);
+            double gamma = 0.0;
+            if (A.periodic) {
+                gamma = -d[0];
+                dt[0] -= gamma;
+                dt[n - 1] -= h[n - 1] * l[0] / gamma;
+                u[0] += gamma;
+                u[n - 1] += h[n - 1];
+            }
+            inv[0] = 1.0 / dt[0];
+            for (int i = 1; i < n; i++) {
+                c[i - 1] = h[i - 1] * inv[i - 1];
+                inv[i] = 1.0 / (dt[i] - l[i] * c[i - 1]);
+            }
+            if (A.periodic) {
+                std::vector<double> y(n);
+                y[0] = u[0] * inv[0];
+                for (int i = 1; i < n; i++) y[i] = (u[i] - l[i] * y[i - 1]) * inv[i];
+                z[n - 1] = y[n - 1];
+                for (int i = n - 2; i >= 0; i--) z[i] = y[i] - c[i] * z[i + 1];
+                const double v = l[0] / gamma;
+                t[RT_ROWS * n] = v;
+                t[RT_ROWS * n + 1] = 1.0 / (1.0 + z[0] + v * z[n - 1]);
+            }
+        }
+    }
+    FTS_HIP_CHECK(hipMemcpy(rtab_.get(), tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice));
+}
+
+template <typename T>
+void Solver<T>::launch_real(const RealBatch& b) {
+    const int pb = std::min<long long>(1024, ceil_div((int)std::min<long long>(M_, 1 << 30), 256));
+    hipLaunchKernelGGL(real_pre_kernel, dim3(pb, b.n), dim3(256), 0, g_->stream(), b, rg_);
+    FTS_HIP_CHECK(hipGetLastError());
+    const double* tab = rtab_.as<double>();
+    for (int a = 0; a < rg_.D; a++) {
+        const int lines = M_ / rg_.ax[a].n;
+        const dim3 grid(ceil_div(lines, 256), b.n);
+        const bool first = a == 0, last = a == rg_.D - 1;
+        if (first && last) hipLaunchKernelGGL((real_sweep_kernel<true, true>), grid, dim3(256), 0, g_->stream(), b, rg_, tab, a);
+        else if (first) hipLaunchKernelGGL((real_sweep_kernel<true, false>), grid, dim3(256), 0, g_->stream(), b, rg_, tab, a);
+        else if (last) hipLaunchKernelGGL((real_sweep_kernel<false, true>), grid, dim3(256), 0, g_->stream(), b, rg_, tab, a);
+        else hipLaunchKernelGGL((real_sweep_kernel<false, false>), grid, dim3(256), 0, g_->stream(), b, rg_, tab, a);
+        FTS_HIP_CHECK(hipGetLastError());
     }
 }
 
@@ -356,13 +500,26 @@ void Solver<T>::build_slice(const std::vector<std::array<int, 3>>& jobs) {
     for (int t = 0; t < D; t++) {
         std::vector<SlabItem<T>> fw, inv1, inv2;
         std::vector<XItem<T>> x1, x2;
+        std::vector<RealItem> ri;
         for (auto& r : runs) {
             const int start = D - r.cnt;
             if (t < start) continue;
             const int n = r.n0 + (t - start);
             const int m = props_[r.p].monomer;
             const bool last = t == D - 1;
-            if (model_ == FTS_CONTINUOUS) {
+            if (real_) {
+                if constexpr (std::is_same<T, double>::value) {
+                    RealItem it{};
+                    it.qin = q(r.p, n);
+                    it.qe = reinterpret_cast<double*>(slot(2 * r.slot));
+                    it.acc = reinterpret_cast<double*>(slot(2 * r.slot + 1));
+                    it.qout = q(r.p, n + 1);
+                    it.w = w(m);
+                    it.mask = mask();
+                    it.monomer = m;
+                    ri.push_back(it);
+                }
+            } else if (model_ == FTS_CONTINUOUS) {
                 T2* c0 = slot(2 * r.slot);
                 T2* c1 = slot(2 * r.slot + 1);
                 if (t == start) {
@@ -416,6 +573,10 @@ void Solver<T>::build_slice(const std::vector<std::array<int, 3>>& jobs) {
                 b.forward = last ? 0 : 1;
                 inv1.push_back(b);
             }
+        }
+        if (real_) {
+            emit_real(ri);
+            continue;
         }
         emit_slab(SLAB_FWD, fw);
         emit_x(x1);
@@ -532,6 +693,7 @@ void Solver<T>::run_launches() {
             case L_SLAB_INV1: g_->launch_slab<T>(SLAB_INV1, slabs_[L.idx]); break;
             case L_SLAB_INV2: g_->launch_slab<T>(SLAB_INV2, slabs_[L.idx]); break;
             case L_XPASS: g_->launch_xpass<T>(xs_[L.idx], bx, by, bz); break;
+            case L_REAL: launch_real(reals_[L.idx]); break;
         }
     }
 }
@@ -629,6 +791,8 @@ void Solver<T>::compute_concentrations() {
 
 template <typename T>
 void Solver<T>::compute_stress() {
+    // CpuSolverReal.cpp:435-450
+    if (real_) throw Error("Currently, the real-space method does not support stress computation.");
     const int K = std::min(n_ws_ / 2, MAX_BATCH);   // terms per chunk: 2K spectra, K reduction items
     std::vector<double> seg(sterms_.size() * 3, 0.0);
     StressGeom sg{};
@@ -768,11 +932,12 @@ void Solver<T>::propagator(int prop, int n, int hf, double* out, bool on_device)
 
 }  // namespace
 
-std::unique_ptr<SolverBase> make_solver(Grid* g, const fts_plan* plan, int precision) {
-    if (precision == FTS_FP64) return std::unique_ptr<SolverBase>(new Solver<double>(g, plan));
+std::unique_ptr<SolverBase> make_solver(Grid* g, const fts_plan* plan, int precision, const int* bc) {
+    if (precision == FTS_FP64) return std::unique_ptr<SolverBase>(new Solver<double>(g, plan, bc));
     if (precision == FTS_FP32) {
+        FTS_REQUIRE(!bc, "the real-space method runs in FP64");
         FTS_REQUIRE(!g->big, "FP32 propagators need grid sizes whose prime factors are 2, 3 and 5");
-        return std::unique_ptr<SolverBase>(new Solver<float>(g, plan));
+        return std::unique_ptr<SolverBase>(new Solver<float>(g, plan, nullptr));
     }
     throw Error("precision must be FTS_FP64 or FTS_FP32");
 }

@@ -137,15 +137,21 @@ void ComputationBox::zero_mean(double* g) const {
 
 // ------------------------------------------------------------------ PropagatorComputation
 PropagatorComputation::PropagatorComputation(ComputationBox* cb, Molecules* molecules, PropagatorAnalyzer* analyzer,
-                                             int precision, int n_streams, int n_replicas)
-    : cb_(cb), molecules_(molecules), analyzer_(analyzer), precision_(precision), n_streams_(n_streams),
-      n_replicas_(n_replicas) {
+                                             int precision, int n_streams, int n_replicas, bool realspace)
+    : cb_(cb), molecules_(molecules), analyzer_(analyzer), precision_(realspace ? FTS_FP64 : precision),
+      n_streams_(n_streams), n_replicas_(n_replicas), realspace_(realspace) {
     if (n_replicas < 1) LFTS_THROW("n_replicas must be positive");
     if (!cb) LFTS_THROW("ComputationBox *cb is a null pointer");
     if (!molecules) LFTS_THROW("Molecules *molecules is a null pointer");
     if (!analyzer) LFTS_THROW("PropagatorAnalyzer *propagator_analyzer is a null pointer");
-    if (!cb->is_periodic())
+    if (realspace) {
+        // CpuSolverReal.cpp:16-17
+        if (molecules->get_model_name() != "continuous")
+            LFTS_THROW("Real-space method only support 'continuous' chain model.");
+        if (n_replicas != 1) LFTS_THROW("the real-space solver has no replica form");
+    } else if (!cb->is_periodic()) {
         LFTS_THROW("Currently, pseudo-spectral method only supports periodic boundary conditions");
+    }
     if (analyzer->get_model_name() != molecules->get_model_name())
         LFTS_THROW("The chain model of the analyzer and of the molecules differ.");
     auto& props = analyzer->get_computation_propagators();
@@ -281,7 +287,12 @@ PropagatorComputation::PropagatorComputation(ComputationBox* cb, Molecules* mole
     pl.job_prop = ip(j_prop);
     pl.job_from = ip(j_from);
     pl.job_to = ip(j_to);
-    if (n_replicas_ == 1) check_fts(fts_solver_create(cb->handle(), &pl, precision_, &solver_));
+    if (realspace_) {
+        std::vector<int> bc;
+        for (const auto& b : cb->get_boundary_conditions())
+            bc.push_back(b == "periodic" ? FTS_BC_PERIODIC : b == "reflecting" ? FTS_BC_REFLECTING : FTS_BC_ABSORBING);
+        check_fts(fts_solver_create_realspace(cb->handle(), &pl, bc.data(), &solver_));
+    } else if (n_replicas_ == 1) check_fts(fts_solver_create(cb->handle(), &pl, precision_, &solver_));
     else check_fts(fts_solver_create_replicas(cb->handle(), &pl, precision_, n_replicas_, &solver_));
 }
 
@@ -757,8 +768,10 @@ PropagatorComputation* AbstractFactory::create_pseudospectral_solver(Computation
     return new PropagatorComputation(cb, molecules, analyzer, precision_, lanes());
 }
 
-PropagatorComputation* AbstractFactory::create_realspace_solver(ComputationBox*, Molecules*, PropagatorAnalyzer*) {
-    LFTS_THROW("The real-space solver is not part of the hip platform.");
+// CpuComputationContinuous(cb, molecules, analyzer, "realspace") (CpuComputationContinuous.cpp:25-26)
+PropagatorComputation* AbstractFactory::create_realspace_solver(ComputationBox* cb, Molecules* molecules,
+                                                                PropagatorAnalyzer* analyzer) {
+    return new PropagatorComputation(cb, molecules, analyzer, FTS_FP64, lanes(), 1, true);
 }
 
 AndersonMixing* AbstractFactory::create_anderson_mixing(int n_var, int max_hist, double start_error, double mix_min,

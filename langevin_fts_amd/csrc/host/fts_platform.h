@@ -36,6 +36,7 @@ public:
     double get_volume() const { return volume_; }
     const double* get_mask() const { return mask_.empty() ? nullptr : mask_.data(); }
     bool is_periodic() const;
+    std::vector<std::string> get_boundary_conditions() const { return bc_; }
     void set_lx(std::vector<double> new_lx);
     double integral(const double* g) const;
     double inner_product(const double* g, const double* h) const;
@@ -56,9 +57,10 @@ private:
 class PropagatorComputation {
 public:
     // n_replicas > 1: that many independent copies of the chain set in one solver (replica L-FTS);
-    // the concentration and partition getters then report replica 0
+    // the concentration and partition getters then report replica 0.
+    // realspace: Crank-Nicolson steps under the box's boundary conditions (FP64, continuous chains)
     PropagatorComputation(ComputationBox* cb, Molecules* molecules, PropagatorAnalyzer* analyzer, int precision,
-                          int n_streams, int n_replicas = 1);
+                          int n_streams, int n_replicas = 1, bool realspace = false);
     ~PropagatorComputation();
     PropagatorComputation(const PropagatorComputation&) = delete;
     PropagatorComputation& operator=(const PropagatorComputation&) = delete;
@@ -109,6 +111,7 @@ private:
     Molecules* molecules_;
     PropagatorAnalyzer* analyzer_;
     int precision_, n_streams_, n_replicas_ = 1;
+    bool realspace_ = false;
     std::vector<std::string> monomers_;
     std::map<std::string, int> prop_index_;
     std::vector<std::string> qinit_names_;

@@ -429,7 +429,8 @@ PYBIND11_MODULE(langevinfts, m) {
         // the solver keeps the box, molecules and analyzer it was built on alive
         .def("create_pseudospectral_solver", &AbstractFactory::create_pseudospectral_solver, py::keep_alive<0, 2>(),
              py::keep_alive<0, 3>(), py::keep_alive<0, 4>())
-        .def("create_realspace_solver", &AbstractFactory::create_realspace_solver)
+        .def("create_realspace_solver", &AbstractFactory::create_realspace_solver, py::keep_alive<0, 2>(),
+             py::keep_alive<0, 3>(), py::keep_alive<0, 4>())
         // n_replicas independent copies of the chains in one solver (replica L-FTS; an addition)
         .def("create_replica_solver", &AbstractFactory::create_replica_solver, py::keep_alive<0, 2>(),
              py::keep_alive<0, 3>(), py::keep_alive<0, 4>())

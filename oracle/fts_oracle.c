@@ -12,6 +12,8 @@
  *   PropagatorComputation.cpp:35-52   stress normalisation by phi/alpha/Q
  *   ComputationBox.cpp:10-308         dv, volume, integral, inner products
  *   CpuAndersonMixing.cpp:76-161, AndersonMixing.cpp:19-47, CircularBuffer.cpp:4-54
+ *   FiniteDifference.cpp:6-101, CpuSolverReal.cpp:96-549
+ *                                     real-space Crank-Nicolson ADI step (continuous)
  *
  * The FFT is this file's own mixed-radix Cooley-Tukey (recursive decimation in
  * time, exact root table), not MKL; it is pinned against numpy.fft.rfftn and
@@ -305,6 +307,10 @@ struct orc_solver {
     double* dqdl;
     cplx *k1, *k2;
     double *r1, *r2;
+    int real;     /* real-space Crank-Nicolson steps instead of pseudo-spectral ones */
+    int bc[6];    /* ORC_BC_* per side, reference axis order */
+    double* lap;  /* per monomer: dim axes x {l, d, h} x nmax */
+    int nmax;
 };
 
 static int has_je(const orc_plan* pl, int p, int n) {
@@ -404,6 +410,7 @@ void orc_solver_destroy(orc_solver* s) {
     free(s->k2);
     free(s->r1);
     free(s->r2);
+    free(s->lap);
     free(s);
 }
 
@@ -460,11 +467,198 @@ static void half_bond(orc_solver* s, const double* qin, double* qout, int m) {
     orc_fft_backward(g, (double*)s->k1, qout);
 }
 
+
+/* ------------------------------------------------------------------ real space */
+/* FiniteDifference.cpp:6-101: per axis, l = h = -f/2 and d = 1 + f with
+ * f = b^2 ds / (6 dx^2); a reflecting side folds l (or h) into d, an absorbing side
+ * drops it. tables: dim blocks of {l, d, h}, each nx[axis] long. */
+void orc_laplacian(const orc_grid* g, const int* bc, double bsq, double ds, double* tables) {
+    double* t = tables;
+    for (int a = 0; a < g->dim; a++) {
+        const int n = g->nx[a];
+        const double f = bsq * ds / (g->dx[a] * g->dx[a] * 6.0);
+        double *l = t, *d = t + n, *h = t + 2 * n;
+        for (int i = 0; i < n; i++) {
+            l[i] = -0.5 * f;
+            d[i] = 1.0 + f;
+            h[i] = -0.5 * f;
+        }
+        if (bc[2 * a] == ORC_BC_REFLECTING) {
+            d[0] += l[0];
+            l[0] = 0.0;
+        } else if (bc[2 * a] == ORC_BC_ABSORBING) {
+            l[0] = 0.0;
+        }
+        if (bc[2 * a + 1] == ORC_BC_REFLECTING) {
+            d[n - 1] += h[n - 1];
+            h[n - 1] = 0.0;
+        } else if (bc[2 * a + 1] == ORC_BC_ABSORBING) {
+            h[n - 1] = 0.0;
+        }
+        t += 3 * n;
+    }
+}
+
+/* CpuSolverReal.cpp:464-491: Thomas algorithm for rows l[i] x[i-1] + d[i] x[i] + h[i] x[i+1] = rhs[i]
+ * (l[0] and h[n-1] unused); x is written with the given stride. */
+void orc_tridiagonal(const double* l, const double* d, const double* h, double* x, int stride, const double* rhs,
+                     int n) {
+    double* c = (double*)malloc(sizeof(double) * (n > 1 ? n : 1));
+    double den = d[0];
+    x[0] = rhs[0] / den;
+    for (int i = 1; i < n; i++) {
+        c[i - 1] = h[i - 1] / den;
+        den = d[i] - l[i] * c[i - 1];
+        x[(long)i * stride] = (rhs[i] - l[i] * x[(long)(i - 1) * stride]) / den;
+    }
+    for (int i = n - 2; i >= 0; i--) x[(long)i * stride] -= c[i] * x[(long)(i + 1) * stride];
+    free(c);
+}
+
+/* CpuSolverReal.cpp:493-549: the cyclic system (row 0 couples to x[n-1] through l[0],
+ * row n-1 to x[0] through h[n-1]) by the Sherman-Morrison formula with gamma = 1:
+ * solve the tridiagonal system with d[0] - 1 and d[n-1] - h[n-1] l[0] for the right
+ * side and for u = (1, 0, ..., 0, h[n-1]), then x = y - z (y0 + l0 y_{n-1}) / (1 + z0 + l0 z_{n-1}). */
+void orc_tridiagonal_periodic(const double* l, const double* d, const double* h, double* x, int stride,
+                              const double* rhs, int n) {
+    double* dd = (double*)malloc(sizeof(double) * n);
+    double* u = (double*)calloc(n, sizeof(double));
+    double* z = (double*)malloc(sizeof(double) * n);
+    double* y = (double*)malloc(sizeof(double) * n);
+    for (int i = 0; i < n; i++) dd[i] = d[i];
+    dd[0] -= 1.0;
+    dd[n - 1] -= h[n - 1] * l[0];
+    u[0] = 1.0;
+    u[n - 1] = h[n - 1];
+    orc_tridiagonal(l, dd, h, y, 1, rhs, n);
+    orc_tridiagonal(l, dd, h, z, 1, u, n);
+    const double s = (y[0] + l[0] * y[n - 1]) / (1.0 + z[0] + l[0] * z[n - 1]);
+    for (int i = 0; i < n; i++) x[(long)i * stride] = y[i] - z[i] * s;
+    free(dd);
+    free(u);
+    free(z);
+    free(y);
+}
+
+int orc_solver_set_realspace(orc_solver* s, const int* bc) {
+    if (s->pl.model != ORC_CONTINUOUS) return -1;
+    const orc_grid* g = s->g;
+    s->real = 1;
+    for (int i = 0; i < 2 * g->dim; i++) s->bc[i] = bc[i];
+    s->nmax = 1;
+    for (int a = 0; a < g->dim; a++)
+        if (g->nx[a] > s->nmax) s->nmax = g->nx[a];
+    free(s->lap);
+    s->lap = (double*)calloc((size_t)s->pl.n_monomers * 3 * 3 * s->nmax, sizeof(double));
+    return 0;
+}
+
+static void update_laplacian_real(orc_solver* s) {
+    for (int m = 0; m < s->pl.n_monomers; m++) {
+        const double a = s->pl.segment_length[m];
+        orc_laplacian(s->g, s->bc, a * a, s->pl.ds, s->lap + (size_t)m * 9 * s->nmax);
+    }
+}
+
+static void solve_line(const orc_solver* s, int axis, const double* t, double* x, int stride, const double* rhs) {
+    const int n = s->g->nx[axis];
+    if (s->bc[2 * axis] == ORC_BC_PERIODIC) orc_tridiagonal_periodic(t, t + n, t + 2 * n, x, stride, rhs, n);
+    else orc_tridiagonal(t, t + n, t + 2 * n, x, stride, rhs, n);
+}
+
+/* neighbour index along an axis: wrap when periodic, else clamp (the coefficient
+ * of a clamped neighbour is zero) -- CpuSolverReal.cpp:205-242 */
+static int nb(int i, int n, int periodic, int dir) {
+    if (periodic) return (i + dir + n) % n;
+    i += dir;
+    return i < 0 ? 0 : (i >= n ? n - 1 : i);
+}
+
+/* CpuSolverReal.cpp:120-188 with the ADI sweeps of :190-451: q_exp = E q_in; an x solve
+ * whose right side holds the full explicit stencil, then one solve per further axis
+ * adding back half of that axis' operator; q_out = E x (mask). */
+static void step_real(orc_solver* s, const double* qin, double* qout, int m) {
+    const orc_grid* g = s->g;
+    const int D = g->dim;
+    const long M = g->M;
+    const double* E = s->E[m];
+    double* qe = s->r1;
+    double* acc = s->r2;
+    for (long i = 0; i < M; i++) qe[i] = E[i] * qin[i];
+    int n[3] = {1, 1, 1};
+    for (int a = 0; a < D; a++) n[a] = g->nx[a];
+    long st[3];
+    st[D - 1] = 1;
+    for (int a = D - 2; a >= 0; a--) st[a] = st[a + 1] * n[a + 1];
+    const double* tab[3];
+    for (int a = 0; a < D; a++) tab[a] = s->lap + (size_t)m * 9 * s->nmax + (size_t)3 * (a == 0 ? 0 : (a == 1 ? n[0] : n[0] + n[1]));
+    double* rhs = (double*)malloc(sizeof(double) * s->nmax);
+    /* first sweep along axis 0, one line per (other coordinates) */
+    const long lines0 = M / n[0];
+    for (long L = 0; L < lines0; L++) {
+        int c[3] = {0, 0, 0};
+        long r = L;
+        for (int a = D - 1; a >= 1; a--) {
+            c[a] = (int)(r % n[a]);
+            r /= n[a];
+        }
+        long base = 0;
+        for (int a = 1; a < D; a++) base += c[a] * st[a];
+        for (int i = 0; i < n[0]; i++) {
+            c[0] = i;
+            const long p = base + i * st[0];
+            double diag = (double)D - 0.5 * tab[0][n[0] + i];
+            double off = 0.0;
+            for (int a = 1; a < D; a++) {
+                const double* t = tab[a];
+                const int per = s->bc[2 * a] == ORC_BC_PERIODIC;
+                diag -= t[n[a] + c[a]];
+                off += t[c[a]] * qe[p + (nb(c[a], n[a], per, -1) - c[a]) * st[a]] +
+                       t[2 * n[a] + c[a]] * qe[p + (nb(c[a], n[a], per, 1) - c[a]) * st[a]];
+            }
+            const int per0 = s->bc[0] == ORC_BC_PERIODIC;
+            rhs[i] = 2.0 * (diag * qe[p] - off) - tab[0][i] * qe[base + nb(i, n[0], per0, -1) * st[0]] -
+                     tab[0][2 * n[0] + i] * qe[base + nb(i, n[0], per0, 1) * st[0]];
+        }
+        solve_line(s, 0, tab[0], (D == 1 ? qout : acc) + base, (int)st[0], rhs);
+    }
+    /* further sweeps: rhs = previous + (d - 1) q_exp + l q_exp(-) + h q_exp(+) */
+    for (int a = 1; a < D; a++) {
+        double* dst = a == D - 1 ? qout : acc;
+        const long lines = M / n[a];
+        const int per = s->bc[2 * a] == ORC_BC_PERIODIC;
+        const double* t = tab[a];
+        for (long L = 0; L < lines; L++) {
+            int c[3] = {0, 0, 0};
+            long r = L;
+            for (int b = D - 1; b >= 0; b--) {
+                if (b == a) continue;
+                c[b] = (int)(r % n[b]);
+                r /= n[b];
+            }
+            long base = 0;
+            for (int b = 0; b < D; b++)
+                if (b != a) base += c[b] * st[b];
+            for (int i = 0; i < n[a]; i++) {
+                const long p = base + i * st[a];
+                rhs[i] = acc[p] + (t[n[a] + i] - 1.0) * qe[p] + t[i] * qe[base + nb(i, n[a], per, -1) * st[a]] +
+                         t[2 * n[a] + i] * qe[base + nb(i, n[a], per, 1) * st[a]];
+            }
+            solve_line(s, a, t, dst + base, (int)st[a], rhs);
+        }
+    }
+    free(rhs);
+    for (long i = 0; i < M; i++) qout[i] *= E[i];
+    if (g->mask)
+        for (long i = 0; i < M; i++) qout[i] *= g->mask[i];
+}
+
 int orc_compute_propagators(orc_solver* s, const double* const* w, const double* const* q_init) {
     orc_grid* g = s->g;
     const orc_plan* pl = &s->pl;
     const long M = g->M;
     const double ds = pl->ds;
+    if (s->real) update_laplacian_real(s);
     /* CpuSolverPseudo.cpp:89-119 */
     for (int m = 0; m < pl->n_monomers; m++) {
         if (!w[m]) continue;
@@ -504,7 +698,10 @@ int orc_compute_propagators(orc_solver* s, const double* const* w, const double*
             }
             if (g->mask)
                 for (long i = 0; i < M; i++) q0[i] *= g->mask[i];
-            for (int n = 0; n < maxn; n++) step_continuous(s, Q_(s, p, n), Q_(s, p, n + 1), m);
+            for (int n = 0; n < maxn; n++) {
+                if (s->real) step_real(s, Q_(s, p, n), Q_(s, p, n + 1), m);
+                else step_continuous(s, Q_(s, p, n), Q_(s, p, n + 1), m);
+            }
         } else {
             /* CpuComputationDiscrete.cpp:260-541 */
             const double* E = s->E[m];

@@ -93,6 +93,18 @@ double orc_dq_dl(const orc_solver* s, int p, int d);
 /* q[prop][n]; half = 1 selects the half-bond array (discrete) */
 const double* orc_propagator(const orc_solver* s, int prop, int n, int half);
 
+/* Real-space Crank-Nicolson ADI (continuous chains): boundary condition per side,
+ * reference axis order (x low, x high, y low, ...). */
+enum { ORC_BC_PERIODIC = 0, ORC_BC_REFLECTING = 1, ORC_BC_ABSORBING = 2 };
+/* switch the solver to real-space steps (CpuComputationContinuous.cpp:25-26); -1 for discrete chains */
+int orc_solver_set_realspace(orc_solver* s, const int* bc);
+/* FiniteDifference::get_laplacian_matrix: tables = dim blocks of {l, d, h}, nx[axis] each */
+void orc_laplacian(const orc_grid* g, const int* bc, double bond_length_sq, double ds, double* tables);
+/* CpuSolverReal::tridiagonal / tridiagonal_periodic: solve for x (written with stride) */
+void orc_tridiagonal(const double* l, const double* d, const double* h, double* x, int stride, const double* rhs,
+                     int n);
+void orc_tridiagonal_periodic(const double* l, const double* d, const double* h, double* x, int stride,
+                              const double* rhs, int n);
 /* SimpsonRule.h:10-59 coefficients for N intervals (N+1 values) */
 void orc_simpson(int N, double* out);
 

@@ -65,6 +65,11 @@ def lib():
         L.orc_propagator.restype = dp
         L.orc_propagator.argtypes = [vp, i, i, i]
         L.orc_simpson.argtypes = [i, dp]
+        L.orc_solver_set_realspace.restype = i
+        L.orc_solver_set_realspace.argtypes = [vp, ctypes.POINTER(i)]
+        L.orc_laplacian.argtypes = [vp, ctypes.POINTER(i), d, d, dp]
+        L.orc_tridiagonal.argtypes = [dp, dp, dp, dp, i, dp, i]
+        L.orc_tridiagonal_periodic.argtypes = [dp, dp, dp, dp, i, dp, i]
         L.orc_am_create.restype = vp
         L.orc_am_create.argtypes = [i, i, d, d, d]
         L.orc_am_destroy.argtypes = [vp]
@@ -100,6 +105,36 @@ def simpson(n):
     out = np.empty(n + 1)
     lib().orc_simpson(n, _dp(out))
     return out
+
+
+BC_CODES = {"periodic": 0, "reflecting": 1, "absorbing": 2}
+
+
+def _bc_array(bc, dim):
+    bc = list(bc) if bc else ["periodic"] * (2 * dim)
+    if len(bc) != 2 * dim:
+        raise ValueError("expected 0 or %d boundary conditions" % (2 * dim))
+    return (ctypes.c_int * len(bc))(*[BC_CODES[b.lower()] for b in bc])
+
+
+def laplacian(grid, bc, bond_length_sq, ds):
+    """FiniteDifference::get_laplacian_matrix: per axis a dict of l, d, h."""
+    t = np.zeros(3 * sum(grid.nx))
+    lib().orc_laplacian(grid.h, _bc_array(bc, grid.dim), bond_length_sq, ds, _dp(t))
+    out, o = [], 0
+    for n in grid.nx:
+        out.append({"l": t[o:o + n].copy(), "d": t[o + n:o + 2 * n].copy(), "h": t[o + 2 * n:o + 3 * n].copy()})
+        o += 3 * n
+    return out
+
+
+def tridiagonal(l, d, h, rhs, periodic=False):
+    """CpuSolverReal::tridiagonal (periodic: tridiagonal_periodic)."""
+    l, d, h, rhs = (np.ascontiguousarray(v, dtype=np.float64) for v in (l, d, h, rhs))
+    x = np.zeros(len(rhs))
+    f = lib().orc_tridiagonal_periodic if periodic else lib().orc_tridiagonal
+    f(_dp(l), _dp(d), _dp(h), _dp(x), 1, _dp(rhs), len(rhs))
+    return x
 
 
 class Grid:
@@ -164,7 +199,9 @@ class Grid:
 class Solver:
     """Propagators, Q, concentrations and stress for a plan on a Grid."""
 
-    def __init__(self, grid, plan, ds, segment_lengths, solvents=()):
+    def __init__(self, grid, plan, ds, segment_lengths, solvents=(), realspace_bc=None):
+        """realspace_bc: None for pseudo-spectral steps, else the boundary conditions
+        (2 per axis) of real-space Crank-Nicolson steps (continuous chains)."""
         self.grid = grid
         self.plan = plan
         self.ds = ds
@@ -249,6 +286,9 @@ class Solver:
         self.solvents = list(solvents)
         self._st = st
         self.h = lib().orc_solver_create(grid.h, ctypes.byref(st))
+        if realspace_bc is not None:
+            if lib().orc_solver_set_realspace(self.h, _bc_array(realspace_bc, grid.dim)) != 0:
+                raise RuntimeError("Real-space method only support 'continuous' chain model.")
 
     def __del__(self):
         if getattr(self, "h", None):

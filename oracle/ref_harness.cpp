@@ -3,7 +3,8 @@
 // This file is test infrastructure. It is linked (by oracle/Makefile.ref) with
 // the reference's unmodified sources read in place from the reference tree:
 //   src/common/{Polymer,Molecules,PropagatorCode,PropagatorAnalyzer,Scheduler,
-//               Pseudo,ComputationBox,AndersonMixing,CircularBuffer}.cpp
+//               Pseudo,ComputationBox,AndersonMixing,CircularBuffer,
+//               FiniteDifference}.cpp
 //   src/platforms/cpu/CpuAndersonMixing.cpp
 // and writes JSON dumps that the tests compare the product against:
 //   mode plan    : Polymer keys, PropagatorAnalyzer propagators/blocks,
@@ -12,6 +13,8 @@
 //   mode simpson : SimpsonRule::get_coeff
 //   mode am      : CpuAndersonMixing trajectory on a fixed synthetic map
 //   mode deps    : PropagatorCode::get_deps_from_key on given keys
+//   mode laplacian : FiniteDifference::get_laplacian_matrix tridiagonal tables
+//                    per axis for given boundary conditions
 // Nothing here is shipped or measured. Input is a line-based spec on stdin.
 #include <cmath>
 #include <cstdio>
@@ -24,6 +27,7 @@
 #include "AndersonMixing.h"
 #include "ComputationBox.h"
 #include "CpuAndersonMixing.h"
+#include "FiniteDifference.h"
 #include "Molecules.h"
 #include "Polymer.h"
 #include "PropagatorAnalyzer.h"
@@ -35,8 +39,8 @@
 // ComputationBox has pure-virtual device methods; a host-only concrete box.
 class HostBox : public ComputationBox {
 public:
-    HostBox(std::vector<int> nx, std::vector<double> lx)
-        : ComputationBox(nx, lx, {}, nullptr) {}
+    HostBox(std::vector<int> nx, std::vector<double> lx, std::vector<std::string> bc = {})
+        : ComputationBox(nx, lx, bc, nullptr) {}
     double integral_device(const double*) override { return 0; }
     double inner_product_device(const double*, const double*) override { return 0; }
     double inner_product_inverse_weight_device(const double*, const double*, const double*) override { return 0; }
@@ -82,6 +86,7 @@ struct Spec {
     std::vector<double> lx;
     std::vector<int> simpson;
     std::vector<std::string> keys;
+    std::vector<std::string> bc;
     // am
     int n_var = 64, max_hist = 20, n_iter = 40;
     double start_error = 1e-1, mix_min = 0.1, mix_init = 0.1;
@@ -111,6 +116,7 @@ static Spec read_spec() {
         else if (tok == "lx") { double l; while (is >> l) s.lx.push_back(l); }
         else if (tok == "simpson") { int n; while (is >> n) s.simpson.push_back(n); }
         else if (tok == "key") { std::string k; is >> k; s.keys.push_back(k); }
+        else if (tok == "bc") { std::string b; while (is >> b) s.bc.push_back(b); }
         else if (tok == "am") is >> s.n_var >> s.max_hist >> s.start_error >> s.mix_min >> s.mix_init >> s.n_iter;
     }
     return s;
@@ -231,6 +237,38 @@ static void dump_pseudo(const Spec& s) {
     std::printf("}\n");
 }
 
+// one tridiagonal table per axis and bond: lower, diagonal, upper coefficients
+static void dump_laplacian(const Spec& s) {
+    HostBox cb(s.nx, s.lx, s.bc);
+    const int dim = (int)s.nx.size();
+    int nmax = 1;
+    for (int n : s.nx) nmax = n > nmax ? n : nmax;
+    std::vector<double> t(9 * (size_t)nmax, 0.0);
+    std::printf("{\"monomers\":{");
+    bool first = true;
+    for (const auto& b : s.bonds) {
+        double* p = t.data();
+        FiniteDifference::get_laplacian_matrix(cb.get_boundary_conditions(), s.nx, cb.get_dx(), p, p + nmax,
+                                               p + 2 * nmax, p + 3 * nmax, p + 4 * nmax, p + 5 * nmax,
+                                               p + 6 * nmax, p + 7 * nmax, p + 8 * nmax, b.second * b.second, s.ds);
+        if (!first) std::printf(",");
+        first = false;
+        std::printf("\"%s\":[", b.first.c_str());
+        for (int d = 0; d < dim; d++) {
+            if (d) std::printf(",");
+            std::printf("{\"l\":");
+            put_arr(p + (3 * d) * nmax, s.nx[d]);
+            std::printf(",\"d\":");
+            put_arr(p + (3 * d + 1) * nmax, s.nx[d]);
+            std::printf(",\"h\":");
+            put_arr(p + (3 * d + 2) * nmax, s.nx[d]);
+            std::printf("}");
+        }
+        std::printf("]");
+    }
+    std::printf("}}\n");
+}
+
 static void dump_simpson(const Spec& s) {
     std::printf("{");
     for (size_t i = 0; i < s.simpson.size(); i++) {
@@ -300,6 +338,7 @@ int main() {
         else if (s.mode == "simpson") dump_simpson(s);
         else if (s.mode == "deps") dump_deps(s);
         else if (s.mode == "am") dump_am(s);
+        else if (s.mode == "laplacian") dump_laplacian(s);
         else { std::fprintf(stderr, "unknown mode '%s'\n", s.mode.c_str()); return 2; }
     } catch (std::exception& e) {
         std::printf("{\"error\":");

@@ -166,6 +166,65 @@ def kat_fixtures():
     return kats
 
 
+def _ints(src, names):
+    return [int(re.search(r"const int %s\{(\d+)\}" % n, src).group(1)) for n in names]
+
+
+def _bc_lists(src):
+    """`std::vector<std::string> bc_xxx = {...};` boundary-condition tables."""
+    out = {}
+    for m in re.finditer(r"std::vector<std::string>\s+(bc_\w+)\s*=\s*\{(.*?)\};", _strip_comments(src), re.S):
+        out[m.group(1)] = re.findall(r"\"(\w+)\"", m.group(2))
+    return out
+
+
+def realspace_fixtures():
+    """Known answers of the reference's real-space tests, the inputs of its
+    Crank-Nicolson boundary-condition tests, and FiniteDifference tables."""
+    out = {}
+    src = _read("TestCpuTridiagonal.cpp")
+    a = arrays(src)
+    out["tridiagonal"] = {"source": "tests/TestCpuTridiagonal.cpp (:10-38)", "l": a["a"], "d": a["b"], "h": a["c"],
+                          "rhs": a["y"], "x": a["x_answer"], "x_periodic": a["x_periodic_answer"], "tol_abs": 1e-7}
+    src = _read("TestRealLinearContinuous3D.cpp")
+    a = arrays(src)
+    blk = block_tables(src, {"f": 0.5})
+    out["linear_continuous_3d"] = {
+        "source": "tests/TestRealLinearContinuous3D.cpp (q :228-244, Q :247-248, phi :254-268)",
+        "nx": _ints(src, ["II", "JJ", "KK"]), "lx": [4.0, 3.0, 2.0], "bc": [], "ds": 0.25,
+        "bonds": {"A": 1.0, "B": 1.0}, "polymers": [[1.0, blk["blocks"], {}]],
+        "w": {"A": a["w_a"], "B": a["w_b"]}, "Q": 0.621996847395786,
+        "chain_propagators": [
+            {"p": 0, "v": 1, "u": 2, "n": "block", "ref": a["q1_last_ref"]},
+            {"p": 0, "v": 1, "u": 0, "n": "block", "ref": a["q2_last_ref"]}],
+        "phi": {"A": a["phi_a_ref"], "B": a["phi_b_ref"]}, "tol_abs": 1e-7}
+    cases = []
+    for fname, names, lx in [("TestCrankNicolson1D.cpp", ["II"], [4.0]),
+                             ("TestCrankNicolson2D.cpp", ["II", "JJ"], None),
+                             ("TestCrankNicolson3D.cpp", ["II", "JJ", "KK"], [4.0, 3.0, 2.0])]:
+        src = _read(fname)
+        a = arrays(src)
+        blk = block_tables(src, {"f": 0.5})
+        if lx is None:
+            lx = [float(re.search(r"L%s\s*=\s*([0-9.]+);" % c, src).group(1)) for c in "xy"]
+        for name, bc in _bc_lists(src).items():
+            cases.append({"source": f"tests/{fname} ({name})", "nx": _ints(src, names), "lx": lx, "bc": bc,
+                          "ds": 0.25, "bonds": {"A": 1.0, "B": 1.0}, "polymers": [[1.0, blk["blocks"], {}]],
+                          "w": {"A": a["w_a"], "B": a["w_b"]}})
+    out["crank_nicolson_inputs"] = cases
+    lap = []
+    for nx, lx, bc in [([5, 4, 3], [4.0, 3.0, 2.0], ["reflecting", "absorbing", "periodic", "periodic",
+                                                     "absorbing", "reflecting"]),
+                       ([7, 6], [3.0, 2.5], ["absorbing", "absorbing", "reflecting", "reflecting"]),
+                       ([9], [2.0], ["reflecting", "reflecting"]),
+                       ([6, 5, 4], [1.5, 2.5, 3.5], [])]:
+        spec = "mode laplacian\nds 0.15\nbond A 1.0\nbond B 1.5\n" + "nx " + " ".join(map(str, nx)) + \
+            "\nlx " + " ".join(map(repr, lx)) + "\n" + ("bc " + " ".join(bc) + "\n" if bc else "")
+        lap.append({"nx": nx, "lx": lx, "bc": bc, "ds": 0.15, **run_dump(spec)})
+    out["laplacian"] = lap
+    return out
+
+
 def molecule_set():
     """Molecules whose analyzer/scheduler output is pinned: every test molecule
     with a block table plus the benchmark molecules."""
@@ -293,6 +352,8 @@ def main():
                     **run_dump(f"mode am\nam {n_var} {hist} {start!r} {mmin!r} {minit!r} {n_iter}\n")}
     with open(os.path.join(HERE, "am.json"), "w") as f:
         json.dump(am, f)
+    with open(os.path.join(HERE, "realspace.json"), "w") as f:
+        json.dump(realspace_fixtures(), f)
     drivers = driver_records()
     with open(os.path.join(HERE, "drivers.json"), "w") as f:
         json.dump(drivers, f, indent=1)
