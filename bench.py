@@ -1,6 +1,6 @@
 """Langevin FTS throughput on one MI355X, with the propagator kernels' roofline.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload lfts64|lfts32]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload lfts64|lfts32|replicas48|realspace3d]
 
 Workload "lfts64" (default): AB diblock melt, discrete chain N=90, f=0.5, chiN=17,
 64^3 periodic grid, L=6.0 R0, nbar=1e4, Langevin step dt=0.8 with PCG64 noise,
@@ -58,6 +58,10 @@ WORKLOADS = {
     # batching form: 32 independent replicas advanced together
     "replicas48": dict(nx=[48, 48, 48], lx=[4.0, 4.0, 4.0], model="discrete", ds=1 / 64, f=0.5, chi_n=17.0,
                        nbar=1e4, dt=0.8, tol=1e-4, am=(20, 5e-1, 0.01, 0.01), max_iter=200, replicas=32),
+    # real-space Crank-Nicolson solver: the reference's 3D nanoparticle example (devel/RealSpace3D.py)
+    "realspace3d": dict(nx=[120, 100, 140], lx=[4.0, 5.0, 6.0], model="continuous", ds=0.01, radius=1.0,
+                        bc=["absorbing", "absorbing", "reflecting", "reflecting", "reflecting", "reflecting"],
+                        realspace=True),
 }
 
 
@@ -306,6 +310,119 @@ def replica_bench(args, w, lf, lfts):
     print(json.dumps(out), flush=True)
 
 
+def realspace_setup(w):
+    """mask (a sphere of radius w["radius"] at the box centre is impenetrable) and
+    the grafting field: planes at index 30 of each axis, 1/dx there (devel/RealSpace3D.py:32-78)"""
+    nx, lx = w["nx"], w["lx"]
+    axes = [np.linspace(-l / 2, l / 2, num=n, endpoint=False) for n, l in zip(nx, lx)]
+    xv, yv, zv = np.meshgrid(*axes, indexing="ij")
+    mask = np.ones(nx)
+    mask[np.sqrt(xv ** 2 + yv ** 2 + zv ** 2) < w["radius"]] = 0.0
+    q_init = np.zeros(nx)
+    q_init[30, :, :] = 1.0 / (lx[0] / nx[0])
+    q_init[:, 30, :] = 1.0 / (lx[1] / nx[1])
+    q_init[:, :, 30] = 1.0 / (lx[2] / nx[2])
+    return mask.ravel(), q_init.ravel()
+
+
+def realspace_cpu_baseline(w, budget_s, steps_per_evaluation):
+    """The FP64 oracle's real-space steps on the same grid, walls and mask; a
+    5-segment chain times the per-step cost, scaled to one evaluation."""
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from oracle import oracle as O
+    from langevin_fts_amd import langevinfts as lf
+
+    factory = lf.PlatformSelector.create_factory("hip", False)
+    mol = factory.create_molecules_information("continuous", w["ds"], {"A": 1.0})
+    mol.add_polymer(1.0, [["A", 5 * w["ds"], 0, 1]], {0: "G"})
+    an = factory.create_propagator_analyzer(mol, False)
+    plan = plan_dict(mol, an, "continuous")
+    mask, q_init = realspace_setup(w)
+    M = int(np.prod(w["nx"]))
+    solver = O.Solver(O.Grid(w["nx"], w["lx"], mask), plan, w["ds"], {"A": 1.0}, realspace_bc=w["bc"])
+    n, t0 = 0, time.perf_counter()
+    while True:
+        solver.compute_propagators({"A": np.zeros(M)}, {"G": q_init})
+        n += 1
+        if time.perf_counter() - t0 > budget_s:
+            break
+    t_step = (time.perf_counter() - t0) / (n * 10)
+    return {"value": 1.0 / (t_step * steps_per_evaluation), "unit": "evaluations/s", "cores": 1, "kind": "port",
+            "sample": f"{10 * n} real-space steps of the same grid on the FP64 C oracle ({1e3 * t_step:.0f} ms each, "
+                      f"one thread) x {steps_per_evaluation} steps per evaluation"}
+
+
+def realspace_bench(args, w, lf):
+    """Real-space solver: one step = compute_propagators + compute_concentrations of
+    the grafted homopolymer (N = 100, two propagators of 100 Crank-Nicolson steps)
+    with w = 0, fields resident in device memory."""
+    from langevin_fts_amd import capi
+
+    M = int(np.prod(w["nx"]))
+    mask, q_init = realspace_setup(w)
+    with stdout_to_stderr():
+        factory = lf.PlatformSelector.create_factory("hip", False)
+        cb = factory.create_computation_box(w["nx"], w["lx"], bc=w["bc"], mask=mask)
+        mol = factory.create_molecules_information("continuous", w["ds"], {"A": 1.0})
+        mol.add_polymer(1.0, [["A", 1.0, 0, 1]], {0: "G"})
+        an = factory.create_propagator_analyzer(mol, False)
+        solver = factory.create_realspace_solver(cb, mol, an)
+        d_w = capi.DeviceArray.from_host(np.zeros(M))
+        d_q = capi.DeviceArray.from_host(q_init)
+        d_phi = capi.DeviceArray(8 * M)
+
+        def step():
+            solver.compute_propagators_device({"A": d_w.ptr}, {"G": d_q.ptr})
+            solver.compute_concentrations()
+            solver.get_total_concentration_device("A", d_phi.ptr)
+
+        for _ in range(args.warmup):
+            step()
+        solver.get_total_partition(0)   # synchronises the stream
+        if not args.no_kernel_timing:
+            lf.set_kernel_timing(SAMPLE)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            step()
+        Q = solver.get_total_partition(0)
+        elapsed = time.perf_counter() - t0
+        kern = lf.get_kernel_timing("real") if not args.no_kernel_timing else None
+        lf.set_kernel_timing(0)
+        # the reference's own calling pattern: host fields in, phi out each evaluation
+        n_host = max(2, args.steps // 5)
+        t0 = time.perf_counter()
+        for _ in range(n_host):
+            solver.compute_propagators({"A": np.zeros(M)}, q_init={"G": q_init})
+            solver.compute_concentrations()
+            solver.get_total_concentration("A")
+        host_s = (time.perf_counter() - t0) / n_host
+    steps_per_eval = sum(e.max_n_segment for e in an.get_computation_propagators().values())
+    out = {
+        "metric": "real-space propagator evaluations/s @120x100x140 (Crank-Nicolson, walls, nanoparticle mask)",
+        "value": args.steps / elapsed, "unit": "evaluations/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * elapsed / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic: w = 0, grafting planes and a spherical mask as devel/RealSpace3D.py",
+        "config": {"workload": f"grafted homopolymer, continuous N=100 (ds=0.01), {w['nx'][0]}x{w['nx'][1]}x"
+                               f"{w['nx'][2]} grid, L={w['lx']}, walls {w['bc']}, nanoparticle radius {w['radius']}",
+                   "crank_nicolson_steps_per_evaluation": steps_per_eval,
+                   "Q": Q, "host_fields_evaluations_per_s": 1.0 / host_s},
+    }
+    if kern and kern["launches"]:
+        achieved = kern["bytes"] / (kern["milliseconds"] * 1e-3) / 1e9
+        out["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBPS, "unit": "GB/s",
+                           "frac": achieved / PEAK_HBM_GBPS, "traffic": None,
+                           "kernel": "one Crank-Nicolson step (real_pre_kernel + 3 real_sweep_kernel launches)",
+                           "timed_launch_count": kern["launches"],
+                           "avg_launch_us": 1e3 * kern["milliseconds"] / kern["launches"],
+                           "bytes_per_launch": kern["bytes"] / kern["launches"], "timed_launches": f"1 in {SAMPLE}"}
+        out["config"]["step_kernel_share"] = SAMPLE * kern["milliseconds"] * 1e-3 / elapsed
+    if args.cpu_seconds > 0:
+        with stdout_to_stderr():
+            out["cpu_baseline"] = realspace_cpu_baseline(w, args.cpu_seconds, steps_per_eval)
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -344,6 +461,9 @@ def main():
     M = int(np.prod(w["nx"]))
     if w.get("replicas", 1) > 1:
         replica_bench(args, w, lf, lfts)
+        return
+    if w.get("realspace"):
+        realspace_bench(args, w, lf)
         return
     params = lfts_params(w, "/tmp/lfts_bench")
     params["learned_guess"] = {"train_steps": args.guess_train_steps, "lr": 1e-3, "train_iters": 4, "replay": 64,

@@ -30,8 +30,9 @@ enum { FTS_FP64 = 0, FTS_FP32 = 1 };
 enum { FTS_LEAF = 0, FTS_LEAF_QINIT = 1, FTS_JUNCTION = 2, FTS_AGGREGATED = 3 };
 /* sources of fts_solver_sum_concentrations */
 enum { FTS_SRC_BLOCK = 0, FTS_SRC_SOLVENT = 1 };
-/* kernel families of fts_context_get_kernel_timing */
-enum { FTS_KERNEL_SLAB = 0, FTS_KERNEL_PENCIL = 1, FTS_KERNEL_FAMILIES = 2 };
+/* kernel families of fts_context_get_kernel_timing (FTS_KERNEL_REAL: the launches of
+ * one real-space Crank-Nicolson step, timed together) */
+enum { FTS_KERNEL_SLAB = 0, FTS_KERNEL_PENCIL = 1, FTS_KERNEL_REAL = 2, FTS_KERNEL_FAMILIES = 3 };
 typedef struct fts_kernel_stats {
     double milliseconds;   /* summed launch durations (HIP events around each launch) */
     double bytes;          /* summed algorithmic bytes: each field, spectrum and w read or written once */

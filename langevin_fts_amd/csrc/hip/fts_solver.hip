@@ -358,21 +358,79 @@ void Solver<T>::build_real_tables() {
     FTS_HIP_CHECK(hipMemcpy(rtab_.get(), tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice));
 }
 
+// Bytes one step must move per item, each launch reading and writing its operands
+// once: pre, on a run's first step (q_in, w -> q_e), first sweep (q_e -> acc),
+// further sweeps (acc, q_e -> acc), last sweep (acc, q_e, w, mask -> q_out, and
+// q_e of the next step unless this is the run's last).
+static double real_step_bytes(const RealGeom& g, const RealItem& it) {
+    const double e = sizeof(double) * (double)g.M;
+    const double masked = it.mask ? 1.0 : 0.0;
+    const double pre = it.need_pre ? 3 * e : 0.0;
+    const double out = (it.write_qe ? 1.0 : 0.0) * e;
+    if (g.D == 1) return pre + (3 + masked) * e + out;
+    return pre + 2 * e + 3 * e * (g.D - 2) + (4 + masked) * e + out;
+}
+
 template <typename T>
 void Solver<T>::launch_real(const RealBatch& b) {
-    const int pb = std::min<long long>(1024, ceil_div((int)std::min<long long>(M_, 1 << 30), 256));
-    hipLaunchKernelGGL(real_pre_kernel, dim3(pb, b.n), dim3(256), 0, g_->stream(), b, rg_);
-    FTS_HIP_CHECK(hipGetLastError());
+    KernelTimer& timer = g_->ctx->timer;
+    hipEvent_t start = timer.begin(g_->stream(), FTS_KERNEL_REAL);
+    bool any_pre = false;
+    for (int k = 0; k < b.n; k++) any_pre = any_pre || b.it[k].need_pre;
+    if (any_pre) {
+        const int pb = std::min(1024, ceil_div(M_, 256));
+        hipLaunchKernelGGL(real_pre_kernel, dim3(pb, b.n), dim3(256), 0, g_->stream(), b, rg_);
+        FTS_HIP_CHECK(hipGetLastError());
+    }
     const double* tab = rtab_.as<double>();
     for (int a = 0; a < rg_.D; a++) {
+        const RealAxis& A = rg_.ax[a];
+        const bool first = a == 0, last = a == rg_.D - 1;
+        if (A.n <= 64 * RT_LMAX) {
+            RealTile t{};
+            t.P = 1;
+            while (t.P * RT_LMAX < A.n) t.P <<= 1;
+            t.Lc = ceil_div(A.n, t.P);
+            t.R = RT_THREADS / t.P;
+            const bool inner = A.iax == 2;
+            int blocks;
+            if (inner) {
+                blocks = ceil_div(M_ / A.n, t.R);
+            } else {
+                t.ktiles = ceil_div(rg_.tn[2], t.R);
+                blocks = rg_.tn[A.iax == 0 ? 1 : 0] * t.ktiles;
+            }
+            const dim3 grid(blocks, b.n);
+            const size_t lds = sizeof(double) * (size_t)(t.R + 4) * A.n;
+#define FTS_REAL_TILED(F, L, I)                                                                                \
+    hipLaunchKernelGGL((real_tiled_kernel<F, L, I>), grid, dim3(RT_THREADS), lds, g_->stream(), b, rg_, tab, a, t)
+            if (inner) {
+                if (first && last) FTS_REAL_TILED(true, true, true);
+                else if (first) FTS_REAL_TILED(true, false, true);
+                else if (last) FTS_REAL_TILED(false, true, true);
+                else FTS_REAL_TILED(false, false, true);
+            } else {
+                if (first && last) FTS_REAL_TILED(true, true, false);
+                else if (first) FTS_REAL_TILED(true, false, false);
+                else if (last) FTS_REAL_TILED(false, true, false);
+                else FTS_REAL_TILED(false, false, false);
+            }
+#undef FTS_REAL_TILED
+            FTS_HIP_CHECK(hipGetLastError());
+            continue;
+        }
         const int lines = M_ / rg_.ax[a].n;
         const dim3 grid(ceil_div(lines, 256), b.n);
-        const bool first = a == 0, last = a == rg_.D - 1;
         if (first && last) hipLaunchKernelGGL((real_sweep_kernel<true, true>), grid, dim3(256), 0, g_->stream(), b, rg_, tab, a);
         else if (first) hipLaunchKernelGGL((real_sweep_kernel<true, false>), grid, dim3(256), 0, g_->stream(), b, rg_, tab, a);
         else if (last) hipLaunchKernelGGL((real_sweep_kernel<false, true>), grid, dim3(256), 0, g_->stream(), b, rg_, tab, a);
         else hipLaunchKernelGGL((real_sweep_kernel<false, false>), grid, dim3(256), 0, g_->stream(), b, rg_, tab, a);
         FTS_HIP_CHECK(hipGetLastError());
+    }
+    if (start) {
+        double bytes = 0.0;
+        for (int k = 0; k < b.n; k++) bytes += real_step_bytes(rg_, b.it[k]);
+        timer.end(g_->stream(), start, FTS_KERNEL_REAL, bytes);
     }
 }
 
@@ -517,6 +575,8 @@ void Solver<T>::build_slice(const std::vector<std::array<int, 3>>& jobs) {
                     it.w = w(m);
                     it.mask = mask();
                     it.monomer = m;
+                    it.need_pre = t == start;
+                    it.write_qe = !last;
                     ri.push_back(it);
                 }
             } else if (model_ == FTS_CONTINUOUS) {

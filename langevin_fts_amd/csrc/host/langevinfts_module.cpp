@@ -452,7 +452,8 @@ PYBIND11_MODULE(langevinfts, m) {
         int f;
         if (family == "slab") f = FTS_KERNEL_SLAB;
         else if (family == "pencil") f = FTS_KERNEL_PENCIL;
-        else LFTS_THROW("kernel family must be \"slab\" or \"pencil\"");
+        else if (family == "real") f = FTS_KERNEL_REAL;
+        else LFTS_THROW("kernel family must be \"slab\", \"pencil\" or \"real\"");
         fts_kernel_stats st{};
         check_fts(fts_context_get_kernel_timing(hip_context(), f, &st));
         py::dict d;
