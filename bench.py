@@ -1,6 +1,6 @@
 """Langevin FTS throughput on one MI355X, with the propagator kernels' roofline.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload lfts64|lfts32|replicas48|realspace3d]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload lfts64|lfts32|replicas48|realspace3d|reduce128]
 
 Workload "lfts64" (default): AB diblock melt, discrete chain N=90, f=0.5, chiN=17,
 64^3 periodic grid, L=6.0 R0, nbar=1e4, Langevin step dt=0.8 with PCG64 noise,
@@ -58,6 +58,8 @@ WORKLOADS = {
     # batching form: 32 independent replicas advanced together
     "replicas48": dict(nx=[48, 48, 48], lx=[4.0, 4.0, 4.0], model="discrete", ds=1 / 64, f=0.5, chi_n=17.0,
                        nbar=1e4, dt=0.8, tol=1e-4, am=(20, 5e-1, 0.01, 0.01), max_iter=200, replicas=32),
+    # reduce-memory form: propagators in pinned host memory (the 128^3 discrete N=90 precision-form shape)
+    "reduce128": dict(nx=[128, 128, 128], lx=[6.0, 6.0, 6.0], model="discrete", ds=1 / 90, f=0.4, reduce=True),
     # real-space Crank-Nicolson solver: the reference's 3D nanoparticle example (devel/RealSpace3D.py)
     "realspace3d": dict(nx=[120, 100, 140], lx=[4.0, 5.0, 6.0], model="continuous", ds=0.01, radius=1.0,
                         bc=["absorbing", "absorbing", "reflecting", "reflecting", "reflecting", "reflecting"],
@@ -423,6 +425,74 @@ def realspace_bench(args, w, lf):
     print(json.dumps(out), flush=True)
 
 
+def reduce_bench(args, w, lf):
+    """Reduce-memory form against the device-resident one on the same evaluation:
+    compute_propagators + compute_concentrations + compute_stress of an AB diblock,
+    fields N(0, 2^2) resident in device memory. value = reduce-memory evaluations/s."""
+    from langevin_fts_amd import capi
+
+    M = int(np.prod(w["nx"]))
+    rng = np.random.Generator(np.random.PCG64(args.seed))
+    fields = {"A": rng.normal(0.0, 2.0, M), "B": rng.normal(0.0, 2.0, M)}
+    res = {}
+    with stdout_to_stderr():
+        d_w = {m: capi.DeviceArray.from_host(v) for m, v in fields.items()}
+        for reduce in (False, True):
+            factory = lf.PlatformSelector.create_factory("hip", reduce)
+            cb = factory.create_computation_box(w["nx"], w["lx"])
+            mol = factory.create_molecules_information(w["model"], w["ds"], {"A": 1.0, "B": 1.0})
+            mol.add_polymer(1.0, [["A", w["f"], 0, 1], ["B", 1 - w["f"], 1, 2]])
+            an = factory.create_propagator_analyzer(mol, True)
+            solver = factory.create_pseudospectral_solver(cb, mol, an)
+
+            def step():
+                solver.compute_propagators_device({m: a.ptr for m, a in d_w.items()})
+                solver.compute_concentrations()
+                solver.compute_stress()   # synchronises the stream
+
+            for _ in range(args.warmup):
+                step()
+            if reduce and not args.no_kernel_timing:
+                lf.set_kernel_timing(SAMPLE)
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                step()
+            elapsed = time.perf_counter() - t0
+            kern = None
+            if reduce and not args.no_kernel_timing:
+                kern = lf.get_kernel_timing("slab")
+                lf.set_kernel_timing(0)
+            res[reduce] = {"s": elapsed / args.steps, "mem": solver.get_memory_usage(), "kern": kern,
+                           "Q": solver.get_total_partition(0)}
+            del solver, an, mol, cb, factory
+    r, d = res[True], res[False]
+    host_bytes = r["mem"][1]
+    out = {
+        "metric": "propagator evaluations/s @128³, reduce-memory form (propagators in pinned host memory)",
+        "value": 1.0 / r["s"], "unit": "evaluations/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * r["s"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic: w_A, w_B ~ N(0, 2^2) from PCG64(seed)",
+        "config": {"workload": f"AB diblock, discrete N=90, f={w['f']}, {w['nx'][0]}^3, L={w['lx'][0]}; one step = "
+                               "compute_propagators + compute_concentrations + compute_stress",
+                   "device_form_evaluations_per_s": 1.0 / d["s"], "reduce_vs_device": d["s"] / r["s"],
+                   "device_bytes": {"device_form": d["mem"][0], "reduce_memory": r["mem"][0]},
+                   "pinned_host_bytes": host_bytes,
+                   # each propagator slice is written once and read back by the concentration and stress passes
+                   "host_link_GBps_lower_bound": 2 * host_bytes / r["s"] / 1e9,
+                   "same_Q": r["Q"] == d["Q"]},
+    }
+    kern = r["kern"]
+    if kern and kern["launches"]:
+        achieved = kern["bytes"] / (kern["milliseconds"] * 1e-3) / 1e9
+        out["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBPS, "unit": "GB/s",
+                           "frac": achieved / PEAK_HBM_GBPS, "traffic": None,
+                           "kernel": "slab pass of the reduce-memory form (q slices stored over the host link)",
+                           "timed_launch_count": kern["launches"],
+                           "avg_launch_us": 1e3 * kern["milliseconds"] / kern["launches"],
+                           "bytes_per_launch": kern["bytes"] / kern["launches"], "timed_launches": f"1 in {SAMPLE}"}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -464,6 +534,9 @@ def main():
         return
     if w.get("realspace"):
         realspace_bench(args, w, lf)
+        return
+    if w.get("reduce"):
+        reduce_bench(args, w, lf)
         return
     params = lfts_params(w, "/tmp/lfts_bench")
     params["learned_guess"] = {"train_steps": args.guess_train_steps, "lr": 1e-3, "train_iters": 4, "replay": 64,

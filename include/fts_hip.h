@@ -12,6 +12,8 @@
 #ifndef FTS_HIP_H
 #define FTS_HIP_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -141,6 +143,18 @@ enum { FTS_BC_PERIODIC = 0, FTS_BC_REFLECTING = 1, FTS_BC_ABSORBING = 2 };
  * axis are periodic or neither is. fts_solver_compute_stress fails on such a solver,
  * as CpuSolverReal::compute_single_segment_stress_continuous does. */
 int fts_solver_create_realspace(fts_grid* g, const fts_plan* plan, const int* bc, fts_solver** out);
+/* The reduce-memory forms CudaComputationReduceMemoryContinuous / ...Discrete
+ * (CudaComputationReduceMemoryContinuous.cu:13-120, CudaComputationReduceMemoryDiscrete.cu; selected by
+ * PlatformSelector::create_factory(name, reduce_memory_usage = true), PlatformSelector.cpp:16-54):
+ * every propagator slice lives in pinned host memory, which the kernels write and read in
+ * place over the host link; fields, spectra and concentrations stay in device memory.
+ * Results equal those of fts_solver_create / fts_solver_create_realspace.
+ * bc: NULL for pseudo-spectral steps, else as fts_solver_create_realspace (then FP64). */
+int fts_solver_create_reduce_memory(fts_grid* g, const fts_plan* plan, int precision, const int* bc,
+                                    fts_solver** out);
+/* Bytes of device memory and of pinned host memory a solver holds (no reference
+ * counterpart: lets callers and tests see where the propagators live). */
+int fts_solver_get_memory_usage(fts_solver* s, size_t* device_bytes, size_t* host_bytes);
 /* n_replicas independent copies of the plan in one solver (no reference counterpart:
  * the replica-batched L-FTS form). Replica r has its own monomer fields (index
  * r n_monomers + m in the w array of compute_propagators), blocks (r n_blocks + b),
@@ -174,6 +188,10 @@ int fts_solver_get_propagator(fts_solver* s, int prop, int n, int half, double* 
 
 /* ---- Anderson mixing (AndersonMixing.cpp:3-47, CpuAndersonMixing.cpp:76-161,
  *      CudaAndersonMixing.cu:9-312); history kept on the device, FP64 Gram solve */
+/* CudaAndersonMixingReduceMemory (CudaAndersonMixingReduceMemory.cu): the history rings of
+ * fts_am_create in pinned host memory; same results. */
+int fts_am_create_reduce_memory(fts_context* ctx, int n_var, int max_hist, double start_error, double mix_min,
+                                double mix_init, fts_am** out);
 int fts_am_create(fts_context* ctx, int n_var, int max_hist, double start_error, double mix_min, double mix_init,
                   fts_am** out);
 int fts_am_destroy(fts_am* am);

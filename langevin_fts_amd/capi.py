@@ -66,6 +66,9 @@ def lib():
             "fts_grid_fft_backward": (i, [vp, i, vp, vp, i]),
             "fts_solver_create": (i, [vp, ctypes.POINTER(FtsPlan), i, VPP]),
             "fts_solver_create_realspace": (i, [vp, ctypes.POINTER(FtsPlan), _IP, VPP]),
+            "fts_solver_create_reduce_memory": (i, [vp, ctypes.POINTER(FtsPlan), i, _IP, VPP]),
+            "fts_solver_get_memory_usage": (i, [vp, ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)]),
+            "fts_am_create_reduce_memory": (i, [vp, i, i, d, d, d, VPP]),
             "fts_solver_destroy": (i, [vp]),
             "fts_solver_update_laplacian": (i, [vp]),
             "fts_solver_compute_propagators": (i, [vp, ctypes.POINTER(_DP), ctypes.POINTER(_DP), i]),

@@ -63,7 +63,7 @@ __global__ void am_mix(AmMixArgs args, int n, double* out) {
 }  // namespace
 
 AndersonMixingDevice::AndersonMixingDevice(fts_context* ctx, int n_var, int max_hist, double start_error,
-                                           double mix_min, double mix_init)
+                                           double mix_min, double mix_init, bool reduce_memory)
     : ctx_(ctx),
       n_var_(n_var),
       max_hist_(max_hist),
@@ -75,8 +75,14 @@ AndersonMixingDevice::AndersonMixingDevice(fts_context* ctx, int n_var, int max_
     FTS_REQUIRE(n_var > 0, "n_var must be positive");
     FTS_REQUIRE(max_hist >= 1 && max_hist < AM_MAX_HIST, "max_hist must be in [1, 63]");
     const size_t rows = max_hist + 1;
-    w_hist_.allocate(sizeof(double) * rows * n_var);
-    d_hist_.allocate(sizeof(double) * rows * n_var);
+    // reduce-memory form (CudaAndersonMixingReduceMemory.cu): the history rings live in pinned host memory
+    if (reduce_memory) {
+        w_hist_.allocate_host_pinned(sizeof(double) * rows * n_var);
+        d_hist_.allocate_host_pinned(sizeof(double) * rows * n_var);
+    } else {
+        w_hist_.allocate(sizeof(double) * rows * n_var);
+        d_hist_.allocate(sizeof(double) * rows * n_var);
+    }
     in_w_.allocate(sizeof(double) * n_var);
     in_d_.allocate(sizeof(double) * n_var);
     out_.allocate(sizeof(double) * n_var);
@@ -113,8 +119,9 @@ void AndersonMixingDevice::calculate_new_fields(double* w_new, const double* w, 
     if (n_anderson_ >= 0) {
         n_anderson_ = std::min(max_hist_, n_anderson_);
         const int slot = (ring_start_ + ring_n_) % L;
-        FTS_HIP_CHECK(hipMemcpyAsync(w_hist_.as<double>() + (size_t)slot * n_var_, dw, nb, hipMemcpyDeviceToDevice, st));
-        FTS_HIP_CHECK(hipMemcpyAsync(d_hist_.as<double>() + (size_t)slot * n_var_, dd, nb, hipMemcpyDeviceToDevice, st));
+        // the rings may be pinned host memory: the runtime picks the direction
+        FTS_HIP_CHECK(hipMemcpyAsync(w_hist_.as<double>() + (size_t)slot * n_var_, dw, nb, hipMemcpyDefault, st));
+        FTS_HIP_CHECK(hipMemcpyAsync(d_hist_.as<double>() + (size_t)slot * n_var_, dd, nb, hipMemcpyDefault, st));
         if (ring_n_ == L) ring_start_ = (ring_start_ + 1) % L;
         ring_n_ = std::min(ring_n_ + 1, L);
         // dots[i] = d . d_hist[i], i = 0..n_anderson (no volume weights)

@@ -211,6 +211,32 @@ int fts_solver_create(fts_grid* g, const fts_plan* plan, int precision, fts_solv
     });
 }
 
+int fts_solver_create_reduce_memory(fts_grid* g, const fts_plan* plan, int precision, const int* bc,
+                                    fts_solver** out) {
+    return guarded([&] {
+        need(g, "grid");
+        need(plan, "plan");
+        need(out, "out");
+        auto* s = new fts_solver;
+        try {
+            s->s = fts::make_solver(g->g.get(), plan, bc ? FTS_FP64 : precision, bc, true);
+        } catch (...) {
+            delete s;
+            throw;
+        }
+        *out = s;
+    });
+}
+
+int fts_solver_get_memory_usage(fts_solver* s, size_t* device_bytes, size_t* host_bytes) {
+    return guarded([&] {
+        need(s, "solver");
+        need(device_bytes, "device_bytes");
+        need(host_bytes, "host_bytes");
+        s->s->memory_usage(device_bytes, host_bytes);
+    });
+}
+
 int fts_solver_create_realspace(fts_grid* g, const fts_plan* plan, const int* bc, fts_solver** out) {
     return guarded([&] {
         need(g, "grid");
@@ -424,6 +450,22 @@ int fts_solver_get_propagator(fts_solver* s, int prop, int n, int half, double* 
         need(s, "solver");
         need(out, "out");
         s->s->propagator(prop, n, half, out, on_device != 0);
+    });
+}
+
+int fts_am_create_reduce_memory(fts_context* ctx, int n_var, int max_hist, double start_error, double mix_min,
+                                double mix_init, fts_am** out) {
+    return guarded([&] {
+        need(ctx, "ctx");
+        need(out, "out");
+        auto* a = new fts_am;
+        try {
+            a->a.reset(new fts::AndersonMixingDevice(ctx, n_var, max_hist, start_error, mix_min, mix_init, true));
+        } catch (...) {
+            delete a;
+            throw;
+        }
+        *out = a;
     });
 }
 

@@ -47,17 +47,21 @@ public:
     explicit DeviceBuffer(size_t bytes) { allocate(bytes); }
     DeviceBuffer(const DeviceBuffer&) = delete;
     DeviceBuffer& operator=(const DeviceBuffer&) = delete;
-    DeviceBuffer(DeviceBuffer&& o) noexcept : ptr_(o.ptr_), bytes_(o.bytes_) {
-        o.ptr_ = nullptr;
+    DeviceBuffer(DeviceBuffer&& o) noexcept : ptr_(o.ptr_), host_(o.host_), bytes_(o.bytes_), pinned_(o.pinned_) {
+        o.ptr_ = o.host_ = nullptr;
         o.bytes_ = 0;
+        o.pinned_ = false;
     }
     DeviceBuffer& operator=(DeviceBuffer&& o) noexcept {
         if (this != &o) {
             release();
             ptr_ = o.ptr_;
+            host_ = o.host_;
             bytes_ = o.bytes_;
-            o.ptr_ = nullptr;
+            pinned_ = o.pinned_;
+            o.ptr_ = o.host_ = nullptr;
             o.bytes_ = 0;
+            o.pinned_ = false;
         }
         return *this;
     }
@@ -67,11 +71,28 @@ public:
         if (bytes) FTS_HIP_CHECK(hipMalloc(&ptr_, bytes));
         bytes_ = bytes;
     }
-    void release() {
-        if (ptr_) (void)hipFree(ptr_);
-        ptr_ = nullptr;
-        bytes_ = 0;
+    // Pinned host memory mapped into the device's address space: kernels read and
+    // write it in place over the host link (the reduce-memory storage).
+    void allocate_host_pinned(size_t bytes) {
+        release();
+        if (bytes) {
+            FTS_HIP_CHECK(hipHostMalloc(&host_, bytes, hipHostMallocMapped | hipHostMallocPortable));
+            FTS_HIP_CHECK(hipHostGetDevicePointer(&ptr_, host_, 0));
+            pinned_ = true;
+        }
+        bytes_ = bytes;
     }
+    void release() {
+        if (pinned_) {
+            if (host_) (void)hipHostFree(host_);
+        } else if (ptr_) {
+            (void)hipFree(ptr_);
+        }
+        ptr_ = host_ = nullptr;
+        bytes_ = 0;
+        pinned_ = false;
+    }
+    bool pinned_host() const { return pinned_; }
     template <typename T>
     T* as() const {
         return static_cast<T*>(ptr_);
@@ -80,8 +101,10 @@ public:
     size_t bytes() const { return bytes_; }
 
 private:
-    void* ptr_ = nullptr;
+    void* ptr_ = nullptr;    // device-usable address
+    void* host_ = nullptr;   // host address of a pinned allocation
     size_t bytes_ = 0;
+    bool pinned_ = false;
 };
 
 inline int ceil_div(int a, int b) { return (a + b - 1) / b; }

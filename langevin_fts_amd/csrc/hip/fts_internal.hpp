@@ -167,15 +167,19 @@ public:
     virtual void sum_concentrations(int n, const int* kind, const int* idx, const double* scale, double* out,
                                     bool on_device) = 0;
     virtual void propagator(int prop, int n, int half, double* out, bool on_device) = 0;
+    // bytes of device memory and of pinned host memory the solver holds
+    virtual void memory_usage(size_t* device, size_t* host) const = 0;
 };
 
 // bc: NULL for pseudo-spectral steps, else 2 dim FTS_BC_* codes (real-space steps, FP64 continuous)
-std::unique_ptr<SolverBase> make_solver(Grid* g, const fts_plan* plan, int precision, const int* bc = nullptr);
+// reduce_memory: propagator slices in pinned host memory (CudaComputationReduceMemory*.cu)
+std::unique_ptr<SolverBase> make_solver(Grid* g, const fts_plan* plan, int precision, const int* bc = nullptr,
+                                        bool reduce_memory = false);
 
 class AndersonMixingDevice {
 public:
     AndersonMixingDevice(fts_context* ctx, int n_var, int max_hist, double start_error, double mix_min,
-                         double mix_init);
+                         double mix_init, bool reduce_memory = false);
     void reset();
     void calculate_new_fields(double* w_new, const double* w, const double* d, double old_err, double err,
                               bool on_device);

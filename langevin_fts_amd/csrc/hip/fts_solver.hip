@@ -24,7 +24,7 @@ class Solver final : public SolverBase {
     using T2 = typename vec2<T>::type;
 
 public:
-    Solver(Grid* grid, const fts_plan* pl, const int* bc);
+    Solver(Grid* grid, const fts_plan* pl, const int* bc, bool reduce_memory);
     void update_laplacian() override {
         if (real_) build_real_tables();
         else bt_.build(*g_, bsq_, ds_);
@@ -38,6 +38,14 @@ public:
     void sum_concentrations(int n, const int* kind, const int* idx, const double* scale, double* out,
                             bool on_device) override;
     void propagator(int prop, int n, int half, double* out, bool on_device) override;
+    void memory_usage(size_t* device, size_t* host) const override {
+        size_t d = 0, h = 0;
+        for (const DeviceBuffer* b : {&qstore_, &hstore_, &w64_, &w32_, &qinit_, &ws_, &phib_, &phis_, &Qd_,
+                                      &stress_out_, &rtab_})
+            (b->pinned_host() ? h : d) += b->bytes();
+        *device = d;
+        *host = h;
+    }
 
 private:
     struct Prop {
@@ -118,7 +126,7 @@ private:
 };
 
 template <typename T>
-Solver<T>::Solver(Grid* grid, const fts_plan* pl, const int* bc) : g_(grid) {
+Solver<T>::Solver(Grid* grid, const fts_plan* pl, const int* bc, bool reduce_memory) : g_(grid) {
     model_ = pl->model;
     if (bc) {
         // CpuSolverReal.cpp:16-17
@@ -182,7 +190,10 @@ Solver<T>::Solver(Grid* grid, const fts_plan* pl, const int* bc) : g_(grid) {
         qoff_.push_back(tot);
         tot += (size_t)(pr.max_n + 1) * M_;
     }
-    qstore_.allocate(sizeof(T) * tot);
+    // reduce-memory form (CudaComputationReduceMemoryContinuous.cu:55-70): every propagator slice in pinned
+    // host memory, which the kernels write and read in place over the host link
+    if (reduce_memory) qstore_.allocate_host_pinned(sizeof(T) * tot);
+    else qstore_.allocate(sizeof(T) * tot);
     size_t htot = 0;
     if (model_ == FTS_DISCRETE) {
         for (int p = 0; p < (int)props_.size(); p++) {
@@ -197,7 +208,10 @@ Solver<T>::Solver(Grid* grid, const fts_plan* pl, const int* bc) : g_(grid) {
                 }
         }
     }
-    if (htot) hstore_.allocate(sizeof(T) * htot);
+    if (htot) {
+        if (reduce_memory) hstore_.allocate_host_pinned(sizeof(T) * htot);
+        else hstore_.allocate(sizeof(T) * htot);
+    }
     w64_.allocate(sizeof(double) * n_mon_ * (size_t)M_);
     if (std::is_same<T, double>::value) {
         wT_ = reinterpret_cast<T*>(w64_.as<double>());
@@ -992,12 +1006,13 @@ void Solver<T>::propagator(int prop, int n, int hf, double* out, bool on_device)
 
 }  // namespace
 
-std::unique_ptr<SolverBase> make_solver(Grid* g, const fts_plan* plan, int precision, const int* bc) {
-    if (precision == FTS_FP64) return std::unique_ptr<SolverBase>(new Solver<double>(g, plan, bc));
+std::unique_ptr<SolverBase> make_solver(Grid* g, const fts_plan* plan, int precision, const int* bc,
+                                        bool reduce_memory) {
+    if (precision == FTS_FP64) return std::unique_ptr<SolverBase>(new Solver<double>(g, plan, bc, reduce_memory));
     if (precision == FTS_FP32) {
         FTS_REQUIRE(!bc, "the real-space method runs in FP64");
         FTS_REQUIRE(!g->big, "FP32 propagators need grid sizes whose prime factors are 2, 3 and 5");
-        return std::unique_ptr<SolverBase>(new Solver<float>(g, plan, nullptr));
+        return std::unique_ptr<SolverBase>(new Solver<float>(g, plan, nullptr, reduce_memory));
     }
     throw Error("precision must be FTS_FP64 or FTS_FP32");
 }

@@ -137,13 +137,15 @@ void ComputationBox::zero_mean(double* g) const {
 
 // ------------------------------------------------------------------ PropagatorComputation
 PropagatorComputation::PropagatorComputation(ComputationBox* cb, Molecules* molecules, PropagatorAnalyzer* analyzer,
-                                             int precision, int n_streams, int n_replicas, bool realspace)
+                                             int precision, int n_streams, int n_replicas, bool realspace,
+                                             bool reduce_memory)
     : cb_(cb), molecules_(molecules), analyzer_(analyzer), precision_(realspace ? FTS_FP64 : precision),
       n_streams_(n_streams), n_replicas_(n_replicas), realspace_(realspace) {
     if (n_replicas < 1) LFTS_THROW("n_replicas must be positive");
     if (!cb) LFTS_THROW("ComputationBox *cb is a null pointer");
     if (!molecules) LFTS_THROW("Molecules *molecules is a null pointer");
     if (!analyzer) LFTS_THROW("PropagatorAnalyzer *propagator_analyzer is a null pointer");
+    if (reduce_memory && n_replicas != 1) LFTS_THROW("the replica solver has no reduce-memory form");
     if (realspace) {
         // CpuSolverReal.cpp:16-17
         if (molecules->get_model_name() != "continuous")
@@ -287,13 +289,23 @@ PropagatorComputation::PropagatorComputation(ComputationBox* cb, Molecules* mole
     pl.job_prop = ip(j_prop);
     pl.job_from = ip(j_from);
     pl.job_to = ip(j_to);
-    if (realspace_) {
-        std::vector<int> bc;
+    std::vector<int> bc;
+    if (realspace_)
         for (const auto& b : cb->get_boundary_conditions())
             bc.push_back(b == "periodic" ? FTS_BC_PERIODIC : b == "reflecting" ? FTS_BC_REFLECTING : FTS_BC_ABSORBING);
+    if (reduce_memory) {
+        check_fts(fts_solver_create_reduce_memory(cb->handle(), &pl, precision_, realspace_ ? bc.data() : nullptr,
+                                                  &solver_));
+    } else if (realspace_) {
         check_fts(fts_solver_create_realspace(cb->handle(), &pl, bc.data(), &solver_));
     } else if (n_replicas_ == 1) check_fts(fts_solver_create(cb->handle(), &pl, precision_, &solver_));
     else check_fts(fts_solver_create_replicas(cb->handle(), &pl, precision_, n_replicas_, &solver_));
+}
+
+std::pair<size_t, size_t> PropagatorComputation::memory_usage() const {
+    size_t d = 0, h = 0;
+    check_fts(fts_solver_get_memory_usage(solver_, &d, &h));
+    return {d, h};
 }
 
 PropagatorComputation::~PropagatorComputation() {
@@ -554,9 +566,13 @@ bool PropagatorComputation::check_total_partition() {
 }
 
 // ------------------------------------------------------------------ AndersonMixing
-AndersonMixing::AndersonMixing(int n_var, int max_hist, double start_error, double mix_min, double mix_init)
+AndersonMixing::AndersonMixing(int n_var, int max_hist, double start_error, double mix_min, double mix_init,
+                               bool reduce_memory)
     : n_var_(n_var) {
-    check_fts(fts_am_create(hip_context(), n_var, max_hist, start_error, mix_min, mix_init, &am_));
+    if (reduce_memory)
+        check_fts(fts_am_create_reduce_memory(hip_context(), n_var, max_hist, start_error, mix_min, mix_init, &am_));
+    else
+        check_fts(fts_am_create(hip_context(), n_var, max_hist, start_error, mix_min, mix_init, &am_));
 }
 
 AndersonMixing::~AndersonMixing() {
@@ -765,18 +781,18 @@ PropagatorComputation* AbstractFactory::create_replica_solver(ComputationBox* cb
 
 PropagatorComputation* AbstractFactory::create_pseudospectral_solver(ComputationBox* cb, Molecules* molecules,
                                                                      PropagatorAnalyzer* analyzer) {
-    return new PropagatorComputation(cb, molecules, analyzer, precision_, lanes());
+    return new PropagatorComputation(cb, molecules, analyzer, precision_, lanes(), 1, false, reduce_memory_usage_);
 }
 
 // CpuComputationContinuous(cb, molecules, analyzer, "realspace") (CpuComputationContinuous.cpp:25-26)
 PropagatorComputation* AbstractFactory::create_realspace_solver(ComputationBox* cb, Molecules* molecules,
                                                                 PropagatorAnalyzer* analyzer) {
-    return new PropagatorComputation(cb, molecules, analyzer, FTS_FP64, lanes(), 1, true);
+    return new PropagatorComputation(cb, molecules, analyzer, FTS_FP64, lanes(), 1, true, reduce_memory_usage_);
 }
 
 AndersonMixing* AbstractFactory::create_anderson_mixing(int n_var, int max_hist, double start_error, double mix_min,
                                                         double mix_init) {
-    return new AndersonMixing(n_var, max_hist, start_error, mix_min, mix_init);
+    return new AndersonMixing(n_var, max_hist, start_error, mix_min, mix_init, reduce_memory_usage_);
 }
 
 void AbstractFactory::display_info() const {
@@ -787,7 +803,8 @@ void AbstractFactory::display_info() const {
     std::cout << "Propagators and FFTs in " << (precision_ == FTS_FP64 ? "FP64" : "FP32")
               << "; Q, concentrations, stress and Anderson mixing in FP64" << std::endl;
     std::cout << "Concurrent propagator lanes per schedule slice: " << lanes() << std::endl;
-    if (reduce_memory_usage_) std::cout << "(reduce_memory_usage is ignored: propagators stay in HBM)" << std::endl;
+    if (reduce_memory_usage_)
+        std::cout << "Reduce memory usage: propagators and Anderson-mixing history in pinned host memory" << std::endl;
 }
 
 std::vector<std::string> PlatformSelector::avail_platforms() { return {"hip", "hip-fp32"}; }

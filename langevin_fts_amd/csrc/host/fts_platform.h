@@ -58,9 +58,10 @@ class PropagatorComputation {
 public:
     // n_replicas > 1: that many independent copies of the chain set in one solver (replica L-FTS);
     // the concentration and partition getters then report replica 0.
-    // realspace: Crank-Nicolson steps under the box's boundary conditions (FP64, continuous chains)
+    // realspace: Crank-Nicolson steps under the box's boundary conditions (FP64, continuous chains).
+    // reduce_memory: propagator slices in pinned host memory (CudaComputationReduceMemory*)
     PropagatorComputation(ComputationBox* cb, Molecules* molecules, PropagatorAnalyzer* analyzer, int precision,
-                          int n_streams, int n_replicas = 1, bool realspace = false);
+                          int n_streams, int n_replicas = 1, bool realspace = false, bool reduce_memory = false);
     ~PropagatorComputation();
     PropagatorComputation(const PropagatorComputation&) = delete;
     PropagatorComputation& operator=(const PropagatorComputation&) = delete;
@@ -95,6 +96,8 @@ public:
     void concentration_terms(const std::string& monomer, double scale, std::vector<int>& kinds,
                              std::vector<int>& idx, std::vector<double>& sc) const;
     fts_solver* handle() const { return solver_; }
+    // bytes of device memory and of pinned host memory the solver holds
+    std::pair<size_t, size_t> memory_usage() const;
     int n_replicas() const { return n_replicas_; }
     int n_blocks_per_replica() const { return (int)block_keys_.size(); }
     Molecules* molecules() const { return molecules_; }
@@ -123,7 +126,9 @@ private:
 
 class AndersonMixing {
 public:
-    AndersonMixing(int n_var, int max_hist, double start_error, double mix_min, double mix_init);
+    // reduce_memory: history rings in pinned host memory (CudaAndersonMixingReduceMemory)
+    AndersonMixing(int n_var, int max_hist, double start_error, double mix_min, double mix_init,
+                   bool reduce_memory = false);
     ~AndersonMixing();
     AndersonMixing(const AndersonMixing&) = delete;
     AndersonMixing& operator=(const AndersonMixing&) = delete;

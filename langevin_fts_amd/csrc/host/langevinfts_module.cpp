@@ -160,6 +160,8 @@ PYBIND11_MODULE(langevinfts, m) {
             },
             py::arg("w_input"), py::arg("q_init") = py::none())
         .def("compute_concentrations", &PropagatorComputation::compute_concentrations)
+        // (device bytes, pinned host bytes) held by the solver (an addition: shows the reduce-memory placement)
+        .def("get_memory_usage", &PropagatorComputation::memory_usage)
         .def(
             "compute_statistics",
             [](PropagatorComputation& obj, const std::map<std::string, py::object>& w_input, py::object q_init) {
