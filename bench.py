@@ -13,14 +13,18 @@ steps run untimed. Then, with the same tolerance:
 
   training     --guess-train-steps steps (default 1000) that train the CNN guess on
                the run's own saddle points (timed separately, reported, not in value);
-  plain arm    K timed steps, each saddle point starting from the previous w+
-               (the reference's method);
-  learned arm  K timed steps from the same state whose saddle points start from the
-               CNN's prediction.
+  plain arm    K timed steps, each saddle point starting from the previous w+, all
+               iterations FP64 (the reference's method);
+  learned FP64 K steps whose saddle points start from the CNN's prediction, FP64;
+  plain mixed  K steps from the reference's start with the mixed-precision saddle
+               point: FP32 propagators while the error is >= tol, FP64 after; only
+               FP64 evaluations can declare convergence, so every saddle point
+               meets the same FP64 tolerance;
+  learned      K steps with both (with --precision fp64: the CNN alone).
 
-`value` is the learned arm's Langevin steps/s (the plain arm's with --guess plain);
-config carries the plain rate, their ratio, the iterations per step of each arm and
-the rate of a 10,000-step run that trains on its first steps.
+`value` is the last arm's Langevin steps/s; config carries every arm's rate and
+iterations per step, the ratio to the plain arm and the rate of a 10,000-step run
+that trains on its first steps.
 
 Printed: one JSON line. `roofline` is the slab pass (the dominant kernel:
 inverse y/z FFT, exp(-w ds) factor, q store, forward z/y FFT), timed during the
@@ -509,6 +513,8 @@ def main():
     ap.add_argument("--guess", default="learned", choices=["learned", "plain"],
                     help="learned: after the plain arm, train the CNN guess and time the steps that use it")
     ap.add_argument("--guess-train-steps", type=int, default=1000, help="Langevin steps the CNN trains on")
+    ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"],
+                    help="mixed: saddle iterations on FP32 propagators until the error < tol, then FP64")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
@@ -539,10 +545,19 @@ def main():
         reduce_bench(args, w, lf)
         return
     params = lfts_params(w, "/tmp/lfts_bench")
+    mixed = args.precision == "mixed"
+    params["saddle"]["mixed_precision"] = mixed
     params["learned_guess"] = {"train_steps": args.guess_train_steps, "lr": 1e-3, "train_iters": 4, "replay": 64,
                                "delta": True, "seed": 0}
     with stdout_to_stderr():
         sim = lfts.LFTS(params, random_seed=args.seed)
+        tol = sim.saddle["tolerance"]
+
+        def precision(low):
+            if mixed:
+                sim.saddle_device.set_low_precision_solver(sim.low_precision_solver if low else None, tol)
+
+        precision(False)
         rng = np.random.Generator(np.random.PCG64(args.seed))
         w_mono = np.stack([rng.standard_normal(M), rng.standard_normal(M)])
         sim.saddle_device.set_fields(np.ascontiguousarray(sim.mpt.to_aux_fields(w_mono)))
@@ -553,11 +568,13 @@ def main():
 
         if args.guess == "learned":
             # online training on the run's own saddle points (plain saddle points, then Adam steps)
+            precision(True)
             train = {"iterations": 0, "failures": 0}
             t0 = time.perf_counter()
             langevin_steps(sim, args.guess_train_steps, train, learned=True)
             sim.saddle_device.field_moments()
             train_s = time.perf_counter() - t0
+            precision(False)
 
         # plain arm: the reference's method (each saddle point starts from the previous w+)
         if not args.no_kernel_timing:
@@ -573,15 +590,28 @@ def main():
             kern = {f: lf.get_kernel_timing(f) for f in ("slab", "pencil")}
             lf.set_kernel_timing(0)
 
-        learned = None
-        if args.guess == "learned":
-            # the same state and tolerance, saddle points starting from the CNN's prediction
-            learned = {"iterations": 0, "failures": 0}
+        def arm(learned_guess, low):
+            precision(low)
+            st = {"iterations": 0, "failures": 0}
             t0 = time.perf_counter()
-            langevin_steps(sim, args.steps, learned, learned=True)
+            langevin_steps(sim, args.steps, st, learned=learned_guess)
             sim.saddle_device.field_moments()
-            learned_s = time.perf_counter() - t0
+            st["s"] = time.perf_counter() - t0
+            precision(False)
+            return st
 
+        # the same tolerance, each arm continuing from the state the previous one left
+        learned = learned64 = plain_mixed = None
+        if args.guess == "learned":
+            learned64 = arm(True, False)    # saddle points from the CNN's prediction, FP64 iterations
+        if mixed:
+            plain_mixed = arm(False, True)  # the reference's start, FP32 iterations until err < tol
+        if args.guess == "learned":
+            learned = arm(True, mixed)
+            learned_s = learned["s"]
+
+    if learned is None and plain_mixed is not None:
+        learned, learned_s = plain_mixed, plain_mixed["s"]
     stats = learned if learned is not None else plain
     elapsed = learned_s if learned is not None else plain_s
     steps_per_s = args.steps / elapsed
@@ -591,16 +621,26 @@ def main():
         "metric": "L-FTS steps/s @64³ (learned vs plain AM guess, same tol); propagator HBM GB/s",
         "value": steps_per_s, "unit": "steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": 1e3 * elapsed / args.steps, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f64",
+        "vs_baseline": None,
+        "dtype": "f32 propagators until err < tol, then f64; f64 Q/phi/AM" if mixed else "f64",
         "data": "synthetic: w_A, w_B ~ N(0,1) from PCG64(seed); Langevin noise from numpy PCG64",
         "config": {"workload": describe(args.workload, w),
-                   "guess": "learned CNN (delta form) + Anderson mixing" if learned else "plain Anderson mixing",
+                   "guess": ("learned CNN (delta form) + Anderson mixing" if args.guess == "learned"
+                             else "plain Anderson mixing")
+                            + ("; FP32 saddle iterations until the error < tol, FP64 after (only FP64 "
+                               "evaluations declare convergence)" if mixed else ""),
                    "saddle_iterations_per_step": iters, "saddle_failures": stats["failures"],
                    "plain_steps_per_s": args.steps / plain_s, "plain_saddle_iterations_per_step": plain_iters,
                    "plain_saddle_failures": plain["failures"],
                    "propagator_evaluations_per_s": plain["iterations"] / plain_s},
     }
-    if learned is not None:
+    if learned64 is not None:
+        out["config"]["learned_fp64_steps_per_s"] = args.steps / learned64["s"]
+        out["config"]["learned_fp64_saddle_iterations_per_step"] = learned64["iterations"] / args.steps
+    if plain_mixed is not None:
+        out["config"]["plain_mixed_steps_per_s"] = args.steps / plain_mixed["s"]
+        out["config"]["plain_mixed_saddle_iterations_per_step"] = plain_mixed["iterations"] / args.steps
+    if args.guess == "learned":
         n_train, n_run = args.guess_train_steps, 10000
         train_step_s = train_s / max(n_train, 1)
         out["config"].update({
@@ -615,7 +655,8 @@ def main():
         out["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBPS, "unit": "GB/s",
                            "frac": achieved / PEAK_HBM_GBPS,
                            "traffic": traffic["bytes"] if traffic else None,
-                           "kernel": "slab pass (slab_kernel_wl<double, 1, 64>: 2 propagators per launch)",
+                           "kernel": "slab pass (slab_kernel_wl<double, 1, 64>: 2 propagators per launch), "
+                                     "timed in the plain FP64 arm",
                            "timed_launch_count": sl["launches"],
                            "avg_launch_us": 1e3 * sl["milliseconds"] / max(sl["launches"], 1),
                            "bytes_per_launch": sl["bytes"] / max(sl["launches"], 1),

@@ -291,6 +291,15 @@ int fts_cnn_gradients(fts_cnn* c, const double* w_minus, const double* w_plus_ta
 /* one Adam step (beta 0.9 / 0.999, eps 1e-8) on one (w-, w+) pair */
 int fts_cnn_train_step(fts_cnn* c, const double* w_minus, const double* w_plus_target, int on_device, double lr,
                        double* loss);
+/* Mixed-precision saddle point (no reference counterpart; an addition for the
+ * L-FTS throughput): while the error of any active replica is at or above
+ * switch_error, the iterations evaluate the propagators with `lo`, a solver made
+ * from the same plan and replica count at FTS_FP32; below it they use the
+ * saddle's own solver, and only those evaluations can declare convergence, so the
+ * returned fields, concentrations and Hamiltonian meet the tolerance in the main
+ * solver's precision. lo = NULL switches it off. lo must outlive its use here. */
+int fts_saddle_set_low_precision(fts_saddle* s, fts_solver* lo, double switch_error);
+
 /* Field theories with one real and one imaginary field (AB melts), on the device:
  * prepare: remember f(w_aux[real]) before the Langevin update;
  * predict: w_aux[imag] = f(w_aux[real]) (delta = 0), or w_aux[imag] += f(w_aux[real]) - the

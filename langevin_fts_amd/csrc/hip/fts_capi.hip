@@ -745,6 +745,14 @@ int fts_cnn_train_step(fts_cnn* c, const double* w_minus, const double* w_plus_t
     });
 }
 
+int fts_saddle_set_low_precision(fts_saddle* s, fts_solver* lo, double switch_error) {
+    return guarded([&] {
+        need(s, "saddle");
+        FTS_REQUIRE(switch_error >= 0.0, "switch_error must be non-negative");
+        s->s->set_low_precision(lo ? lo->s.get() : nullptr, switch_error);
+    });
+}
+
 int fts_saddle_prepare_guess(fts_saddle* s, fts_cnn* c) {
     return guarded([&] {
         need(s, "saddle");

@@ -215,6 +215,12 @@ public:
     void langevin_update(const double* noise, bool on_device, double dt);
     void field_moments(double* mean, double* mean_sq);
     int n_replicas() const { return NR_; }
+    // iterations run on `lo` (same plan, lower precision) while any active replica's
+    // error is at or above switch_error; convergence is only declared on the main solver
+    void set_low_precision(SolverBase* lo, double switch_error) {
+        lo_ = lo;
+        lo_switch_ = switch_error;
+    }
     // learned guess of the imaginary field from the real one (single replica)
     void prepare_guess(Cnn& cnn);               // f(w_real) before the Langevin update (delta mode)
     void predict_guess(Cnn& cnn, bool delta);   // w_imag = f(w_real), or w_imag += f(w_real) - prepared
@@ -223,14 +229,16 @@ public:
 private:
     double* wa(int r) const { return w_aux_.as<double>() + (size_t)r * S_ * M_; }
     double* ph(int r) const { return phi_.as<double>() + (size_t)r * S_ * M_; }
-    void concentrations();
+    void concentrations(SolverBase* solver);
     void run_lincomb(int rep, const std::vector<double>& cw, const std::vector<double>& cp,
                      const std::vector<double>& c0, int n_out, double* out);
     void row_sums(const std::vector<const double*>& rows, const double* centre, bool square, double* out_dev);
-    double hamiltonian(int rep);
+    double hamiltonian(int rep, SolverBase* solver);
 
     fts_context* ctx_;
     SolverBase* solver_;
+    SolverBase* lo_ = nullptr;
+    double lo_switch_ = 0.0;
     std::vector<AndersonMixingDevice*> ams_;
     int M_, NR_, n_blocks_, n_solvents_, S_, R_, I_, P_, n_solver_w_;
     std::vector<int> real_idx_, imag_idx_;

@@ -296,21 +296,21 @@ void SaddlePoint::row_sums(const std::vector<const double*>& rows, const double*
     }
 }
 
-void SaddlePoint::concentrations() {
+void SaddlePoint::concentrations(SolverBase* solver) {
     // monomer fields of the solver (lfts.py:534-545), replica r's at r n_solver_w
     for (int r = 0; r < NR_; r++)
         run_lincomb(r, w_map_, {}, {}, n_solver_w_, w_solver_.as<double>() + (size_t)r * n_solver_w_ * M_);
     std::vector<const double*> w(n_solver_w_ * NR_);
     for (size_t m = 0; m < w.size(); m++) w[m] = w_solver_.as<double>() + m * M_;
-    solver_->compute_propagators(w.data(), nullptr, true);
-    solver_->compute_concentrations();
+    solver->compute_propagators(w.data(), nullptr, true);
+    solver->compute_concentrations();
     std::vector<int> idx(term_index_.size());
     for (int r = 0; r < NR_; r++) {
         for (size_t t = 0; t < idx.size(); t++)
             idx[t] = term_index_[t] + r * (term_kind_[t] == FTS_SRC_BLOCK ? n_blocks_ : n_solvents_);
         for (int i = 0; i < S_; i++) {
             const int o = term_off_[i], n = term_off_[i + 1] - o;
-            solver_->sum_concentrations(n, term_kind_.data() + o, idx.data() + o, term_scale_.data() + o,
+            solver->sum_concentrations(n, term_kind_.data() + o, idx.data() + o, term_scale_.data() + o,
                                         ph(r) + (size_t)i * M_, true);
         }
     }
@@ -334,7 +334,7 @@ void SaddlePoint::field_moments(double* mean, double* mean_sq) {
 }
 
 // lfts.py:197-216: H = sum_j (h_sq_j <w_j^2> + h_lin_j <w_j>) - sum_p phi_p/alpha_p ln Q_p + h_const
-double SaddlePoint::hamiltonian(int rep) {
+double SaddlePoint::hamiltonian(int rep, SolverBase* solver) {
     std::vector<const double*> rows;
     for (int j = 0; j < S_; j++) rows.push_back(wa(rep) + (size_t)j * M_);
     double* st = stats_.as<double>();
@@ -347,7 +347,7 @@ double SaddlePoint::hamiltonian(int rep) {
     for (int j = 0; j < S_; j++) h += h_lin_[j] * (m[j] / M_);
     for (int j = 0; j < S_; j++) h += h_sq_[j] * (m[S_ + j] / M_);
     double hp = 0.0;
-    for (int p = 0; p < P_; p++) hp -= pol_weight_[p] * std::log(solver_->partition(p + rep * P_));
+    for (int p = 0; p < P_; p++) hp -= pol_weight_[p] * std::log(solver->partition(p + rep * P_));
     return hp + h + h_const_;
 }
 
@@ -363,8 +363,11 @@ void SaddlePoint::find(double tolerance, int max_iter, int verbose, double* log,
     std::vector<double> stats(2 * nrows);
     bool contiguous = true;
     for (int c = 1; c < I_; c++) contiguous = contiguous && imag_idx_[c] == imag_idx_[0] + c;
+    // low-precision phase while the errors are large (none when no low-precision solver is set)
+    bool low = lo_ != nullptr;
     for (int it = 1; it <= max_iter; it++) {
-        concentrations();
+        SolverBase* solver = low ? lo_ : solver_;
+        concentrations(solver);
         // dH/dw of the imaginary fields, sign already flipped (lfts.py:219-245)
         std::vector<const double*> rows;
         for (int r = 0; r < NR_; r++) {
@@ -390,19 +393,20 @@ void SaddlePoint::find(double tolerance, int max_iter, int verbose, double* log,
             err[r] = e;
             // the Hamiltonian and the printed row of lfts.py:834-849, at every iteration
             // (verbose 2) or at the final one
-            const bool last = e < tolerance || it == max_iter;
+            const bool converged = !low && e < tolerance;
+            const bool last = converged || it == max_iter;
             if ((NR_ == 1 && verbose == 2) || last) {
-                ham[r] = hamiltonian(r);
+                ham[r] = hamiltonian(r, solver);
                 if (NR_ == 1 && log && verbose >= 1) {
                     double* lr = log + (size_t)(verbose == 2 ? it - 1 : 0) * row;
                     lr[0] = it;
                     lr[1] = stats[I_ - 1] / M_;   // mass error: mean of the pressure-field derivative
                     lr[2] = ham[r];
-                    for (int p = 0; p < P_; p++) lr[3 + p] = solver_->partition(p);
+                    for (int p = 0; p < P_; p++) lr[3 + p] = solver->partition(p);
                     for (int c = 0; c < I_; c++) lr[3 + P_ + c] = err_i[c];
                 }
             }
-            if (e < tolerance) {
+            if (converged) {
                 done[r] = it;
                 continue;
             }
@@ -436,6 +440,12 @@ void SaddlePoint::find(double tolerance, int max_iter, int verbose, double* log,
             }
         }
         if (active == 0) break;
+        if (low) {
+            bool below = true;
+            for (int r = 0; r < NR_; r++)
+                if (!done[r] && !(err[r] < lo_switch_)) below = false;
+            low = !below;
+        }
     }
     // zero mean of the pressure field (lfts.py:865)
     std::vector<const double*> last_rows;

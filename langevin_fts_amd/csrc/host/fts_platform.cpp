@@ -592,6 +592,7 @@ DeviceSaddlePoint::DeviceSaddlePoint(PropagatorComputation* solver, AndersonMixi
                                      const std::vector<double>& am_params) {
     if (!solver) LFTS_THROW("solver is required");
     NR_ = solver->n_replicas();
+    n_blocks_ = solver->n_blocks_per_replica();
     if (NR_ == 1 && !am) LFTS_THROW("am is required");
     if (NR_ > 1 && am_params.size() != 4)
         LFTS_THROW("a replica solver needs am_params = [max_hist, start_error, mix_min, mix_init]");
@@ -715,6 +716,13 @@ void DeviceSaddlePoint::langevin_update(const double* noise, double dt) {
 }
 void DeviceSaddlePoint::field_moments(double* mean, double* mean_sq) {
     check_fts(fts_saddle_field_moments(saddle_, mean, mean_sq));
+}
+
+void DeviceSaddlePoint::set_low_precision_solver(PropagatorComputation* lo, double switch_error) {
+    if (lo && lo->n_replicas() != NR_) LFTS_THROW("the low-precision solver must have the same number of replicas");
+    if (lo && lo->n_blocks_per_replica() != n_blocks_)
+        LFTS_THROW("the low-precision solver must be built from the same molecules and analyzer");
+    check_fts(fts_saddle_set_low_precision(saddle_, lo ? lo->handle() : nullptr, switch_error));
 }
 
 void DeviceSaddlePoint::prepare_guess(LearnedGuess& g) { check_fts(fts_saddle_prepare_guess(saddle_, g.handle())); }

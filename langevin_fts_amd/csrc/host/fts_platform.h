@@ -207,10 +207,13 @@ public:
     void field_moments(double* mean, double* mean_sq);
     void prepare_guess(LearnedGuess& g);
     void predict_guess(LearnedGuess& g, bool delta);
+    // iterations on `lo` (an FP32 solver of the same molecules) while the error is at or above switch_error
+    void set_low_precision_solver(PropagatorComputation* lo, double switch_error);
     double train_guess(LearnedGuess& g, double lr, int iters, int replay);
 
 private:
     int S_, R_, I_, P_, M_, NR_ = 1;
+    int n_blocks_ = 0;   // per replica, of the saddle's own solver
     fts_saddle* saddle_ = nullptr;
 };
 

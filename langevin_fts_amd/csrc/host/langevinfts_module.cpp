@@ -366,6 +366,8 @@ PYBIND11_MODULE(langevinfts, m) {
                  o.field_moments(mean.data(), mean_sq.data());
                  return py::make_tuple(mean, mean_sq);
              })
+        .def("set_low_precision_solver", &DeviceSaddlePoint::set_low_precision_solver, py::arg("solver"),
+             py::arg("switch_error"), py::keep_alive<1, 2>())
         .def("prepare_guess", &DeviceSaddlePoint::prepare_guess)
         .def("predict_guess", &DeviceSaddlePoint::predict_guess, py::arg("guess"), py::arg("delta") = false)
         .def("train_guess", &DeviceSaddlePoint::train_guess, py::arg("guess"), py::arg("lr"), py::arg("iters") = 1,

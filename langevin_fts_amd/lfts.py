@@ -332,6 +332,19 @@ class LFTS:
             rcw.ravel().tolist(), rcp.ravel().tolist(), rc0.tolist(), icw.ravel().tolist(), icp.ravel().tolist(),
             ic0.tolist(), self.dt_scaling.tolist(), h_lin.tolist(), h_sq.tolist(), float(mpt.h_const),
             am_params if self.n_replicas > 1 else [])
+        # mixed precision (an addition): saddle iterations on an FP32 solver of the same molecules until
+        # the error falls below the switch value, then FP64 iterations, which alone can meet the tolerance
+        self.low_precision_solver = None
+        if params["saddle"].get("mixed_precision", False):
+            if platform != "hip":
+                raise ValueError("saddle.mixed_precision needs the FP64 platform \"hip\"")
+            f32 = lf.PlatformSelector.create_factory("hip-fp32", False)
+            if self.n_replicas == 1:
+                self.low_precision_solver = f32.create_pseudospectral_solver(cb, molecules, analyzer)
+            else:
+                self.low_precision_solver = f32.create_replica_solver(cb, molecules, analyzer, self.n_replicas)
+            switch = params["saddle"].get("mixed_precision_switch", params["saddle"]["tolerance"])
+            self.saddle_device.set_low_precision_solver(self.low_precision_solver, switch)
 
         print("---------- Simulation Parameters ----------")
         print("Platform :", platform)
