@@ -113,11 +113,9 @@ def langevin_steps(sim, n, stats, learned=False):
     """n Langevin steps on the device state (lfts.py:687-731); with learned=True the
     driver's learned-guess step (training until the network is ready, then prediction)."""
     dev = sim.saddle_device
-    M = sim.cb.get_n_grid()
-    R = len(sim.mpt.aux_fields_real_idx)
     tol = sim.saddle["tolerance"]
     for _ in range(n):
-        noise = sim.random.normal(0.0, sim.langevin["sigma"], [R, M])
+        noise = sim.next_noise()   # drawn ahead while the previous saddle point ran
         if learned:
             it, err, ok = sim.learned_step(noise)
         else:

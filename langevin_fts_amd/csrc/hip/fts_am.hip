@@ -1,8 +1,9 @@
 // Anderson mixing with device-resident history (CpuAndersonMixing.cpp:76-161,
 // AndersonMixing.cpp:19-47, CircularBuffer.cpp:4-54). The n_var-sized history
-// rings and the mixing stay in HBM; only the (n_anderson+1) dot products cross to
-// the host, where the small FP64 Gram system is solved by the unpivoted
-// elimination of AndersonMixing::find_an.
+// rings, the ring of dot-product rows, the small FP64 Gram solve (the unpivoted
+// elimination of AndersonMixing::find_an, in the same operation order) and the
+// mixing all stay on the device, so a call enqueues work without waiting on it;
+// the host keeps only the ring indices and the mixing decisions.
 #include <algorithm>
 #include <cmath>
 
@@ -47,8 +48,54 @@ struct AmMixArgs {
     const double* e1;
     const double* w2[AM_MAX_HIST];
     const double* e2[AM_MAX_HIST];
-    double a[AM_MAX_HIST];
+    const double* a;   // n_hist coefficients from am_solve
 };
+
+struct AmSolveArgs {
+    int na;                     // n_anderson
+    int L;                      // ring length (row stride)
+    int rows[AM_MAX_HIST];      // ring row of D(n, .) for n = 0 (newest) .. na
+};
+
+// U a = V with V_i = D(0,0) - D(0,i+1), U_ij = D(0,0) - D(0,i+1) - D(0,j+1) + D(min(i+1,j+1), |i-j|),
+// by forward elimination without pivoting and back substitution (AndersonMixing.cpp:19-47,
+// CpuAndersonMixing.cpp:128-147), with the host code's operation order and no contraction into
+// fused multiply-adds, so the coefficients are those the host would compute. Row j of the
+// elimination belongs to thread j; one workgroup of AM_MAX_HIST threads.
+__global__ void __launch_bounds__(AM_MAX_HIST) am_solve(const double* ring, AmSolveArgs a, double* coef) {
+#pragma clang fp contract(off)
+    __shared__ double u[AM_MAX_HIST - 1][AM_MAX_HIST];
+    __shared__ double v[AM_MAX_HIST];
+    __shared__ double x[AM_MAX_HIST];
+    const int na = a.na, L = a.L, t = threadIdx.x;
+    const double* d0 = ring + (size_t)a.rows[0] * L;
+    if (t < na) {
+        v[t] = d0[0] - d0[t + 1];
+        for (int j = 0; j < na; j++) {
+            const int m = t < j ? t + 1 : j + 1;
+            const int c = t > j ? t - j : j - t;
+            u[t][j] = d0[0] - d0[t + 1] - d0[j + 1] + ring[(size_t)a.rows[m] * L + c];
+        }
+    }
+    __syncthreads();
+    for (int i = 0; i < na; i++) {
+        if (t > i && t < na) {
+            const double factor = u[t][i] / u[i][i];
+            v[t] = v[t] - v[i] * factor;
+            for (int k = i + 1; k < na; k++) u[t][k] = u[t][k] - u[i][k] * factor;
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        x[na - 1] = v[na - 1] / u[na - 1][na - 1];
+        for (int i = na - 2; i >= 0; i--) {
+            double s = 0.0;
+            for (int j = i + 1; j < na; j++) s = s + u[i][j] * x[j];
+            x[i] = (v[i] - s) / u[i][i];
+        }
+        for (int i = 0; i < na; i++) coef[i] = x[i];
+    }
+}
 
 // w_new = (w_0 + d_0) + sum_i a_i ((w_{i+1} + d_{i+1}) - (w_0 + d_0)), same operation order
 __global__ void am_mix(AmMixArgs args, int n, double* out) {
@@ -87,8 +134,9 @@ AndersonMixingDevice::AndersonMixingDevice(fts_context* ctx, int n_var, int max_
     in_d_.allocate(sizeof(double) * n_var);
     out_.allocate(sizeof(double) * n_var);
     partials_.allocate(sizeof(double) * MAX_BATCH * RED_BLOCKS);
-    dots_d_.allocate(sizeof(double) * rows);
-    dots_.assign(rows * rows, 0.0);
+    dots_ring_.allocate(sizeof(double) * rows * rows);
+    coef_.allocate(sizeof(double) * AM_MAX_HIST);
+    FTS_HIP_CHECK(hipMemset(dots_ring_.get(), 0, dots_ring_.bytes()));
     reset();
 }
 
@@ -113,7 +161,7 @@ void AndersonMixingDevice::calculate_new_fields(double* w_new, const double* w, 
         dd = in_d_.as<double>();
     }
     auto ring_row = [&](int n) { return (ring_start_ + ring_n_ - n - 1 + L) % L; };
-    auto dots_row = [&](int n) { return &dots_[(size_t)((dots_start_ + dots_n_ - n - 1 + L) % L) * L]; };
+    auto dots_row = [&](int n) { return (dots_start_ + dots_n_ - n - 1 + L) % L; };
 
     if (err < start_error_ || n_anderson_ >= 0) n_anderson_ = n_anderson_ + 1;
     if (n_anderson_ >= 0) {
@@ -124,7 +172,10 @@ void AndersonMixingDevice::calculate_new_fields(double* w_new, const double* w, 
         FTS_HIP_CHECK(hipMemcpyAsync(d_hist_.as<double>() + (size_t)slot * n_var_, dd, nb, hipMemcpyDefault, st));
         if (ring_n_ == L) ring_start_ = (ring_start_ + 1) % L;
         ring_n_ = std::min(ring_n_ + 1, L);
-        // dots[i] = d . d_hist[i], i = 0..n_anderson (no volume weights)
+        // dots[i] = d . d_hist[i], i = 0..n_anderson (no volume weights), straight into the next row
+        // of the device ring of dot-product rows
+        const int ds = (dots_start_ + dots_n_) % L;
+        double* drow = dots_ring_.as<double>() + (size_t)ds * L;
         for (int i0 = 0; i0 <= n_anderson_; i0 += MAX_BATCH) {
             AmDotBatch b{};
             b.n = std::min(MAX_BATCH, n_anderson_ + 1 - i0);
@@ -133,15 +184,9 @@ void AndersonMixingDevice::calculate_new_fields(double* w_new, const double* w, 
             hipLaunchKernelGGL(am_dot_stage1, dim3(RED_BLOCKS, b.n), dim3(RED_THREADS), 0, st, b, n_var_,
                                partials_.as<double>());
             hipLaunchKernelGGL(am_dot_stage2, dim3(b.n), dim3(RED_BLOCKS), 0, st, partials_.as<double>(),
-                               dots_d_.as<double>() + i0);
+                               drow + i0);
             FTS_HIP_CHECK(hipGetLastError());
         }
-        std::vector<double> row(L, 0.0);
-        FTS_HIP_CHECK(hipMemcpyAsync(row.data(), dots_d_.get(), sizeof(double) * (n_anderson_ + 1),
-                                     hipMemcpyDeviceToHost, st));
-        FTS_HIP_CHECK(hipStreamSynchronize(st));
-        const int ds = (dots_start_ + dots_n_) % L;
-        std::copy(row.begin(), row.end(), dots_.begin() + (size_t)ds * L);
         if (dots_n_ == L) dots_start_ = (dots_start_ + 1) % L;
         dots_n_ = std::min(dots_n_ + 1, L);
     }
@@ -154,27 +199,13 @@ void AndersonMixingDevice::calculate_new_fields(double* w_new, const double* w, 
         FTS_HIP_CHECK(hipGetLastError());
     } else {
         const int na = n_anderson_;
-        std::vector<double> u((size_t)na * na), v(na), a(na);
-        const double* d0 = dots_row(0);
-        for (int i = 0; i < na; i++) {
-            v[i] = d0[0] - d0[i + 1];
-            for (int j = 0; j < na; j++)
-                u[(size_t)i * na + j] =
-                    d0[0] - d0[i + 1] - d0[j + 1] + dots_row(std::min(i + 1, j + 1))[std::abs(i - j)];
-        }
-        // AndersonMixing::find_an: Gaussian elimination without pivoting
-        for (int i = 0; i < na; i++)
-            for (int j = i + 1; j < na; j++) {
-                const double factor = u[(size_t)j * na + i] / u[(size_t)i * na + i];
-                v[j] = v[j] - v[i] * factor;
-                for (int k = i + 1; k < na; k++) u[(size_t)j * na + k] = u[(size_t)j * na + k] - u[(size_t)i * na + k] * factor;
-            }
-        a[na - 1] = v[na - 1] / u[(size_t)(na - 1) * na + na - 1];
-        for (int i = na - 2; i >= 0; i--) {
-            double t = 0.0;
-            for (int j = i + 1; j < na; j++) t = t + u[(size_t)i * na + j] * a[j];
-            a[i] = (v[i] - t) / u[(size_t)i * na + i];
-        }
+        AmSolveArgs sa{};
+        sa.na = na;
+        sa.L = L;
+        for (int n = 0; n <= na; n++) sa.rows[n] = dots_row(n);
+        hipLaunchKernelGGL(am_solve, dim3(1), dim3(AM_MAX_HIST), 0, st, dots_ring_.as<const double>(), sa,
+                           coef_.as<double>());
+        FTS_HIP_CHECK(hipGetLastError());
         AmMixArgs args{};
         args.n_hist = na;
         args.w1 = w_hist_.as<double>() + (size_t)ring_row(0) * n_var_;
@@ -182,8 +213,8 @@ void AndersonMixingDevice::calculate_new_fields(double* w_new, const double* w, 
         for (int i = 0; i < na; i++) {
             args.w2[i] = w_hist_.as<double>() + (size_t)ring_row(i + 1) * n_var_;
             args.e2[i] = d_hist_.as<double>() + (size_t)ring_row(i + 1) * n_var_;
-            args.a[i] = a[i];
         }
+        args.a = coef_.as<const double>();
         hipLaunchKernelGGL(am_mix, dim3(gb), dim3(256), 0, st, args, n_var_, out);
         FTS_HIP_CHECK(hipGetLastError());
     }

@@ -188,10 +188,10 @@ private:
     fts_context* ctx_;
     int n_var_, max_hist_, n_anderson_;
     double start_error_, mix_min_, mix_, mix_init_;
-    // device rings of max_hist+1 rows; host ring of dot products (CircularBuffer.cpp semantics)
-    DeviceBuffer w_hist_, d_hist_, in_w_, in_d_, out_, partials_, dots_d_;
+    // device rings of max_hist+1 rows: fields, derivatives and dot-product rows
+    // (CircularBuffer.cpp semantics; the host keeps the indices)
+    DeviceBuffer w_hist_, d_hist_, in_w_, in_d_, out_, partials_, dots_ring_, coef_;
     int ring_start_ = 0, ring_n_ = 0;
-    std::vector<double> dots_;   // (max_hist+1)^2 ring rows
     int dots_start_ = 0, dots_n_ = 0;
 };
 

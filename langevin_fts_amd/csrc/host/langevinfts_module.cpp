@@ -345,8 +345,13 @@ PYBIND11_MODULE(langevinfts, m) {
                 const int R = o.n_replicas();
                 std::vector<int> n_iter(R, 0);
                 std::vector<double> err(R, 0.0), ham(R, 0.0);
-                o.find(tolerance, max_iter, R == 1 ? verbose : 0, R == 1 ? log.mutable_data() : nullptr,
-                       n_iter.data(), err.data(), ham.data());
+                double* log_rows = R == 1 ? log.mutable_data() : nullptr;
+                {
+                    // the iteration waits on the device: let other Python threads (the next
+                    // step's noise) run meanwhile
+                    py::gil_scoped_release nogil;
+                    o.find(tolerance, max_iter, R == 1 ? verbose : 0, log_rows, n_iter.data(), err.data(), ham.data());
+                }
                 if (R == 1) return py::tuple(py::make_tuple(n_iter[0], err[0], ham[0], log));
                 return py::tuple(py::make_tuple(n_iter, err, ham, py::none()));
             },

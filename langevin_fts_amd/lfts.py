@@ -218,6 +218,43 @@ def guess_initial_params(seed):
     return p
 
 
+class NoiseStream:
+    """The Langevin noise of one numpy Generator, drawn one step ahead on a helper
+    thread: the next step's noise is filled while the device runs the current saddle
+    point (numpy releases the GIL while it fills an array, and the saddle-point call
+    releases it while it waits on the device). Draws come in the order of direct
+    calls; rewind() puts the generator back to the state before a pending draw, so a
+    checkpoint stores the state the next step will draw from."""
+
+    def __init__(self, generator, sigma, shape):
+        from concurrent.futures import ThreadPoolExecutor
+
+        self.generator = generator
+        self.sigma = sigma
+        self.shape = list(shape)
+        self.pool = ThreadPoolExecutor(max_workers=1)
+        self.pending = None   # (generator state before the draw, future)
+
+    def _launch(self):
+        state = self.generator.bit_generator.state
+        self.pending = (state, self.pool.submit(self.generator.normal, 0.0, self.sigma, self.shape))
+
+    def next(self):
+        if self.pending is None:
+            self._launch()
+        _, future = self.pending
+        noise = future.result()
+        self._launch()
+        return noise
+
+    def rewind(self):
+        if self.pending is not None:
+            state, future = self.pending
+            future.result()
+            self.generator.bit_generator.state = state
+            self.pending = None
+
+
 class LFTS:
     """L-FTS driver; `params` has the keys of the reference examples (fts/Lamella.py).
     An optional "learned_guess" entry {"train_steps", "lr", "train_iters", "seed"}
@@ -313,6 +350,7 @@ class LFTS:
 
         self.random_bg = np.random.PCG64() if random_seed is None else np.random.PCG64(random_seed)
         self.random = np.random.Generator(self.random_bg)
+        self.noise_stream = None   # NoiseStream over self.random, made on the first next_noise()
         if self.n_replicas > 1:
             base = 0 if random_seed is None else random_seed
             self.replica_random = [np.random.Generator(np.random.PCG64(base + r)) for r in range(self.n_replicas)]
@@ -457,6 +495,19 @@ class LFTS:
         self.guess_trained_steps += 1
         return loss
 
+    def next_noise(self):
+        """The next step's Langevin noise N(0, sigma^2) of shape [R, M] from self.random,
+        drawn ahead on a helper thread (NoiseStream)."""
+        if self.noise_stream is None:
+            R = len(self.mpt.aux_fields_real_idx)
+            self.noise_stream = NoiseStream(self.random, self.langevin["sigma"], [R, self.cb.get_n_grid()])
+        return self.noise_stream.next()
+
+    def rewind_noise(self):
+        """Undo a draw made ahead, so self.random's state is that of the next step."""
+        if self.noise_stream is not None:
+            self.noise_stream.rewind()
+
     def guess_ready(self):
         return self.guess is not None and self.guess_trained_steps >= self.learned["train_steps"]
 
@@ -519,6 +570,7 @@ class LFTS:
 
     def save_simulation_data(self, path, w, phi, langevin_step, normal_noise_prev):
         """`.mat` checkpoint with the keys of lfts.py:569-598."""
+        self.rewind_noise()   # the stored state is the one the next step draws from
         from scipy.io import savemat
 
         mdic = {
@@ -551,6 +603,7 @@ class LFTS:
             print(f"(Warning!) 'langevin_step' of {file_name} is not a multiple of 'sf_recording_period'.")
             nxt = (data["langevin_step"] // period + 1) * period
             print(f"The structure function will be correctly recorded after {nxt}th langevin_step.")
+        self.rewind_noise()
         self.random_bg.state = {"bit_generator": "PCG64",
                                 "state": {"state": int(data["random_state_state"]),
                                           "inc": int(data["random_state_inc"])},
@@ -602,7 +655,7 @@ class LFTS:
         for langevin_step in range(start_langevin_step, self.langevin["max_step"] + 1):
             print("Langevin step: ", langevin_step)
             dev.backup()
-            noise = self.random.normal(0.0, self.langevin["sigma"], [R, M])
+            noise = self.next_noise()
             ready = self.guess_ready()
             if ready and self.learned["delta"]:
                 dev.prepare_guess(self.guess)
