@@ -147,8 +147,18 @@ void AndersonMixingDevice::reset() {
     dots_start_ = dots_n_ = 0;
 }
 
+double* AndersonMixingDevice::staged_w() const {
+    return w_hist_.as<double>() + (size_t)((ring_start_ + ring_n_) % (max_hist_ + 1)) * n_var_;
+}
+
+double* AndersonMixingDevice::staged_d() const {
+    return d_hist_.as<double>() + (size_t)((ring_start_ + ring_n_) % (max_hist_ + 1)) * n_var_;
+}
+
 void AndersonMixingDevice::calculate_new_fields(double* w_new, const double* w, const double* d, double old_err,
-                                                double err, bool on_device) {
+                                                double err, bool on_device, bool staged) {
+    FTS_REQUIRE(!staged || (on_device && will_insert(err) && w == staged_w() && d == staged_d()),
+                "staged fields must sit in the next history slot");
     hipStream_t st = ctx_->stream;
     const int L = max_hist_ + 1;
     const size_t nb = sizeof(double) * n_var_;
@@ -168,8 +178,10 @@ void AndersonMixingDevice::calculate_new_fields(double* w_new, const double* w, 
         n_anderson_ = std::min(max_hist_, n_anderson_);
         const int slot = (ring_start_ + ring_n_) % L;
         // the rings may be pinned host memory: the runtime picks the direction
-        FTS_HIP_CHECK(hipMemcpyAsync(w_hist_.as<double>() + (size_t)slot * n_var_, dw, nb, hipMemcpyDefault, st));
-        FTS_HIP_CHECK(hipMemcpyAsync(d_hist_.as<double>() + (size_t)slot * n_var_, dd, nb, hipMemcpyDefault, st));
+        if (!staged) {
+            FTS_HIP_CHECK(hipMemcpyAsync(w_hist_.as<double>() + (size_t)slot * n_var_, dw, nb, hipMemcpyDefault, st));
+            FTS_HIP_CHECK(hipMemcpyAsync(d_hist_.as<double>() + (size_t)slot * n_var_, dd, nb, hipMemcpyDefault, st));
+        }
         if (ring_n_ == L) ring_start_ = (ring_start_ + 1) % L;
         ring_n_ = std::min(ring_n_ + 1, L);
         // dots[i] = d . d_hist[i], i = 0..n_anderson (no volume weights), straight into the next row

@@ -135,6 +135,9 @@ __device__ __forceinline__ void wl_stages(typename WRegs<T2, L, REV, S, NLW, LF>
 }
 
 // ------------------------------------------------------------------ slab pass
+#ifndef FTS_PENCIL_NL
+#define FTS_PENCIL_NL 8
+#endif
 #ifndef FTS_SLAB_THREADS_64
 // workgroup size of the 64^3 slab pass: 5 wavefronts own 7 y columns / 7 packed z rows
 // each (88% of the lanes busy in both phases); measured 2-3% faster than 4, 8 or 16
@@ -413,7 +416,7 @@ __global__ void __launch_bounds__(WSlab<N>::BD) slab_kernel_wl(SlabBatch<T> batc
 template <int N>
 struct WPencil {
     static constexpr int NZC = N / 2 + 1;
-    static constexpr int NL = 8;   // kz columns per wavefront
+    static constexpr int NL = FTS_PENCIL_NL;   // kz columns per wavefront
     static constexpr int TILES = (NZC + NL - 1) / NL;
     static constexpr int BD = 64;
     static constexpr size_t lds_elems = (size_t)N * NL + N;

@@ -181,8 +181,15 @@ public:
     AndersonMixingDevice(fts_context* ctx, int n_var, int max_hist, double start_error, double mix_min,
                          double mix_init, bool reduce_memory = false);
     void reset();
+    // staged: w and d are already the ring slots of staged_w() / staged_d() (the caller
+    // wrote them there, which it may do when will_insert(err))
     void calculate_new_fields(double* w_new, const double* w, const double* d, double old_err, double err,
-                              bool on_device);
+                              bool on_device, bool staged = false);
+    // whether a call with this error level inserts into the history (CpuAndersonMixing.cpp:86-93)
+    bool will_insert(double err) const { return err < start_error_ || n_anderson_ >= 0; }
+    // the ring slots the next insertion fills (device, n_var doubles each)
+    double* staged_w() const;
+    double* staged_d() const;
 
 private:
     fts_context* ctx_;

@@ -415,19 +415,23 @@ void SaddlePoint::find(double tolerance, int max_iter, int verbose, double* log,
             double* am_w = am_w_.as<double>() + (size_t)r * I_ * M_;
             double* am_d = am_d_.as<double>() + (size_t)r * I_ * M_;
             const double* hr = h_.as<double>() + (size_t)r * I_ * M_;
+            // when the mixing keeps this iterate, the gather writes it straight into the history slot
+            const bool staged = ams_[r]->will_insert(e);
+            double* in_w = staged ? ams_[r]->staged_w() : am_w;
+            double* in_d = staged ? ams_[r]->staged_d() : am_d;
             Gather g{};
             g.n = I_;
             double* w_im = contiguous ? wa(r) + (size_t)imag_idx_[0] * M_ : am_w;
             for (int c = 0; c < I_; c++) {
                 g.src[c] = wa(r) + (size_t)imag_idx_[c] * M_;
                 g.h[c] = hr + (size_t)c * M_;
-                g.w_out[c] = contiguous ? nullptr : am_w + (size_t)c * M_;
-                g.d_out[c] = am_d + (size_t)c * M_;
+                g.w_out[c] = staged || !contiguous ? in_w + (size_t)c * M_ : nullptr;
+                g.d_out[c] = in_d + (size_t)c * M_;
                 g.scale[c] = dt_scaling_[imag_idx_[c]];
             }
             hipLaunchKernelGGL(gather_kernel, dim3(grid_blocks(M_)), dim3(256), 0, st, g, M_);
             FTS_HIP_CHECK(hipGetLastError());
-            ams_[r]->calculate_new_fields(w_im, w_im, am_d, old_err, e, true);
+            ams_[r]->calculate_new_fields(w_im, staged ? in_w : w_im, in_d, old_err, e, true, staged);
             if (!contiguous) {
                 Scatter sc{};
                 sc.n = I_;

@@ -194,6 +194,19 @@ __global__ void convert_kernel(const double* in, T* out, int M) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < M; i += gridDim.x * blockDim.x) out[i] = (T)in[i];
 }
 
+// FP64 fields (one per blockIdx.y) into consecutive fields of the solver's precision
+struct ConvertBatch {
+    int n;
+    const double* src[MAX_BATCH];
+};
+
+template <typename T>
+__global__ void convert_batch_kernel(ConvertBatch b, T* out, int M) {
+    const double* in = b.src[blockIdx.y];
+    T* o = out + (size_t)blockIdx.y * M;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < M; i += gridDim.x * blockDim.x) o[i] = (T)in[i];
+}
+
 template <typename T>
 __global__ void widen_kernel(const T* in, double* out, int M) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < M; i += gridDim.x * blockDim.x) out[i] = (double)in[i];

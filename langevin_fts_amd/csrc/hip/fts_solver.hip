@@ -775,14 +775,31 @@ void Solver<T>::run_launches() {
 template <typename T>
 void Solver<T>::compute_propagators(const double* const* wf, const double* const* q_init, bool on_device) {
     const hipMemcpyKind kind = on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    const int cb = std::min(1024, ceil_div(M_, 256));
     for (int m = 0; m < n_mon_; m++) {
         FTS_REQUIRE(wf[m] != nullptr, "missing field for a monomer type");
-        FTS_HIP_CHECK(hipMemcpyAsync(w64_.as<double>() + (size_t)m * M_, wf[m], sizeof(double) * M_, kind,
-                                     g_->stream()));
+        if (std::is_same<T, double>::value) {
+            FTS_HIP_CHECK(hipMemcpyAsync(w64_.as<double>() + (size_t)m * M_, wf[m], sizeof(double) * M_, kind,
+                                         g_->stream()));
+        } else if (on_device) {
+            continue;   // converted below, straight from the caller's device fields
+        } else {
+            FTS_HIP_CHECK(hipMemcpyAsync(w64_.as<double>() + (size_t)m * M_, wf[m], sizeof(double) * M_, kind,
+                                         g_->stream()));
+            hipLaunchKernelGGL((convert_kernel<T>), dim3(cb), dim3(256), 0, g_->stream(),
+                               w64_.as<double>() + (size_t)m * M_, wT_ + (size_t)m * M_, M_);
+        }
     }
-    if (!std::is_same<T, double>::value)
-        hipLaunchKernelGGL((convert_kernel<T>), dim3(std::min(1024, ceil_div(n_mon_ * M_, 256))), dim3(256), 0,
-                           g_->stream(), w64_.as<double>(), wT_, n_mon_ * M_);
+    if (!std::is_same<T, double>::value && on_device) {
+        for (int m0 = 0; m0 < n_mon_; m0 += MAX_BATCH) {
+            ConvertBatch cv{};
+            cv.n = std::min(MAX_BATCH, n_mon_ - m0);
+            for (int k = 0; k < cv.n; k++) cv.src[k] = wf[m0 + k];
+            hipLaunchKernelGGL((convert_batch_kernel<T>), dim3(cb, cv.n), dim3(256), 0, g_->stream(), cv,
+                               wT_ + (size_t)m0 * M_, M_);
+        }
+    }
+    if (!std::is_same<T, double>::value) FTS_HIP_CHECK(hipGetLastError());
     for (int i = 0; i < n_qinit_; i++) {
         FTS_REQUIRE(q_init && q_init[i], "missing q_init field");
         FTS_HIP_CHECK(hipMemcpyAsync(qinit_.as<double>() + (size_t)i * M_, q_init[i], sizeof(double) * M_, kind,
@@ -795,10 +812,16 @@ void Solver<T>::compute_propagators(const double* const* wf, const double* const
     std::vector<int> target;
     auto flush = [&]() {
         if (!db.n) return;
-        g_->launch_dots<T>(db, g_->scalars.as<double>());
-        for (int k = 0; k < db.n; k++)
-            FTS_HIP_CHECK(hipMemcpyAsync(Qd_.as<double>() + target[k], g_->scalars.as<double>() + k, sizeof(double),
-                                         hipMemcpyDeviceToDevice, g_->stream()));
+        bool consecutive = true;
+        for (int k = 1; k < db.n; k++) consecutive = consecutive && target[k] == target[0] + k;
+        if (consecutive) {
+            g_->launch_dots<T>(db, Qd_.as<double>() + target[0]);
+        } else {
+            g_->launch_dots<T>(db, g_->scalars.as<double>());
+            for (int k = 0; k < db.n; k++)
+                FTS_HIP_CHECK(hipMemcpyAsync(Qd_.as<double>() + target[k], g_->scalars.as<double>() + k,
+                                             sizeof(double), hipMemcpyDeviceToDevice, g_->stream()));
+        }
         db.n = 0;
         target.clear();
     };
