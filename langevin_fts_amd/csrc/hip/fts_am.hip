@@ -64,17 +64,26 @@ struct AmSolveArgs {
 // elimination belongs to thread j; one workgroup of AM_MAX_HIST threads.
 __global__ void __launch_bounds__(AM_MAX_HIST) am_solve(const double* ring, AmSolveArgs a, double* coef) {
 #pragma clang fp contract(off)
-    __shared__ double u[AM_MAX_HIST - 1][AM_MAX_HIST];
-    __shared__ double v[AM_MAX_HIST];
-    __shared__ double x[AM_MAX_HIST];
+    // dynamic LDS: D (na+1 rows of L), u (na x na), v, x
+    extern __shared__ double lds[];
     const int na = a.na, L = a.L, t = threadIdx.x;
-    const double* d0 = ring + (size_t)a.rows[0] * L;
+    double* D = lds;
+    double (*u)[AM_MAX_HIST] = reinterpret_cast<double (*)[AM_MAX_HIST]>(D + (size_t)(na + 1) * L);
+    double* v = reinterpret_cast<double*>(u + na);
+    double* x = v + AM_MAX_HIST;
+    // the rows D(n, .), n = 0 (newest) .. na, in one coalesced pass
+    for (int e = t; e < (na + 1) * L; e += blockDim.x) {
+        const int n = e / L;
+        D[e] = ring[(size_t)a.rows[n] * L + (e - n * L)];
+    }
+    __syncthreads();
+    const double* d0 = D;
     if (t < na) {
         v[t] = d0[0] - d0[t + 1];
         for (int j = 0; j < na; j++) {
             const int m = t < j ? t + 1 : j + 1;
             const int c = t > j ? t - j : j - t;
-            u[t][j] = d0[0] - d0[t + 1] - d0[j + 1] + ring[(size_t)a.rows[m] * L + c];
+            u[t][j] = d0[0] - d0[t + 1] - d0[j + 1] + D[m * L + c];
         }
     }
     __syncthreads();
@@ -137,6 +146,11 @@ AndersonMixingDevice::AndersonMixingDevice(fts_context* ctx, int n_var, int max_
     dots_ring_.allocate(sizeof(double) * rows * rows);
     coef_.allocate(sizeof(double) * AM_MAX_HIST);
     FTS_HIP_CHECK(hipMemset(dots_ring_.get(), 0, dots_ring_.bytes()));
+    // the solve's LDS grows with the history (up to about 66 KB at 63)
+    const size_t solve_lds = sizeof(double) * (rows * rows + (rows - 1) * AM_MAX_HIST + 2 * AM_MAX_HIST);
+    if (solve_lds > 64 * 1024)
+        FTS_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&am_solve),
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)solve_lds));
     reset();
 }
 
@@ -215,7 +229,8 @@ void AndersonMixingDevice::calculate_new_fields(double* w_new, const double* w, 
         sa.na = na;
         sa.L = L;
         for (int n = 0; n <= na; n++) sa.rows[n] = dots_row(n);
-        hipLaunchKernelGGL(am_solve, dim3(1), dim3(AM_MAX_HIST), 0, st, dots_ring_.as<const double>(), sa,
+        const size_t lds = sizeof(double) * ((size_t)(na + 1) * L + (size_t)na * AM_MAX_HIST + 2 * AM_MAX_HIST);
+        hipLaunchKernelGGL(am_solve, dim3(1), dim3(AM_MAX_HIST), lds, st, dots_ring_.as<const double>(), sa,
                            coef_.as<double>());
         FTS_HIP_CHECK(hipGetLastError());
         AmMixArgs args{};

@@ -319,6 +319,7 @@ Cnn::Cnn(fts_context* ctx, int nx, int ny, int nz, double in_scale, double out_s
 }
 
 void Cnn::set_params(const float* host) {
+    version_++;
     hipStream_t st = ctx_->stream;
     FTS_HIP_CHECK(hipMemcpyAsync(params_.get(), host, sizeof(float) * N_PARAMS, hipMemcpyHostToDevice, st));
     FTS_HIP_CHECK(hipMemsetAsync(adam_m_.get(), 0, sizeof(float) * N_PARAMS, st));
@@ -446,6 +447,7 @@ double Cnn::gradients(const double* w_minus, const double* w_plus_target, bool o
 double Cnn::train_step(const double* w_minus, const double* w_plus_target, bool on_device, double lr) {
     const double loss = gradients(w_minus, w_plus_target, on_device);
     step_++;
+    version_++;
     const float b1 = 0.9f, b2 = 0.999f;
     const float c1 = 1.f - std::pow(b1, (float)step_), c2 = 1.f - std::pow(b2, (float)step_);
     hipLaunchKernelGGL(adam_step, dim3(blocks(N_PARAMS)), dim3(256), 0, ctx_->stream, params_.as<float>(),

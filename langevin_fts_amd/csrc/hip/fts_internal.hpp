@@ -260,6 +260,17 @@ private:
         partials_, stats_;
     // learned guess: replay ring of (w_real, w_imag) pairs and the prepared prediction
     DeviceBuffer replay_, guess_old_, guess_new_;
+    // which network (and parameter version) and which real field a guess buffer holds f of:
+    // the real field's version changes with every update of it and returns with a restore
+    struct GuessTag {
+        const Cnn* cnn = nullptr;
+        long long cnn_version = -1, field_version = -1;
+        bool operator==(const GuessTag& o) const {
+            return cnn == o.cnn && cnn_version == o.cnn_version && field_version == o.field_version;
+        }
+    };
+    GuessTag old_tag_, new_tag_;
+    long long field_version_ = 0, backup_field_version_ = 0, version_counter_ = 0;
     int replay_cap_ = 0, replay_n_ = 0, replay_head_ = 0;
     unsigned long long replay_rng_ = 0x9E3779B97F4A7C15ull;
     bool prepared_ = false;
@@ -271,6 +282,8 @@ int cnn_n_params();
 class Cnn {
 public:
     Cnn(fts_context* ctx, int nx, int ny, int nz, double in_scale, double out_scale);
+    // changes whenever the parameters do
+    long long version() const { return version_; }
     void set_params(const float* host);
     void get_params(float* host);
     void get_grads(float* host);
@@ -288,6 +301,7 @@ private:
     int nx_, ny_, nz_, M_, n_chunks_;
     double in_scale_, out_scale_;
     long long step_ = 0;
+    long long version_ = 0;
     DeviceBuffer params_, grads_, adam_m_, adam_v_, wf_, wb_, x_, target_, y_, dy_, acts_, grad_a_, grad_b_, partial_,
         loss_partials_, loss_, stage_in_, stage_out_;
 };

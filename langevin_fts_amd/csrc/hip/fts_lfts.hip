@@ -208,6 +208,7 @@ SaddlePoint::SaddlePoint(fts_context* ctx, SolverBase* solver, std::vector<Ander
 }
 
 void SaddlePoint::set_fields(const double* w, bool on_device) {
+    field_version_ = ++version_counter_;
     FTS_HIP_CHECK(hipMemcpyAsync(w_aux_.get(), w, sizeof(double) * M_ * S_ * NR_,
                                  on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, ctx_->stream));
     FTS_HIP_CHECK(hipStreamSynchronize(ctx_->stream));
@@ -241,12 +242,15 @@ void SaddlePoint::get_noise(double* noise, bool on_device) {
 }
 
 void SaddlePoint::backup() {
+    backup_field_version_ = field_version_;
     const size_t b = sizeof(double) * M_ * S_ * NR_;
     FTS_HIP_CHECK(hipMemcpyAsync(w_backup_.get(), w_aux_.get(), b, hipMemcpyDeviceToDevice, ctx_->stream));
     FTS_HIP_CHECK(hipMemcpyAsync(phi_backup_.get(), phi_.get(), b, hipMemcpyDeviceToDevice, ctx_->stream));
 }
 
 void SaddlePoint::restore(const int* replicas) {
+    // a partial restore leaves a mix of versions: give it a new one
+    field_version_ = replicas ? ++version_counter_ : backup_field_version_;
     const size_t b = sizeof(double) * M_ * S_;
     for (int r = 0; r < NR_; r++) {
         if (replicas && !replicas[r]) continue;
@@ -469,6 +473,7 @@ void SaddlePoint::find(double tolerance, int max_iter, int verbose, double* log,
 
 void SaddlePoint::langevin_update(const double* noise, bool on_device, double dt) {
     if (R_ == 0) return;
+    field_version_ = ++version_counter_;
     hipStream_t st = ctx_->stream;
     FTS_HIP_CHECK(hipMemcpyAsync(noise_cur_.get(), noise, sizeof(double) * M_ * R_ * NR_,
                                  on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st));
@@ -500,7 +505,18 @@ void SaddlePoint::prepare_guess(Cnn& cnn) {
     FTS_REQUIRE(R_ == 1 && I_ == 1, "the learned guess maps one real field to one imaginary field");
     FTS_REQUIRE(cnn.n_grid() == M_, "the network's grid differs from the simulation grid");
     if (guess_old_.bytes() == 0) guess_old_.allocate(sizeof(double) * M_);
-    cnn.predict(w_aux_.as<double>() + (size_t)real_idx_[0] * M_, guess_old_.as<double>(), true);
+    // f(w_real) is often already there: the previous step's prediction was made from this
+    // same field when that step was accepted, and a restored step keeps the prepared one
+    const GuessTag tag{&cnn, cnn.version(), field_version_};
+    if (!(old_tag_ == tag)) {
+        if (new_tag_ == tag) {
+            std::swap(guess_old_, guess_new_);
+            std::swap(old_tag_, new_tag_);
+        } else {
+            cnn.predict(w_aux_.as<double>() + (size_t)real_idx_[0] * M_, guess_old_.as<double>(), true);
+            old_tag_ = tag;
+        }
+    }
     prepared_ = true;
 }
 
@@ -517,6 +533,7 @@ void SaddlePoint::predict_guess(Cnn& cnn, bool delta) {
     FTS_REQUIRE(prepared_, "delta guess needs prepare_guess before the Langevin update");
     if (guess_new_.bytes() == 0) guess_new_.allocate(sizeof(double) * M_);
     cnn.predict(w_real, guess_new_.as<double>(), true);
+    new_tag_ = GuessTag{&cnn, cnn.version(), field_version_};
     hipLaunchKernelGGL(add_difference_kernel, dim3(grid_blocks(M_)), dim3(256), 0, ctx_->stream, w_imag,
                        guess_new_.as<const double>(), guess_old_.as<const double>(), M_);
     FTS_HIP_CHECK(hipGetLastError());
