@@ -1,6 +1,6 @@
 """Langevin FTS throughput on one MI355X, with the propagator kernels' roofline.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload lfts64|lfts32|replicas48|realspace3d|reduce128]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload lfts64|lfts32|replicas48|realspace3d|reduce128|c1|c2|c2d|c3|c4]
 
 Workload "lfts64" (default): AB diblock melt, discrete chain N=90, f=0.5, chiN=17,
 64^3 periodic grid, L=6.0 R0, nbar=1e4, Langevin step dt=0.8 with PCG64 noise,
@@ -40,7 +40,9 @@ import argparse
 import contextlib
 import csv
 import glob
+import itertools
 import json
+import math
 import os
 import re
 import shutil
@@ -64,6 +66,24 @@ WORKLOADS = {
                        nbar=1e4, dt=0.8, tol=1e-4, am=(20, 5e-1, 0.01, 0.01), max_iter=200, replicas=32),
     # reduce-memory form: propagators in pinned host memory (the 128^3 discrete N=90 precision-form shape)
     "reduce128": dict(nx=[128, 128, 128], lx=[6.0, 6.0, 6.0], model="discrete", ds=1 / 90, f=0.4, reduce=True),
+    # propagator evaluations of the measurement plan's other configurations
+    "c1": dict(kind="eval", nx=[32, 32, 32], lx=[4.0, 4.0, 4.0], model="continuous", ds=0.01, sigma=1.0,
+               bonds={"A": 1.0, "B": 1.0}, polymers=[[1.0, [["A", 0.5, 0, 1], ["B", 0.5, 1, 2]]]],
+               label="C1: AB diblock, continuous, f=0.5, ds=0.01, 32^3, L=4"),
+    "c3": dict(kind="eval", nx=[128, 128, 128], lx=[6.0, 6.0, 6.0], model="continuous", ds=0.01, sigma=2.0,
+               bonds={"A": 1.0, "B": 1.0}, polymers=[[1.0, [["A", 0.4, 0, 1], ["B", 0.6, 1, 2]]]],
+               precisions=["hip-fp32", "hip"],
+               label="C3: AB diblock, continuous, f=0.4, ds=0.01, 128^3, L=6, w ~ N(0, 2^2)"),
+    "c4": dict(kind="eval", nx=[64, 64, 64], lx=[4.0, 4.0, 4.0], model="continuous", ds=0.01, sigma=1.0,
+               bonds={"A": 1.0, "B": 1.0, "C": 1.2},
+               polymers=[[0.8, [["A", 0.3, 0, 1], ["B", 0.3, 0, 2], ["C", 0.4, 0, 3]]], [0.2, [["A", 0.5, 0, 1]]]],
+               label="C4: ABC star (0.3/0.3/0.4, b=1/1/1.2, phi=0.8) + A homopolymer (0.5, phi=0.2), continuous, "
+                     "ds=0.01, 64^3, L=4"),
+    # SCFT iterations (propagators, concentrations, stress, Anderson mixing on [w, L], box relaxation)
+    "c2": dict(kind="scft", nx=[64, 64, 64], lx=[3.3, 3.3, 3.3], model="continuous", ds=0.01, f=0.36, chi_n=20.0,
+               label="C2: SCFT of an AB diblock, continuous, f=0.36, chiN=20, ds=0.01, 64^3, L=3.3, box relaxation"),
+    "c2d": dict(kind="scft", nx=[64, 64, 64], lx=[3.3, 3.3, 3.3], model="discrete", ds=0.01, f=0.36, chi_n=20.0,
+                label="C2: SCFT of an AB diblock, discrete N=100, f=0.36, chiN=20, 64^3, L=3.3, box relaxation"),
     # real-space Crank-Nicolson solver: the reference's 3D nanoparticle example (devel/RealSpace3D.py)
     "realspace3d": dict(nx=[120, 100, 140], lx=[4.0, 5.0, 6.0], model="continuous", ds=0.01, radius=1.0,
                         bc=["absorbing", "absorbing", "reflecting", "reflecting", "reflecting", "reflecting"],
@@ -495,6 +515,181 @@ def reduce_bench(args, w, lf):
     print(json.dumps(out), flush=True)
 
 
+def make_solver(lf, platform, w, reduce=False):
+    factory = lf.PlatformSelector.create_factory(platform, reduce)
+    cb = factory.create_computation_box(w["nx"], w["lx"])
+    mol = factory.create_molecules_information(w["model"], w["ds"], w["bonds"])
+    for vf, blocks in w["polymers"]:
+        mol.add_polymer(vf, blocks)
+    an = factory.create_propagator_analyzer(mol, True)
+    return factory, cb, mol, an, factory.create_pseudospectral_solver(cb, mol, an)
+
+
+def oracle_eval_seconds(w, budget_s, with_stress=True):
+    """FP64 oracle time of one evaluation (propagators, concentrations, stress) of the
+    workload: the same polymers with an up to 10x coarser contour time the per-step cost,
+    which is then scaled by the real number of steps (the step cost does not depend
+    on ds)."""
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from oracle import oracle as O
+    from langevin_fts_amd import langevinfts as lf
+
+    def plan_for(ds):
+        factory = lf.PlatformSelector.create_factory("hip", False)
+        mol = factory.create_molecules_information(w["model"], ds, w["bonds"])
+        for vf, blocks in w["polymers"]:
+            mol.add_polymer(vf, blocks)
+        an = factory.create_propagator_analyzer(mol, True)
+        steps = sum(e.max_n_segment for e in an.get_computation_propagators().values())
+        return plan_dict(mol, an, w["model"]), steps
+
+    _, steps_full = plan_for(w["ds"])
+    # the coarsening keeps every block an integer number of segments
+    seg = [int(round(b[1] / w["ds"])) for _, blocks in w["polymers"] for b in blocks]
+    g = 0
+    for n in seg:
+        g = math.gcd(g, n)
+    k = max(d for d in range(1, 11) if g % d == 0)
+    plan, steps = plan_for(w["ds"] * k)
+    M = int(np.prod(w["nx"]))
+    rng = np.random.default_rng(0)
+    fields = {m: w.get("sigma", 1.0) * rng.standard_normal(M) for m in w["bonds"]}
+    solver = O.Solver(O.Grid(w["nx"], w["lx"]), plan, w["ds"] * k, w["bonds"])
+    n, t0 = 0, time.perf_counter()
+    while True:
+        solver.compute_statistics(fields)
+        if with_stress:
+            solver.compute_stress()
+        n += 1
+        if time.perf_counter() - t0 > budget_s:
+            break
+    t_step = (time.perf_counter() - t0) / (n * steps)
+    return t_step * steps_full, f"{n * steps} oracle steps of the same grid and polymers at {k}x ds ({1e3 * t_step:.1f} ms " \
+                                f"each, FFT lines over {int(os.environ.get('OMP_NUM_THREADS', os.cpu_count() or 1))} " \
+                                f"OpenMP threads) x {steps_full} steps per evaluation"
+
+
+def eval_bench(args, w, lf):
+    """Evaluations/s (compute_propagators + compute_concentrations + compute_stress,
+    fields resident in device memory) of one configuration, per precision form."""
+    from langevin_fts_amd import capi
+
+    M = int(np.prod(w["nx"]))
+    rng = np.random.Generator(np.random.PCG64(args.seed))
+    fields = {m: w["sigma"] * rng.standard_normal(M) for m in sorted(w["bonds"])}
+    rates, kern, steps_per_eval = {}, None, 0
+    with stdout_to_stderr():
+        d_w = {m: capi.DeviceArray.from_host(v) for m, v in fields.items()}
+        for platform in w.get("precisions", ["hip"]):
+            factory, cb, mol, an, solver = make_solver(lf, platform, w)
+            steps_per_eval = sum(e.max_n_segment for e in an.get_computation_propagators().values())
+
+            def step():
+                solver.compute_propagators_device({m: a.ptr for m, a in d_w.items()})
+                solver.compute_concentrations()
+                solver.compute_stress()   # synchronises the stream
+
+            for _ in range(args.warmup):
+                step()
+            first = platform == w.get("precisions", ["hip"])[0]
+            if first and not args.no_kernel_timing:
+                lf.set_kernel_timing(SAMPLE)
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                step()
+            rates[platform] = args.steps / (time.perf_counter() - t0)
+            if first and not args.no_kernel_timing:
+                kern = lf.get_kernel_timing("slab")
+                lf.set_kernel_timing(0)
+            del solver, an, mol, cb, factory
+    main_platform = w.get("precisions", ["hip"])[0]
+    out = {
+        "metric": f"propagator evaluations/s, {w['label']}", "value": rates[main_platform],
+        "unit": "evaluations/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 / rates[main_platform], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 propagators/FFTs, f64 Q/phi/stress" if main_platform == "hip-fp32" else "f64",
+        "data": f"synthetic: fields N(0, {w['sigma']}^2) from PCG64(seed)",
+        "config": {"workload": w["label"] + "; one step = compute_propagators + compute_concentrations + "
+                                            "compute_stress", "steps_per_evaluation": steps_per_eval,
+                   "evaluations_per_s": rates},
+    }
+    if kern and kern["launches"]:
+        achieved = kern["bytes"] / (kern["milliseconds"] * 1e-3) / 1e9
+        out["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBPS, "unit": "GB/s",
+                           "frac": achieved / PEAK_HBM_GBPS, "traffic": None,
+                           "kernel": f"slab pass ({main_platform})", "timed_launch_count": kern["launches"],
+                           "avg_launch_us": 1e3 * kern["milliseconds"] / kern["launches"],
+                           "bytes_per_launch": kern["bytes"] / kern["launches"], "timed_launches": f"1 in {SAMPLE}"}
+    if args.cpu_seconds > 0:
+        with stdout_to_stderr():
+            t, sample = oracle_eval_seconds(w, args.cpu_seconds)
+        out["cpu_baseline"] = {"value": 1.0 / t, "unit": "evaluations/s",
+                               "cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)), "kind": "port",
+                               "sample": sample}
+    print(json.dumps(out), flush=True)
+
+
+def c2_fields(w, seed):
+    """w_A = -w_B = chiN/2 sum_{j=1..8} cos(k_j . r + phase_j), k_j among the 24 integer
+    vectors with |n|^2 = 6 (times 2 pi / L), phases uniform, both from PCG64(seed)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    vecs = sorted({p for v in itertools.permutations((1, 1, 2)) for p in itertools.product(
+        *[(a, -a) for a in v])})
+    pick = rng.choice(len(vecs), size=8, replace=False)
+    phases = rng.uniform(0.0, 2 * np.pi, 8)
+    axes = [np.arange(n) * l / n for n, l in zip(w["nx"], w["lx"])]
+    x, y, z = np.meshgrid(*axes, indexing="ij")
+    field = np.zeros(w["nx"])
+    for j, ph in zip(pick, phases):
+        k = 2 * np.pi * np.array(vecs[j]) / np.array(w["lx"])
+        field += np.cos(k[0] * x + k[1] * y + k[2] * z + ph)
+    field *= w["chi_n"] / 2
+    return {"A": field, "B": -field}
+
+
+def scft_bench(args, w, lf):
+    """SCFT iterations/s with box relaxation (the driver of examples/scft/scft.py)."""
+    from langevin_fts_amd import scft
+
+    params = {"nx": w["nx"], "lx": w["lx"], "box_is_altering": True, "chain_model": w["model"], "ds": w["ds"],
+              "segment_lengths": {"A": 1.0, "B": 1.0}, "chi_n": {"A,B": w["chi_n"]},
+              "distinct_polymers": [{"volume_fraction": 1.0, "blocks": [{"type": "A", "length": w["f"]},
+                                                                        {"type": "B", "length": 1 - w["f"]}]}],
+              "optimizer": {"name": "am", "max_hist": 20, "start_error": 1e-2, "mix_min": 0.01, "mix_init": 0.1},
+              "max_iter": args.warmup + args.steps, "tolerance": 1e-12}
+    fields = c2_fields(w, args.seed)
+    with stdout_to_stderr():
+        calc = scft.SCFT(params=params)
+        # two runs from the same start, W and W + K iterations: their difference is K iterations
+        elapsed = []
+        for n in (args.warmup, args.warmup + args.steps):
+            calc.max_iter = n
+            t0 = time.perf_counter()
+            calc.run(initial_fields=fields)
+            elapsed.append(time.perf_counter() - t0)
+    rate = args.steps / (elapsed[1] - elapsed[0])
+    out = {
+        "metric": f"SCFT iterations/s, {w['label']}", "value": rate, "unit": "iterations/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 / rate, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic: w_A = -w_B = chiN/2 sum of 8 cosines with |n|^2 = 6 wave vectors, PCG64(seed) "
+                "picks and phases",
+        "config": {"workload": w["label"] + "; AM(20, 1e-2, 0.01, 0.1) on [w, L]",
+                   "timing": f"run of {args.warmup + args.steps} iterations minus run of {args.warmup} from the same "
+                             "start"},
+    }
+    if args.cpu_seconds > 0:
+        wl = dict(w, bonds={"A": 1.0, "B": 1.0}, polymers=[[1.0, [["A", w["f"], 0, 1], ["B", 1 - w["f"], 1, 2]]]],
+                  sigma=1.0)
+        with stdout_to_stderr():
+            tt, sample = oracle_eval_seconds(wl, args.cpu_seconds)
+        out["cpu_baseline"] = {"value": 1.0 / tt, "unit": "iterations/s",
+                               "cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)), "kind": "port",
+                               "sample": sample + " (one SCFT iteration = one evaluation; the host-side mixing of "
+                                                  "[w, L] is not counted)"}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -538,6 +733,12 @@ def main():
         return
     if w.get("realspace"):
         realspace_bench(args, w, lf)
+        return
+    if w.get("kind") == "eval":
+        eval_bench(args, w, lf)
+        return
+    if w.get("kind") == "scft":
+        scft_bench(args, w, lf)
         return
     if w.get("reduce"):
         reduce_bench(args, w, lf)
