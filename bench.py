@@ -293,8 +293,11 @@ def replica_bench(args, w, lf, lfts):
     value = replica Langevin steps per second over all replicas."""
     R = w["replicas"]
     M = int(np.prod(w["nx"]))
+    params = lfts_params(w, "/tmp/lfts_bench")
+    mixed = args.precision == "mixed"
+    params["saddle"]["mixed_precision"] = mixed
     with stdout_to_stderr():
-        sim = lfts.LFTS(lfts_params(w, "/tmp/lfts_bench"), random_seed=args.seed, n_replicas=R)
+        sim = lfts.LFTS(params, random_seed=args.seed, n_replicas=R)
         rng = np.random.Generator(np.random.PCG64(args.seed))
         w_aux = np.stack([sim.mpt.to_aux_fields(rng.standard_normal((2, M))) for _ in range(R)])
         dev = sim.saddle_device
@@ -322,7 +325,8 @@ def replica_bench(args, w, lf, lfts):
         "metric": f"L-FTS replica steps/s @{w['nx'][0]}³ ({R} replicas batched)",
         "value": R * args.steps / elapsed, "unit": "replica steps/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": 1e3 * elapsed / args.steps, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 propagators until err < tol, then f64; f64 Q/phi/AM" if mixed else "f64",
         "data": "synthetic: w_A, w_B ~ N(0,1) from PCG64(seed); replica r's noise from PCG64(seed + r)",
         "config": {"workload": describe(args.workload, w), "replicas": R,
                    "saddle_iterations_per_step": float(np.mean(its)), "saddle_failures": fails,

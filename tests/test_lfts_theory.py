@@ -78,3 +78,25 @@ def test_abc_general_func_deriv():
 
 def test_sigma():
     assert abs(calculate_sigma(1e4, 0.8, 64 ** 3, 6.0 ** 3) - np.sqrt(2 * 0.8 * 64 ** 3 / (216.0 * 100.0))) < 1e-12
+
+
+def test_noise_stream_keeps_the_draw_order():
+    """Noise drawn one step ahead equals direct draws of the same generator, and a
+    rewind puts the generator back to the state the next step draws from."""
+    from langevin_fts_amd.lfts import NoiseStream
+
+    direct = np.random.Generator(np.random.PCG64(123))
+    expected = [direct.normal(0.0, 0.3, [1, 1000]) for _ in range(6)]
+    g = np.random.Generator(np.random.PCG64(123))
+    stream = NoiseStream(g, 0.3, [1, 1000])
+    got = [stream.next() for _ in range(3)]
+    for a, b in zip(got, expected[:3]):
+        np.testing.assert_array_equal(a, b)
+    stream.rewind()   # one draw was pending: the state must be that after 3 draws
+    check = np.random.Generator(np.random.PCG64(123))
+    for _ in range(3):
+        check.normal(0.0, 0.3, [1, 1000])
+    assert g.bit_generator.state == check.bit_generator.state
+    got += [stream.next() for _ in range(3)]
+    for a, b in zip(got, expected):
+        np.testing.assert_array_equal(a, b)
