@@ -8,6 +8,7 @@
 // With boundary conditions given, continuous chains take real-space
 // Crank-Nicolson steps instead (CpuComputationContinuous.cpp:25-26).
 #include <algorithm>
+#include <cstdlib>
 #include <cmath>
 #include <map>
 
@@ -114,6 +115,7 @@ private:
     std::vector<XBatch<T>> xs_;
     // real-space Crank-Nicolson form
     bool real_ = false;
+    bool real_plain_ = false;   // env FTS_REAL_PLAIN_SWEEPS=1: one thread per line on every axis
     std::vector<int> bc_;
     RealGeom rg_{};
     DeviceBuffer rtab_;
@@ -133,6 +135,7 @@ Solver<T>::Solver(Grid* grid, const fts_plan* pl, const int* bc, bool reduce_mem
         FTS_REQUIRE(model_ == FTS_CONTINUOUS, "Real-space method only support 'continuous' chain model.");
         FTS_REQUIRE((std::is_same<T, double>::value), "the real-space method runs in FP64");
         real_ = true;
+        real_plain_ = std::getenv("FTS_REAL_PLAIN_SWEEPS") != nullptr;
         for (int i = 0; i < 2 * g_->dim; i++) {
             FTS_REQUIRE(bc[i] >= BC_PERIODIC && bc[i] <= BC_ABSORBING, "invalid boundary condition code");
             bc_.push_back(bc[i]);
@@ -400,7 +403,7 @@ void Solver<T>::launch_real(const RealBatch& b) {
     for (int a = 0; a < rg_.D; a++) {
         const RealAxis& A = rg_.ax[a];
         const bool first = a == 0, last = a == rg_.D - 1;
-        if (A.n <= 64 * RT_LMAX) {
+        if (A.n <= 64 * RT_LMAX && !real_plain_) {
             RealTile t{};
             t.P = 1;
             while (t.P * RT_LMAX < A.n) t.P <<= 1;

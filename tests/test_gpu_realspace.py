@@ -182,3 +182,37 @@ def test_realspace_grid_beyond_fft_sizes(lf):
     ref.compute_statistics(w)
     assert abs(solver.get_total_partition(0) / ref.get_total_partition(0) - 1) < 1e-10
     assert rel(solver.get_total_concentration("A"), ref.get_total_concentration("A")) < 1e-10
+
+
+def test_plain_and_tiled_sweeps_agree():
+    """the one-thread-per-line sweeps (lines above 1024 points, forced here by
+    FTS_REAL_PLAIN_SWEEPS) and the tiled sweeps give the same propagators"""
+    import os
+
+    from langevin_fts_amd import capi
+
+    say("realspace plain vs tiled sweeps")
+    plan = golden("plans")["grafted|continuous|0"]
+    inp = plan["input"]
+    nx, lx = [20, 18, 23], [2.0, 1.8, 2.3]
+    bc = ["absorbing", "reflecting", "periodic", "periodic", "reflecting", "absorbing"]
+    M = int(np.prod(nx))
+    rng = np.random.default_rng(4)
+    w = {"A": 0.5 * rng.standard_normal(M)}
+    q_init = {"G": 0.5 + rng.random(M)}
+    ctx = capi.Context(0)
+    out = {}
+    for plain in (False, True):
+        if plain:
+            os.environ["FTS_REAL_PLAIN_SWEEPS"] = "1"
+        try:
+            g = capi.Grid(ctx, nx, lx)
+            P = capi.Plan(plan, inp["ds"], inp["bonds"], plan["schedules"]["4"])
+            s = capi.Solver(g, P, capi.FTS_FP64, realspace_bc=bc)
+        finally:
+            os.environ.pop("FTS_REAL_PLAIN_SWEEPS", None)
+        s.compute_statistics(w, q_init)
+        out[plain] = (s.get_total_partition(0), s.propagator("{G}A", 100), s.get_total_concentration("A"))
+    assert abs(out[True][0] / out[False][0] - 1) < 1e-12
+    assert rel(out[True][1], out[False][1]) < 1e-12
+    assert rel(out[True][2], out[False][2]) < 1e-12
