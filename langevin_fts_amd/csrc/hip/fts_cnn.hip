@@ -47,9 +47,32 @@ struct Geo {
     int nx, ny, nz, M;
 };
 
-__device__ __forceinline__ int neighbour(const Geo& g, int v, int t) {
-    const int z = v % g.nz, y = (v / g.nz) % g.ny, x = v / (g.nz * g.ny);
-    int xx = x + t / 9 - 1, yy = y + (t / 3) % 3 - 1, zz = z + t % 3 - 1;
+// a voxel's coordinates: computed once per voxel, so the 27 taps need no division
+struct Vox {
+    int x, y, z;
+};
+
+__device__ __forceinline__ Vox coords(const Geo& g, int v) {
+    const int r = v / g.nz;
+    return Vox{r / g.ny, r % g.ny, v % g.nz};
+}
+
+// the voxel k further along the flat index (k >= 0), by carries
+__device__ __forceinline__ Vox advance(const Geo& g, Vox c, int k) {
+    c.z += k;
+    while (c.z >= g.nz) {
+        c.z -= g.nz;
+        if (++c.y == g.ny) {
+            c.y = 0;
+            ++c.x;
+        }
+    }
+    return c;
+}
+
+// periodic neighbour of the voxel at c through tap t
+__device__ __forceinline__ int neighbour(const Geo& g, Vox c, int t) {
+    int xx = c.x + t / 9 - 1, yy = c.y + (t / 3) % 3 - 1, zz = c.z + t % 3 - 1;
     xx = xx < 0 ? xx + g.nx : (xx >= g.nx ? xx - g.nx : xx);
     yy = yy < 0 ? yy + g.ny : (yy >= g.ny ? yy - g.ny : yy);
     zz = zz < 0 ? zz + g.nz : (zz >= g.nz ? zz - g.nz : zz);
@@ -63,8 +86,9 @@ __global__ void __launch_bounds__(256) conv_first(Geo g, const float* x, const f
     float acc[C];
 #pragma unroll
     for (int c = 0; c < C; c++) acc[c] = b[c];
+    const Vox cv = coords(g, v);
     for (int t = 0; t < TAPS; t++) {
-        const float xv = x[neighbour(g, v, t)];
+        const float xv = x[neighbour(g, cv, t)];
 #pragma unroll
         for (int c = 0; c < C; c++) acc[c] += W[t * C + c] * xv;
     }
@@ -87,9 +111,10 @@ __global__ void __launch_bounds__(256) conv_hidden(Geo g, const __bf16* X, const
     if (v0 >= g.M) return;
     const int half = lane >> 5, col = lane & 31;
     const int vi = v0 + col;   // voxel of this lane's A rows
+    const Vox ci = coords(g, vi);
     f32x16 acc = {};
     for (int t = 0; t < TAPS; t++) {
-        const __bf16* xa = X + (size_t)neighbour(g, vi, t) * C + half * 8;
+        const __bf16* xa = X + (size_t)neighbour(g, ci, t) * C + half * 8;
         const __bf16* wb = W + ((size_t)t * C + col) * C + half * 8;
 #pragma unroll
         for (int ch = 0; ch < 2; ch++) {
@@ -114,8 +139,9 @@ __global__ void __launch_bounds__(256) conv_last(Geo g, const __bf16* X, const f
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= g.M) return;
     float acc = b[0];
+    const Vox cv = coords(g, v);
     for (int t = 0; t < TAPS; t++) {
-        const bf16x8* xv = reinterpret_cast<const bf16x8*>(X + (size_t)neighbour(g, v, t) * C);
+        const bf16x8* xv = reinterpret_cast<const bf16x8*>(X + (size_t)neighbour(g, cv, t) * C);
 #pragma unroll
         for (int q = 0; q < C / 8; q++) {
             const bf16x8 h = xv[q];
@@ -155,8 +181,9 @@ __global__ void __launch_bounds__(256) back_last(Geo g, const float* dy, const f
     const int u = blockIdx.x * blockDim.x + threadIdx.x;
     if (u >= g.M) return;
     float acc[C] = {};
+    const Vox cu = coords(g, u);
     for (int t = 0; t < TAPS; t++) {
-        const float d = dy[neighbour(g, u, 26 - t)];
+        const float d = dy[neighbour(g, cu, 26 - t)];
 #pragma unroll
         for (int c = 0; c < C; c++) acc[c] += W[t * C + c] * d;
     }
@@ -184,6 +211,8 @@ __global__ void __launch_bounds__(256) wgrad_hidden(Geo g, const __bf16* dZ, con
     const int vbeg = chunk * WG_CHUNK + wave * (WG_CHUNK / 4);
     const int vend = min(vbeg + WG_CHUNK / 4, g.M);
     f32x16 acc = {};
+    // coordinates of this lane's first voxel, carried along the loop
+    Vox cb = coords(g, min(vbeg + half * 8, g.M - 1));
     for (int v0 = vbeg; v0 < vend; v0 += 16) {
         bf16x8 a, b;
 #pragma unroll
@@ -191,9 +220,10 @@ __global__ void __launch_bounds__(256) wgrad_hidden(Geo g, const __bf16* dZ, con
             const int v = v0 + half * 8 + e;
             const bool ok = v < vend;
             a[e] = ok ? dZ[(size_t)v * C + col] : (__bf16)0.f;
-            b[e] = ok ? X[(size_t)neighbour(g, v, t) * C + col] : (__bf16)0.f;
+            b[e] = ok ? X[(size_t)neighbour(g, advance(g, cb, e), t) * C + col] : (__bf16)0.f;
         }
         acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+        if (v0 + 16 < vend) cb = advance(g, cb, 16);
     }
     // D[i = co][j = ci]: j = lane % 32, i = 8 (r / 4) + 4 (lane / 32) + r % 4
 #pragma unroll
@@ -204,18 +234,37 @@ __global__ void __launch_bounds__(256) wgrad_hidden(Geo g, const __bf16* dZ, con
 }
 
 // First layer: G[t][c] = sum_v dZ[v][c] x[nbr(v,t)]; last layer: G[t][c] = sum_v dy[v] X[nbr(v,t)][c].
-// Thread (t, c) of the block sums the chunk's voxels; partial[chunk][t][c].
-__global__ void __launch_bounds__(TAPS * C) wgrad_edge(Geo g, int first, const __bf16* H, const float* s,
-                                                       float* partial) {
-    const int t = threadIdx.x / C, c = threadIdx.x % C;
+// Thread (group, c) of the block takes every 8th voxel of the chunk and all 27 taps; the 8 groups
+// meet in LDS in a fixed order; partial[chunk][t][c].
+__global__ void __launch_bounds__(256) wgrad_edge(Geo g, int first, const __bf16* H, const float* s,
+                                                  float* partial) {
+    __shared__ float red[8][TAPS * C];
+    const int c = threadIdx.x % C, grp = threadIdx.x / C;
     const int vbeg = blockIdx.x * WG_CHUNK, vend = min(vbeg + WG_CHUNK, g.M);
-    float acc = 0.f;
-    if (first) {
-        for (int v = vbeg; v < vend; v++) acc += (float)H[(size_t)v * C + c] * s[neighbour(g, v, t)];
-    } else {
-        for (int v = vbeg; v < vend; v++) acc += s[v] * (float)H[(size_t)neighbour(g, v, t) * C + c];
+    float acc[TAPS] = {};
+    if (vbeg + grp < vend) {
+        Vox cv = coords(g, vbeg + grp);
+        for (int v = vbeg + grp; v < vend; v += 8) {
+            if (first) {
+                const float h = (float)H[(size_t)v * C + c];
+#pragma unroll
+                for (int t = 0; t < TAPS; t++) acc[t] += h * s[neighbour(g, cv, t)];
+            } else {
+                const float sv = s[v];
+#pragma unroll
+                for (int t = 0; t < TAPS; t++) acc[t] += sv * (float)H[(size_t)neighbour(g, cv, t) * C + c];
+            }
+            if (v + 8 < vend) cv = advance(g, cv, 8);
+        }
     }
-    partial[(size_t)blockIdx.x * TAPS * C + threadIdx.x] = acc;
+#pragma unroll
+    for (int t = 0; t < TAPS; t++) red[grp][t * C + c] = acc[t];
+    __syncthreads();
+    for (int i = threadIdx.x; i < TAPS * C; i += blockDim.x) {
+        float sum = 0.f;
+        for (int k = 0; k < 8; k++) sum += red[k][i];
+        partial[(size_t)blockIdx.x * TAPS * C + i] = sum;
+    }
 }
 
 // Bias gradients: channel sums of dZ (C channels) or of dy (1 channel); partial[chunk][c]
@@ -225,8 +274,10 @@ __global__ void __launch_bounds__(256) bias_grad(Geo g, const __bf16* dZ, const 
     const int c = threadIdx.x % C, lane_group = threadIdx.x / C;   // 8 groups of 32
     float acc = 0.f;
     if (dZ) {
+#pragma unroll 8
         for (int v = vbeg + lane_group; v < vend; v += 8) acc += (float)dZ[(size_t)v * C + c];
     } else if (c == 0) {
+#pragma unroll 8
         for (int v = vbeg + lane_group; v < vend; v += 8) acc += dy[v];
     }
     red[threadIdx.x] = acc;
@@ -244,6 +295,8 @@ __global__ void __launch_bounds__(256) reduce_chunks(const float* partial, int n
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     float s = 0.f;
+    // unrolled so that several chunks' loads are in flight; the sum keeps its order
+#pragma unroll 8
     for (int k = 0; k < n_chunks; k++) s += partial[(size_t)k * stride + i];
     grad[i] = s;
 }
@@ -410,7 +463,7 @@ double Cnn::gradients(const double* w_minus, const double* w_plus_target, bool o
     __bf16* a = acts_.as<__bf16>();
     const size_t AS = (size_t)M_ * C;
     // last layer: weights, bias, then dZ of the last hidden activation
-    hipLaunchKernelGGL(wgrad_edge, dim3(n_chunks_), dim3(TAPS * C), 0, st, g, 0, a + NHID * AS, dy_.as<float>(), part);
+    hipLaunchKernelGGL(wgrad_edge, dim3(n_chunks_), dim3(256), 0, st, g, 0, a + NHID * AS, dy_.as<float>(), part);
     hipLaunchKernelGGL(reduce_chunks, dim3(blocks(TAPS * C)), dim3(256), 0, st, part, n_chunks_, TAPS * C, TAPS * C,
                        gr + P_WL);
     hipLaunchKernelGGL(bias_grad, dim3(n_chunks_), dim3(256), 0, st, g, (const __bf16*)nullptr, dy_.as<float>(), part);
@@ -432,7 +485,7 @@ double Cnn::gradients(const double* w_minus, const double* w_plus_target, bool o
         std::swap(dz, other);
     }
     // first layer
-    hipLaunchKernelGGL(wgrad_edge, dim3(n_chunks_), dim3(TAPS * C), 0, st, g, 1, dz, x_.as<float>(), part);
+    hipLaunchKernelGGL(wgrad_edge, dim3(n_chunks_), dim3(256), 0, st, g, 1, dz, x_.as<float>(), part);
     hipLaunchKernelGGL(reduce_chunks, dim3(blocks(TAPS * C)), dim3(256), 0, st, part, n_chunks_, TAPS * C, TAPS * C,
                        gr + P_W0);
     hipLaunchKernelGGL(bias_grad, dim3(n_chunks_), dim3(256), 0, st, g, dz, (const float*)nullptr, part);
