@@ -710,6 +710,7 @@ def main():
     ap.add_argument("--guess", default="learned", choices=["learned", "plain"],
                     help="learned: after the plain arm, train the CNN guess and time the steps that use it")
     ap.add_argument("--guess-train-steps", type=int, default=1000, help="Langevin steps the CNN trains on")
+    ap.add_argument("--guess-train-iters", type=int, default=4, help="Adam iterations per training step")
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"],
                     help="mixed: saddle iterations on FP32 propagators until the error < tol, then FP64")
     args = ap.parse_args()
@@ -750,7 +751,8 @@ def main():
     params = lfts_params(w, "/tmp/lfts_bench")
     mixed = args.precision == "mixed"
     params["saddle"]["mixed_precision"] = mixed
-    params["learned_guess"] = {"train_steps": args.guess_train_steps, "lr": 1e-3, "train_iters": 4, "replay": 64,
+    params["learned_guess"] = {"train_steps": args.guess_train_steps, "lr": 1e-3, "train_iters": args.guess_train_iters,
+                               "replay": 64,
                                "delta": True, "seed": 0}
     with stdout_to_stderr():
         sim = lfts.LFTS(params, random_seed=args.seed)

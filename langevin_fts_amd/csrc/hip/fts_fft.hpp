@@ -303,7 +303,7 @@ struct SlabItem {
     T* qout;           // real output of the inverse part (nullptr: no write)
     typename vec2<T>::type* c0;
     typename vec2<T>::type* c1;
-    const T* w;        // field for exp(-w*coef) factors (nullptr: all factors 1)
+    const T* w;        // field for exp(-w*coef) factors (nullptr: all factors 1; coef < 0: w is the factor)
     const T* mask;     // nullptr: no mask
     T post0, post1;    // inverse part: r_o *= exp(-w*post_o) / M
     T pre0, pre1;      // forward part: spectrum_o = FFT[exp(-w*pre_o) q]
@@ -441,7 +441,8 @@ __device__ __forceinline__ void store_spectrum(const T2* S, const SlabGeom& g, T
 
 template <typename T>
 __device__ __forceinline__ T factor(T w, bool has_w, T coef) {
-    return (has_w && coef != (T)0) ? fexp<T>(-w * coef) : (T)1;
+    // coef < 0: the array already holds the factor exp(-w ds) (a table made once per call)
+    return (has_w && coef != (T)0) ? (coef < (T)0 ? w : fexp<T>(-w * coef)) : (T)1;
 }
 
 // write this thread's real values times exp(-w*coef) into the packed rows

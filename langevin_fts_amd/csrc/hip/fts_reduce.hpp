@@ -183,7 +183,7 @@ __global__ void init_kernel(InitBatch<T> b, int M) {
             v = it.accumulate ? it.dst[i] : (T)1;
             for (int s = 0; s < it.n_src; s++) v *= it.src[s][i];
         }
-        if (it.w) v *= fexp<T>(-it.w[i] * it.wcoef);
+        if (it.w) v *= it.wcoef < (T)0 ? it.w[i] : fexp<T>(-it.w[i] * it.wcoef);
         if (it.mask) v *= it.mask[i];
         it.dst[i] = v;
     }
@@ -192,6 +192,12 @@ __global__ void init_kernel(InitBatch<T> b, int M) {
 template <typename T>
 __global__ void convert_kernel(const double* in, T* out, int M) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < M; i += gridDim.x * blockDim.x) out[i] = (T)in[i];
+}
+
+// out = exp(-w coef), the factor of every discrete step (w fixed during a call)
+template <typename T>
+__global__ void exp_table_kernel(const T* w, T coef, T* out, int n) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) out[i] = fexp<T>(-w[i] * coef);
 }
 
 // FP64 fields (one per blockIdx.y) into consecutive fields of the solver's precision

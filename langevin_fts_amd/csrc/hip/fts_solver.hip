@@ -41,7 +41,7 @@ public:
     void propagator(int prop, int n, int half, double* out, bool on_device) override;
     void memory_usage(size_t* device, size_t* host) const override {
         size_t d = 0, h = 0;
-        for (const DeviceBuffer* b : {&qstore_, &hstore_, &w64_, &w32_, &qinit_, &ws_, &phib_, &phis_, &Qd_,
+        for (const DeviceBuffer* b : {&qstore_, &hstore_, &w64_, &w32_, &ew_, &qinit_, &ws_, &phib_, &phis_, &Qd_,
                                       &stress_out_, &rtab_})
             (b->pinned_host() ? h : d) += b->bytes();
         *device = d;
@@ -75,6 +75,11 @@ private:
     }
     T* slice_ptr(int p, int n, int half) const { return half ? h(p, n) : q(p, n); }
     const T* w(int m) const { return wT_ + (size_t)m * M_; }
+    // discrete FP64 chains: exp(-w ds) of monomer m, made once per compute_propagators (the FP32
+    // kernels compute their single-precision exponential on the fly, which costs less than the table)
+    bool table() const { return model_ == FTS_DISCRETE && std::is_same<T, double>::value; }
+    const T* ew(int m) const { return table() ? ew_.as<T>() + (size_t)m * M_ : w(m); }
+    T ew_coef() const { return table() ? (T)-1 : (T)ds_; }   // negative: the array is the factor
     T2* slot(int s) const { return ws_.as<T2>() + (size_t)s * Mc_; }
     const T* mask() const { return g_->mask_h.empty() ? nullptr : g_->mask<T>(); }
     int full(int m) const { return 2 * m; }
@@ -102,7 +107,7 @@ private:
     std::vector<int> smon_;
     std::vector<std::vector<std::array<int, 3>>> slices_;
 
-    DeviceBuffer qstore_, hstore_, w64_, w32_, qinit_, ws_, phib_, phis_, Qd_, stress_out_;
+    DeviceBuffer qstore_, hstore_, w64_, w32_, ew_, qinit_, ws_, phib_, phis_, Qd_, stress_out_;
     T* wT_ = nullptr;
     std::vector<size_t> qoff_;
     std::map<std::pair<int, int>, size_t> hoff_;
@@ -222,6 +227,7 @@ Solver<T>::Solver(Grid* grid, const fts_plan* pl, const int* bc, bool reduce_mem
         w32_.allocate(sizeof(float) * n_mon_ * (size_t)M_);
         wT_ = reinterpret_cast<T*>(w32_.as<float>());
     }
+    if (table()) ew_.allocate(sizeof(T) * n_mon_ * (size_t)M_);
     if (n_qinit_) qinit_.allocate(sizeof(double) * n_qinit_ * (size_t)M_);
     n_ws_ = std::max<int>(2 * (int)jmax, 16);
     ws_.allocate(sizeof(T2) * (size_t)n_ws_ * Mc_);
@@ -505,8 +511,8 @@ void Solver<T>::build_slice(const std::vector<std::array<int, 3>>& jobs) {
                 it.dst = q(p, 1);
                 it.op = pr.kind == FTS_LEAF ? INIT_ONE : INIT_QINIT;
                 if (pr.kind == FTS_LEAF_QINIT) it.qinit = qinit_.as<double>() + (size_t)pr.qinit * M_;
-                it.w = w(m);
-                it.wcoef = ds;
+                it.w = ew(m);
+                it.wcoef = ew_coef();
                 it.mask = mask();
                 add_init(0, it);
             } else {
@@ -546,8 +552,8 @@ void Solver<T>::build_slice(const std::vector<std::array<int, 3>>& jobs) {
             xi.push_back({slot(f.slot), f.table});
             SlabItem<T> b{};
             b.c0 = slot(f.slot);
-            b.w = w(props_[f.p].monomer);
-            b.post0 = ds;
+            b.w = ew(props_[f.p].monomer);
+            b.post0 = ew_coef();
             b.mask = mask();
             b.qout = q(f.p, 1);
             inv.push_back(b);
@@ -643,8 +649,8 @@ void Solver<T>::build_slice(const std::vector<std::array<int, 3>>& jobs) {
                 x1.push_back({c0, full(m)});
                 SlabItem<T> b{};
                 b.c0 = c0;
-                b.w = w(m);
-                b.post0 = ds;
+                b.w = ew(m);
+                b.post0 = ew_coef();
                 b.mask = mask();
                 b.qout = q(r.p, n + 1);
                 b.forward = last ? 0 : 1;
@@ -803,6 +809,11 @@ void Solver<T>::compute_propagators(const double* const* wf, const double* const
         }
     }
     if (!std::is_same<T, double>::value) FTS_HIP_CHECK(hipGetLastError());
+    if (table()) {
+        hipLaunchKernelGGL((exp_table_kernel<T>), dim3(std::min(1024, ceil_div(n_mon_ * M_, 256))), dim3(256), 0,
+                           g_->stream(), wT_, (T)ds_, ew_.as<T>(), n_mon_ * M_);
+        FTS_HIP_CHECK(hipGetLastError());
+    }
     for (int i = 0; i < n_qinit_; i++) {
         FTS_REQUIRE(q_init && q_init[i], "missing q_init field");
         FTS_HIP_CHECK(hipMemcpyAsync(qinit_.as<double>() + (size_t)i * M_, q_init[i], sizeof(double) * M_, kind,
