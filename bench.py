@@ -710,7 +710,7 @@ def main():
     ap.add_argument("--guess", default="learned", choices=["learned", "plain"],
                     help="learned: after the plain arm, train the CNN guess and time the steps that use it")
     ap.add_argument("--guess-train-steps", type=int, default=1000, help="Langevin steps the CNN trains on")
-    ap.add_argument("--guess-train-iters", type=int, default=4, help="Adam iterations per training step")
+    ap.add_argument("--guess-train-iters", type=int, default=16, help="Adam iterations per training step")
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"],
                     help="mixed: saddle iterations on FP32 propagators until the error < tol, then FP64")
     args = ap.parse_args()

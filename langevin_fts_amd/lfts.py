@@ -257,7 +257,7 @@ class NoiseStream:
 
 class LFTS:
     """L-FTS driver; `params` has the keys of the reference examples (fts/Lamella.py).
-    An optional "learned_guess" entry {"train_steps", "lr", "train_iters", "seed"}
+    An optional "learned_guess" entry {"train_steps", "lr", "lr_final", "train_iters", "seed"}
     trains a periodic 3D CNN on the (w-, w+) saddle points of the first train_steps
     accepted Langevin steps and then starts every saddle point from its prediction."""
 
@@ -418,7 +418,7 @@ class LFTS:
         self.langevin.update({"sigma": langevin_sigma})
         self.verbose_level = params["verbose_level"]
         # learned initial guess of w+ (optional):
-        # {"train_steps", "lr", "train_iters", "replay", "delta", "seed"}
+        # {"train_steps", "lr", "lr_final", "train_iters", "replay", "delta", "seed"}
         self.learned = dict(params["learned_guess"]) if params.get("learned_guess") else None
         if self.learned is not None:
             assert len(self.mpt.aux_fields_real_idx) == 1 and len(self.mpt.aux_fields_imag_idx) == 1, \
@@ -426,8 +426,9 @@ class LFTS:
             assert cb.get_dim() == 3, "the learned guess needs a 3D grid"
             self.learned.setdefault("train_steps", 1000)
             self.learned.setdefault("lr", 1e-3)
-            self.learned.setdefault("train_iters", 4)
-            self.learned.setdefault("replay", 32)
+            self.learned.setdefault("lr_final", self.learned["lr"] / 10)
+            self.learned.setdefault("train_iters", 16)
+            self.learned.setdefault("replay", 64)
             self.learned.setdefault("delta", True)
             self.learned.setdefault("seed", 0)
         self.guess = None
@@ -487,10 +488,14 @@ class LFTS:
     def train_guess(self):
         """Adam steps on the current (w-, w+*) pair and on pairs replayed from a device
         ring of recent saddle points; the network and its scales are created at the
-        first call. Returns the newest pair's loss."""
+        first call. The learning rate falls from "lr" to "lr_final" on a cosine over
+        the "train_steps" training steps. Returns the newest pair's loss."""
         if self.guess is None:
             self.init_guess()
-        loss = self.saddle_device.train_guess(self.guess, self.learned["lr"], self.learned["train_iters"],
+        lr0, lr1 = self.learned["lr"], self.learned["lr_final"]
+        frac = min(self.guess_trained_steps / max(self.learned["train_steps"], 1), 1.0)
+        lr = lr1 + 0.5 * (lr0 - lr1) * (1.0 + np.cos(np.pi * frac))
+        loss = self.saddle_device.train_guess(self.guess, lr, self.learned["train_iters"],
                                               self.learned["replay"])
         self.guess_trained_steps += 1
         return loss
