@@ -78,10 +78,19 @@ struct WRegs {
     using type = T2[WStageMap<L, REV, S, NLW, LF>::NT][WPlan<L, REV>::r(S)];
 };
 
+// Position of element i of a line in the exchange between two stages: with SK one pad
+// slot after every 8 elements, so that the stride-R stores of a stage's butterflies
+// (positions R j + r) fall on different LDS banks; the line then spans L + (L-1)/8 slots.
+template <bool SK>
+__device__ __forceinline__ int wl_slot(int i) {
+    return SK ? i + (i >> 3) : i;
+}
+
 // Stages S.. of the plan over this wavefront's lines (line l at X + l LP, element i
-// at stride ES). v holds stage S's inputs; `done` receives the last stage's outputs
-// in registers (positions j + r L/R_last).
-template <int DIR, typename T, typename T2, int L, bool REV, int NLW, bool LF, int LP, int ES, int S, typename F>
+// at stride ES; skewed exchange positions with SK). v holds stage S's inputs; `done`
+// receives the last stage's outputs in registers (positions j + r L/R_last).
+template <int DIR, typename T, typename T2, int L, bool REV, int NLW, bool LF, int LP, int ES, bool SK, int S,
+          typename F>
 __device__ __forceinline__ void wl_stages(typename WRegs<T2, L, REV, S, NLW, LF>::type& v, T2* X, const T2* W,
                                           int lane, int nvalid, F&& done) {
     using PL = WPlan<L, REV>;
@@ -113,7 +122,7 @@ __device__ __forceinline__ void wl_stages(typename WRegs<T2, L, REV, S, NLW, LF>
                 const int d = (j / NS) * NS * R + j % NS;
                 T2* base = X + M::line(g) * LP;
 #pragma unroll
-                for (int r = 0; r < R; r++) base[(d + r * NS) * ES] = v[t][r];
+                for (int r = 0; r < R; r++) base[wl_slot<SK>(d + r * NS) * ES] = v[t][r];
             }
         }
         wave_sync();
@@ -127,10 +136,10 @@ __device__ __forceinline__ void wl_stages(typename WRegs<T2, L, REV, S, NLW, LF>
                 const T2* base = X + M2::line(g) * LP;
                 const int j = M2::idx(g);
 #pragma unroll
-                for (int r = 0; r < R2; r++) u[t][r] = base[(j + r * M2::B) * ES];
+                for (int r = 0; r < R2; r++) u[t][r] = base[wl_slot<SK>(j + r * M2::B) * ES];
             }
         }
-        wl_stages<DIR, T, T2, L, REV, NLW, LF, LP, ES, S + 1>(u, X, W, lane, nvalid, done);
+        wl_stages<DIR, T, T2, L, REV, NLW, LF, LP, ES, SK, S + 1>(u, X, W, lane, nvalid, done);
     }
 }
 
@@ -143,10 +152,25 @@ __device__ __forceinline__ void wl_stages(typename WRegs<T2, L, REV, S, NLW, LF>
 // each (88% of the lanes busy in both phases); measured 2-3% faster than 4, 8 or 16
 #define FTS_SLAB_THREADS_64 320
 #endif
+// Plane row pitch (complex elements) of the slab pass. A y column exchange puts the
+// NLY columns of a wavefront on consecutive lanes and the next butterfly on the next
+// row, so a pitch of NLY (mod 8) keeps 8 consecutive lanes' 16-byte slots on distinct
+// banks; the pitch also holds a skewed packed z line in two rows (2P >= N + (N-1)/8).
+// Chosen with a bank model of every LDS access of the pass.
+// FTS_SLAB_PAD=0 (a tuning define) keeps the unpadded pitch N/2+1 and unskewed z lines.
+#ifndef FTS_SLAB_PAD
+#define FTS_SLAB_PAD 1
+#endif
+constexpr bool WSLAB_SKEW = FTS_SLAB_PAD != 0;
+constexpr int wslab_pitch(int N) {
+    if (!WSLAB_SKEW) return N / 2 + 1;
+    return N == 32 ? 21 : N == 48 ? 37 : N == 64 ? 39 : N == 96 ? 60 : N == 128 ? 77 : N / 2 + 1;
+}
+
 template <int N>
 struct WSlab {
     static constexpr int NZC = N / 2 + 1;
-    static constexpr int P = NZC;   // plane row pitch (complex); packed z line p spans rows 2p, 2p+1
+    static constexpr int P = wslab_pitch(N);   // plane row pitch; packed z line p spans rows 2p, 2p+1
     static constexpr int NP = N / 2;
     static constexpr int BD = N <= 32 ? 256 : (N <= 48 ? 384 : (N <= 64 ? FTS_SLAB_THREADS_64 : 1024));
     static constexpr int NW = BD / 64;
@@ -154,6 +178,9 @@ struct WSlab {
     static constexpr int NLZ = (NP + NW - 1) / NW;    // packed z rows per wavefront
     static constexpr int NSPEC = N * NZC;
     static constexpr size_t lds_elems = (size_t)N * P + 2 * N;
+    static_assert(P >= NZC && (!WSLAB_SKEW || 2 * P >= N + (N - 1) / 8),
+                  "a row holds a half spectrum, two rows a skewed z line");
+    static_assert(lds_elems * 16 <= 160 * 1024, "the FP64 plane and twiddles fit the LDS");
     using YI = WPlan<N, false>;
     using YF = WPlan<N, false>;
     using ZI = WPlan<N, false>;
@@ -218,7 +245,7 @@ __global__ void __launch_bounds__(WSlab<N>::BD) slab_kernel_wl(SlabBatch<T> batc
             }
         }
         __syncthreads();   // twiddles stored; the plane's previous users are done
-        wl_stages<+1, T, T2, N, false, G::NLY, true, 1, P, 0>(v, S + col0, Wy, lane, ncol, [&](auto& o) {
+        wl_stages<+1, T, T2, N, false, G::NLY, true, 1, P, false, 0>(v, S + col0, Wy, lane, ncol, [&](auto& o) {
             using MO = WStageMap<N, false, G::YI::n - 1, G::NLY, true>;
             wave_sync();
 #pragma unroll
@@ -259,7 +286,7 @@ __global__ void __launch_bounds__(WSlab<N>::BD) slab_kernel_wl(SlabBatch<T> batc
                 }
             }
         }
-        wl_stages<+1, T, T2, N, false, G::NLZ, false, 2 * P, 1, 0>(u, S + 2 * p0 * P, Wz, lane, nrow, [&](auto& o) {
+        wl_stages<+1, T, T2, N, false, G::NLZ, false, 2 * P, 1, WSLAB_SKEW, 0>(u, S + 2 * p0 * P, Wz, lane, nrow, [&](auto& o) {
 #pragma unroll
             for (int t = 0; t < MZ::NT; t++) {
                 const int g = lane + 64 * t;
@@ -286,7 +313,7 @@ __global__ void __launch_bounds__(WSlab<N>::BD) slab_kernel_wl(SlabBatch<T> batc
                     u[t][r] = T2{qe[t][r] * factor<T>(we[t][r], has_w, coef), qo[t][r] * factor<T>(wo[t][r], has_w, coef)};
             }
         }
-        wl_stages<-1, T, T2, N, true, G::NLZ, false, 2 * P, 1, 0>(u, S + 2 * p0 * P, Wz, lane, nrow, [&](auto& o) {
+        wl_stages<-1, T, T2, N, true, G::NLZ, false, 2 * P, 1, WSLAB_SKEW, 0>(u, S + 2 * p0 * P, Wz, lane, nrow, [&](auto& o) {
             using MO = WStageMap<N, true, G::ZF::n - 1, G::NLZ, false>;
             wave_sync();
 #pragma unroll
@@ -339,7 +366,7 @@ __global__ void __launch_bounds__(WSlab<N>::BD) slab_kernel_wl(SlabBatch<T> batc
                 for (int r = 0; r < G::YF::r(0); r++) v[t][r] = S[(j + r * MD::B) * P + c];
             }
         }
-        wl_stages<-1, T, T2, N, false, G::NLY, true, 1, P, 0>(v, S + col0, Wy, lane, ncol, [&](auto& o) {
+        wl_stages<-1, T, T2, N, false, G::NLY, true, 1, P, false, 0>(v, S + col0, Wy, lane, ncol, [&](auto& o) {
             using MO = WStageMap<N, false, G::YF::n - 1, G::NLY, true>;
 #pragma unroll
             for (int t = 0; t < MO::NT; t++) {
@@ -470,7 +497,7 @@ __global__ void __launch_bounds__(64) xpass_kernel_wl(XBatch<T> batch, const typ
             }
         }
         wave_sync();
-        wl_stages<+1, T, T2, N, true, NL, true, 1, NL, 0>(v, S, Wx, lane, nl, store);
+        wl_stages<+1, T, T2, N, true, NL, true, 1, NL, false, 0>(v, S, Wx, lane, nl, store);
         return;
     }
     using M0 = WStageMap<N, false, 0, NL, true>;
@@ -485,7 +512,7 @@ __global__ void __launch_bounds__(64) xpass_kernel_wl(XBatch<T> batch, const typ
         }
     }
     wave_sync();
-    wl_stages<-1, T, T2, N, false, NL, true, 1, NL, 0>(v, S, Wx, lane, nl, [&](auto& o) {
+    wl_stages<-1, T, T2, N, false, NL, true, 1, NL, false, 0>(v, S, Wx, lane, nl, [&](auto& o) {
         using MF = WStageMap<N, false, F::n - 1, NL, true>;
         constexpr int RF = F::r(F::n - 1);
         if (table == -1) {
@@ -516,7 +543,7 @@ __global__ void __launch_bounds__(64) xpass_kernel_wl(XBatch<T> batch, const typ
                 }
             }
         }
-        wl_stages<+1, T, T2, N, true, NL, true, 1, NL, 0>(o, S, Wx, lane, nl, store);
+        wl_stages<+1, T, T2, N, true, NL, true, 1, NL, false, 0>(o, S, Wx, lane, nl, store);
     });
 }
 
