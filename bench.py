@@ -711,6 +711,9 @@ def main():
                     help="learned: after the plain arm, train the CNN guess and time the steps that use it")
     ap.add_argument("--guess-train-steps", type=int, default=1000, help="Langevin steps the CNN trains on")
     ap.add_argument("--guess-train-iters", type=int, default=16, help="Adam iterations per training step")
+    ap.add_argument("--guess-lr", type=float, default=1e-3, help="initial Adam learning rate of the guess")
+    ap.add_argument("--guess-lr-final", type=float, default=1e-4, help="learning rate at the end of training")
+    ap.add_argument("--guess-replay", type=int, default=64, help="stored (w-, w+) pairs replayed in training")
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"],
                     help="mixed: saddle iterations on FP32 propagators until the error < tol, then FP64")
     args = ap.parse_args()
@@ -751,9 +754,9 @@ def main():
     params = lfts_params(w, "/tmp/lfts_bench")
     mixed = args.precision == "mixed"
     params["saddle"]["mixed_precision"] = mixed
-    params["learned_guess"] = {"train_steps": args.guess_train_steps, "lr": 1e-3, "train_iters": args.guess_train_iters,
-                               "replay": 64,
-                               "delta": True, "seed": 0}
+    params["learned_guess"] = {"train_steps": args.guess_train_steps, "lr": args.guess_lr,
+                               "lr_final": args.guess_lr_final, "train_iters": args.guess_train_iters,
+                               "replay": args.guess_replay, "delta": True, "seed": 0}
     with stdout_to_stderr():
         sim = lfts.LFTS(params, random_seed=args.seed)
         tol = sim.saddle["tolerance"]
