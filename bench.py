@@ -201,7 +201,17 @@ def plan_dict(mol, an, model):
     return {"model": model, "polymers": pols, "propagators": props, "blocks": blocks}
 
 
-SLAB_KERNEL = re.compile(r"slab_kernel_\w+<double, 1, 64>")
+SLAB_KERNEL = re.compile(r"slab_kernel_\w+<double, 1, 64(, \d+)?>")
+
+
+def fft_schedule_info(n):
+    """The FFT launch shapes the autotuner picked for an n^3 grid in this process
+    (slab workgroup threads, pencil kz columns per wavefront, measured candidates)."""
+    from langevin_fts_amd import capi
+
+    g = capi.Grid(capi.Context(0), [n] * 3, [1.0] * 3)
+    return {p: dict(zip(("slab_threads", "pencil_columns", "note"), g.fft_schedule(code)))
+            for p, code in (("fp64", capi.FTS_FP64), ("fp32", capi.FTS_FP32))}
 
 
 def pmc_child(workload):
@@ -330,11 +340,11 @@ def replica_bench(args, w, lf, lfts):
         "data": "synthetic: w_A, w_B ~ N(0,1) from PCG64(seed); replica r's noise from PCG64(seed + r)",
         "config": {"workload": describe(args.workload, w), "replicas": R,
                    "saddle_iterations_per_step": float(np.mean(its)), "saddle_failures": fails,
-                   "max_saddle_iterations": int(np.max(its))},
+                   "max_saddle_iterations": int(np.max(its)), "fft_schedule": fft_schedule_info(w["nx"][0])},
     }
     if kern:
         out["roofline"] = kernel_roofline(
-            kern, f"slab pass (slab_kernel_wl<double, 1, {w['nx'][0]}>: up to 32 propagators per launch)")
+            kern, f"slab pass (slab_kernel_wl<double, 1, {w['nx'][0]}, threads>: up to 32 propagators per launch)")
     print(json.dumps(out), flush=True)
 
 
@@ -842,6 +852,7 @@ def main():
                    "plain_saddle_failures": plain["failures"],
                    "propagator_evaluations_per_s": plain["iterations"] / plain_s},
     }
+    out["config"]["fft_schedule"] = fft_schedule_info(w["nx"][0])
     if learned64 is not None:
         out["config"]["learned_fp64_steps_per_s"] = args.steps / learned64["s"]
         out["config"]["learned_fp64_saddle_iterations_per_step"] = learned64["iterations"] / args.steps
@@ -863,8 +874,8 @@ def main():
         out["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBPS, "unit": "GB/s",
                            "frac": achieved / PEAK_HBM_GBPS,
                            "traffic": traffic["bytes"] if traffic else None,
-                           "kernel": "slab pass (slab_kernel_wl<double, 1, 64>: 2 propagators per launch), "
-                                     "timed in the plain FP64 arm",
+                           "kernel": "slab pass (slab_kernel_wl<double, 1, 64, threads>: 2 propagators per "
+                                     "launch), timed in the plain FP64 arm",
                            "timed_launch_count": sl["launches"],
                            "avg_launch_us": 1e3 * sl["milliseconds"] / max(sl["launches"], 1),
                            "bytes_per_launch": sl["bytes"] / max(sl["launches"], 1),

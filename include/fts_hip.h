@@ -83,6 +83,18 @@ int fts_grid_inner_product(fts_grid* g, const double* a, const double* b, int on
  * allowed. batch fields are contiguous. */
 int fts_grid_fft_forward(fts_grid* g, int precision, const void* d_in, void* d_out, int batch);
 int fts_grid_fft_backward(fts_grid* g, int precision, const void* d_in, void* d_out, int batch);
+/* FFT schedule of N^3 grids (N = 32, 48, 64, 96, 128); no counterpart in the
+ * reference, whose FFTs are library calls (MklFFT3D.cpp:11-79, CudaFFT). Per
+ * precision, the grid picks the slab-pass workgroup size (slab_threads) and the kz
+ * columns per wavefront of the pencil pass (pencil_columns) by timing the candidates
+ * when it is created (environment FTS_FFT_TUNE=0: the defaults); results of the
+ * candidates agree to rounding (bit-identical at 32^3 and 64^3), and one process
+ * keeps one choice per (N, precision). get writes the choice and a note with the measured
+ * candidates (NUL-terminated, truncated to notelen); set pins a candidate. Other
+ * grids report 0 and refuse set. */
+int fts_grid_get_fft_schedule(fts_grid* g, int precision, int* slab_threads, int* pencil_columns, char* note,
+                              int notelen);
+int fts_grid_set_fft_schedule(fts_grid* g, int precision, int slab_threads, int pencil_columns);
 
 /* ---- propagator computation plan: the output of PropagatorAnalyzer
  * (PropagatorAnalyzer.cpp:142-269) and Scheduler (Scheduler.cpp:11-190) as flat

@@ -64,6 +64,8 @@ def lib():
             "fts_grid_inner_product": (i, [vp, _DP, _DP, i, _DP]),
             "fts_grid_fft_forward": (i, [vp, i, vp, vp, i]),
             "fts_grid_fft_backward": (i, [vp, i, vp, vp, i]),
+            "fts_grid_get_fft_schedule": (i, [vp, i, _IP, _IP, ctypes.c_char_p, i]),
+            "fts_grid_set_fft_schedule": (i, [vp, i, i, i]),
             "fts_solver_create": (i, [vp, ctypes.POINTER(FtsPlan), i, VPP]),
             "fts_solver_create_realspace": (i, [vp, ctypes.POINTER(FtsPlan), _IP, VPP]),
             "fts_solver_create_reduce_memory": (i, [vp, ctypes.POINTER(FtsPlan), i, _IP, VPP]),
@@ -229,6 +231,17 @@ class Grid:
 
     def fft_backward(self, d_in, d_out, batch=1, precision=FTS_FP64):
         check(lib().fts_grid_fft_backward(self.h, precision, ctypes.c_void_p(d_in), ctypes.c_void_p(d_out), batch))
+
+    def fft_schedule(self, precision=FTS_FP64):
+        """(slab workgroup threads, pencil kz columns per wavefront, note) of the
+        wave-local FFT kernels; (0, 0, ...) on grids without them."""
+        bd, nl = ctypes.c_int(), ctypes.c_int()
+        buf = ctypes.create_string_buffer(512)
+        check(lib().fts_grid_get_fft_schedule(self.h, precision, ctypes.byref(bd), ctypes.byref(nl), buf, 512))
+        return bd.value, nl.value, buf.value.decode()
+
+    def set_fft_schedule(self, slab_threads, pencil_columns, precision=FTS_FP64):
+        check(lib().fts_grid_set_fft_schedule(self.h, precision, slab_threads, pencil_columns))
 
 
 def q_input_name(key):

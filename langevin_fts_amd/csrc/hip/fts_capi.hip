@@ -1,5 +1,7 @@
 // extern "C" entry points of include/fts_hip.h: argument checks, exception to
 // status-code conversion, thread-local error text.
+#include <algorithm>
+#include <cstring>
 #include <string>
 #include <vector>
 
@@ -161,6 +163,32 @@ int fts_grid_inner_product(fts_grid* g, const double* a, const double* b, int on
         need(b, "b");
         need(out, "out");
         *out = g->g->inner_product(a, b, on_device != 0);
+    });
+}
+
+int fts_grid_get_fft_schedule(fts_grid* g, int precision, int* slab_threads, int* pencil_columns, char* note,
+                              int notelen) {
+    return guarded([&] {
+        need(g, "grid");
+        FTS_REQUIRE(precision == FTS_FP64 || precision == FTS_FP32, "precision must be FTS_FP64 or FTS_FP32");
+        const fts::Grid& gr = *g->g;
+        const int p = precision == FTS_FP32 ? 1 : 0;
+        if (slab_threads) *slab_threads = gr.wave_local ? gr.slab_bd[p] : 0;
+        if (pencil_columns) *pencil_columns = gr.wave_local ? gr.pencil_nl[p] : 0;
+        if (note && notelen > 0) {
+            const std::string& s = gr.wave_local ? gr.fft_schedule[p] : std::string("runtime kernels (no schedule)");
+            const size_t n = std::min(s.size(), (size_t)notelen - 1);
+            std::memcpy(note, s.data(), n);
+            note[n] = 0;
+        }
+    });
+}
+
+int fts_grid_set_fft_schedule(fts_grid* g, int precision, int slab_threads, int pencil_columns) {
+    return guarded([&] {
+        need(g, "grid");
+        g->g->set_fft_schedule(precision == FTS_FP32 ? 1 : (precision == FTS_FP64 ? 0 : -1), slab_threads,
+                               pencil_columns);
     });
 }
 

@@ -144,40 +144,50 @@ __device__ __forceinline__ void wl_stages(typename WRegs<T2, L, REV, S, NLW, LF>
 }
 
 // ------------------------------------------------------------------ slab pass
-#ifndef FTS_PENCIL_NL
-#define FTS_PENCIL_NL 8
-#endif
-#ifndef FTS_SLAB_THREADS_64
-// workgroup size of the 64^3 slab pass: 5 wavefronts own 7 y columns / 7 packed z rows
-// each (88% of the lanes busy in both phases); measured 2-3% faster than 4, 8 or 16
-#define FTS_SLAB_THREADS_64 320
-#endif
+// Launch shapes of the wave-local kernels. Each (N, precision) picks its slab workgroup
+// size and pencil width from the candidates below when a grid is created, by timing
+// them (Grid::tune_fft_schedule); every candidate runs the same per-line transforms,
+// so results agree to rounding (bit-identical at 32^3 and 64^3). The defaults are used
+// when tuning is off (FTS_FFT_TUNE=0).
+//   64^3 slab default: 5 wavefronts own 7 y columns / 7 packed z rows each (88% of the
+//   lanes busy in both phases).
+constexpr int wslab_default_bd(int N) {
+    return N <= 32 ? 256 : (N <= 48 ? 384 : (N <= 64 ? 320 : 1024));
+}
+constexpr int WPENCIL_DEFAULT_NL = 8;
+
 // Plane row pitch (complex elements) of the slab pass. A y column exchange puts the
 // NLY columns of a wavefront on consecutive lanes and the next butterfly on the next
 // row, so a pitch of NLY (mod 8) keeps 8 consecutive lanes' 16-byte slots on distinct
 // banks; the pitch also holds a skewed packed z line in two rows (2P >= N + (N-1)/8).
-// Chosen with a bank model of every LDS access of the pass.
+// Chosen with a bank model of every LDS access of the pass (48^3: one more step of 8,
+// better for the 64-bank case of that model).
 // FTS_SLAB_PAD=0 (a tuning define) keeps the unpadded pitch N/2+1 and unskewed z lines.
 #ifndef FTS_SLAB_PAD
 #define FTS_SLAB_PAD 1
 #endif
 constexpr bool WSLAB_SKEW = FTS_SLAB_PAD != 0;
-constexpr int wslab_pitch(int N) {
+constexpr int wslab_pitch(int N, int NLY) {
     if (!WSLAB_SKEW) return N / 2 + 1;
-    return N == 32 ? 21 : N == 48 ? 37 : N == 64 ? 39 : N == 96 ? 60 : N == 128 ? 77 : N / 2 + 1;
+    const int lo0 = (N + (N - 1) / 8 + 1) / 2;
+    const int lo = lo0 > N / 2 + 1 ? lo0 : N / 2 + 1;
+    const int P = lo + ((NLY - lo) % 8 + 8) % 8;
+    return (N == 48 && NLY == 5) ? P + 8 : P;
 }
 
-template <int N>
+template <int N, int BD_ = wslab_default_bd(N)>
 struct WSlab {
     static constexpr int NZC = N / 2 + 1;
-    static constexpr int P = wslab_pitch(N);   // plane row pitch; packed z line p spans rows 2p, 2p+1
     static constexpr int NP = N / 2;
-    static constexpr int BD = N <= 32 ? 256 : (N <= 48 ? 384 : (N <= 64 ? FTS_SLAB_THREADS_64 : 1024));
+    static constexpr int BD = BD_;
     static constexpr int NW = BD / 64;
     static constexpr int NLY = (NZC + NW - 1) / NW;   // y columns per wavefront
     static constexpr int NLZ = (NP + NW - 1) / NW;    // packed z rows per wavefront
+    static constexpr int P = wslab_pitch(N, NLY);     // plane row pitch; packed z line p spans rows 2p, 2p+1
     static constexpr int NSPEC = N * NZC;
     static constexpr size_t lds_elems = (size_t)N * P + 2 * N;
+    static_assert(BD % 64 == 0 && BD <= 1024, "whole wavefronts, at most 1024 threads");
+    static_assert(NW * NLY >= NZC && NW * NLZ >= NP, "the wavefronts cover every y column and packed z row");
     static_assert(P >= NZC && (!WSLAB_SKEW || 2 * P >= N + (N - 1) / 8),
                   "a row holds a half spectrum, two rows a skewed z line");
     static_assert(lds_elems * 16 <= 160 * 1024, "the FP64 plane and twiddles fit the LDS");
@@ -189,11 +199,11 @@ struct WSlab {
     using MZ = WMap<N, RZ, NLZ, false>;   // real values of a wavefront's packed rows
 };
 
-template <typename T, int MODE, int N>
-__global__ void __launch_bounds__(WSlab<N>::BD) slab_kernel_wl(SlabBatch<T> batch, const typename vec2<T>::type* twy,
-                                                              const typename vec2<T>::type* twz, double inv_m_d) {
+template <typename T, int MODE, int N, int BD>
+__global__ void __launch_bounds__(BD) slab_kernel_wl(SlabBatch<T> batch, const typename vec2<T>::type* twy,
+                                                     const typename vec2<T>::type* twz, double inv_m_d) {
     using T2 = typename vec2<T>::type;
-    using G = WSlab<N>;
+    using G = WSlab<N, BD>;
     using MZ = typename G::MZ;
     constexpr int RZ = G::RZ, NZC = G::NZC, P = G::P;
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -440,10 +450,10 @@ __global__ void __launch_bounds__(WSlab<N>::BD) slab_kernel_wl(SlabBatch<T> batc
 }
 
 // ------------------------------------------------------------------ pencil pass
-template <int N>
+template <int N, int NL_ = WPENCIL_DEFAULT_NL>
 struct WPencil {
     static constexpr int NZC = N / 2 + 1;
-    static constexpr int NL = FTS_PENCIL_NL;   // kz columns per wavefront
+    static constexpr int NL = NL_;   // kz columns per wavefront
     static constexpr int TILES = (NZC + NL - 1) / NL;
     static constexpr int BD = 64;
     static constexpr size_t lds_elems = (size_t)N * NL + N;
@@ -451,11 +461,11 @@ struct WPencil {
     using I = WPlan<N, true>;   // first radix = last radix of F
 };
 
-template <typename T, int N>
+template <typename T, int N, int NLW>
 __global__ void __launch_bounds__(64) xpass_kernel_wl(XBatch<T> batch, const typename vec2<T>::type* twx,
                                                       const T* bx, const T* by, const T* bz) {
     using T2 = typename vec2<T>::type;
-    using G = WPencil<N>;
+    using G = WPencil<N, NLW>;
     using F = typename G::F;
     using I = typename G::I;
     constexpr int NL = G::NL, NZC = G::NZC;

@@ -1,7 +1,11 @@
 // Grid (ComputationBox + FFT plan), separable Boltzmann tables and the
 // deterministic reductions (replacing cub::DeviceReduce::Sum, CudaComputationBox.cu:56-118).
+#include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
+#include <map>
+#include <string>
 
 #include "fts_internal.hpp"
 #include "fts_reduce.hpp"
@@ -129,14 +133,81 @@ void allow_lds_ct() {
     allow_lds(&slab_kernel_ct<float, SLAB_INV2, N>);
     allow_lds(&xpass_kernel_ct<double, N>);
     allow_lds(&xpass_kernel_ct<float, N>);
-    allow_lds(&slab_kernel_wl<double, SLAB_FWD, N>);
-    allow_lds(&slab_kernel_wl<double, SLAB_INV1, N>);
-    allow_lds(&slab_kernel_wl<double, SLAB_INV2, N>);
-    allow_lds(&slab_kernel_wl<float, SLAB_FWD, N>);
-    allow_lds(&slab_kernel_wl<float, SLAB_INV1, N>);
-    allow_lds(&slab_kernel_wl<float, SLAB_INV2, N>);
-    allow_lds(&xpass_kernel_wl<double, N>);
-    allow_lds(&xpass_kernel_wl<float, N>);
+}
+
+// Launch-shape candidates of the wave-local kernels (the autotuner's search space)
+constexpr bool slab_bd_ok(int N, int bd) {
+    return (N == 32 && (bd == 128 || bd == 256)) || (N == 48 && (bd == 256 || bd == 384)) ||
+           (N == 64 && (bd == 256 || bd == 320 || bd == 512)) || (N == 96 && (bd == 512 || bd == 1024)) ||
+           (N == 128 && (bd == 512 || bd == 1024));
+}
+constexpr int SLAB_BD_ALL[] = {128, 256, 320, 384, 512, 1024};
+constexpr int PENCIL_NL_ALL[] = {4, 8, 16};
+
+template <typename T, int N, int BD>
+void launch_slab_wl(const Grid& g, int mode, const SlabBatch<T>& b) {
+    static const bool attr = [] {
+        allow_lds(&slab_kernel_wl<T, SLAB_FWD, N, BD>);
+        allow_lds(&slab_kernel_wl<T, SLAB_INV1, N, BD>);
+        allow_lds(&slab_kernel_wl<T, SLAB_INV2, N, BD>);
+        return true;
+    }();
+    (void)attr;
+    using G = WSlab<N, BD>;
+    const dim3 grid(N, b.n), block(BD);
+    const size_t lds = G::lds_elems * sizeof(typename vec2<T>::type);
+    if (mode == SLAB_FWD)
+        hipLaunchKernelGGL((slab_kernel_wl<T, SLAB_FWD, N, BD>), grid, block, lds, g.stream(), b, g.tw<T>(1),
+                           g.tw<T>(2), g.sg.inv_m);
+    else if (mode == SLAB_INV1)
+        hipLaunchKernelGGL((slab_kernel_wl<T, SLAB_INV1, N, BD>), grid, block, lds, g.stream(), b, g.tw<T>(1),
+                           g.tw<T>(2), g.sg.inv_m);
+    else
+        hipLaunchKernelGGL((slab_kernel_wl<T, SLAB_INV2, N, BD>), grid, block, lds, g.stream(), b, g.tw<T>(1),
+                           g.tw<T>(2), g.sg.inv_m);
+}
+
+template <typename T, int N>
+void launch_slab_wl_bd(const Grid& g, int bd, int mode, const SlabBatch<T>& b) {
+    switch (bd) {
+        case 128: if constexpr (slab_bd_ok(N, 128)) return launch_slab_wl<T, N, 128>(g, mode, b); break;
+        case 256: if constexpr (slab_bd_ok(N, 256)) return launch_slab_wl<T, N, 256>(g, mode, b); break;
+        case 320: if constexpr (slab_bd_ok(N, 320)) return launch_slab_wl<T, N, 320>(g, mode, b); break;
+        case 384: if constexpr (slab_bd_ok(N, 384)) return launch_slab_wl<T, N, 384>(g, mode, b); break;
+        case 512: if constexpr (slab_bd_ok(N, 512)) return launch_slab_wl<T, N, 512>(g, mode, b); break;
+        case 1024: if constexpr (slab_bd_ok(N, 1024)) return launch_slab_wl<T, N, 1024>(g, mode, b); break;
+        default: break;
+    }
+    throw Error("no wave-local slab kernel of " + std::to_string(bd) + " threads for N=" + std::to_string(N));
+}
+
+template <typename T, int N, int NL>
+void launch_xpass_wl(const Grid& g, const XBatch<T>& b, const T* bx, const T* by, const T* bz) {
+    using G = WPencil<N, NL>;
+    static const bool attr = [] {
+        allow_lds(&xpass_kernel_wl<T, N, NL>);
+        return true;
+    }();
+    (void)attr;
+    const dim3 grid(N * G::TILES, b.n), block(G::BD);
+    hipLaunchKernelGGL((xpass_kernel_wl<T, N, NL>), grid, block, G::lds_elems * sizeof(typename vec2<T>::type),
+                       g.stream(), b, g.tw<T>(0), bx, by, bz);
+}
+
+template <typename T, int N>
+void launch_xpass_wl_nl(const Grid& g, int nl, const XBatch<T>& b, const T* bx, const T* by, const T* bz) {
+    switch (nl) {
+        case 4: return launch_xpass_wl<T, N, 4>(g, b, bx, by, bz);
+        case 8: return launch_xpass_wl<T, N, 8>(g, b, bx, by, bz);
+        case 16: return launch_xpass_wl<T, N, 16>(g, b, bx, by, bz);
+        default: break;
+    }
+    throw Error("no wave-local pencil kernel of " + std::to_string(nl) + " columns");
+}
+
+template <typename T>
+constexpr int prec_index() {
+    return std::is_same<T, float>::value ? 1 : 0;
 }
 
 void set_lds_attributes_ct() {
@@ -153,18 +224,8 @@ void set_lds_attributes_ct() {
 template <typename T, int N>
 void launch_slab_ct(const Grid& g, int mode, const SlabBatch<T>& b) {
     if (g.wave_local) {
-        using G = WSlab<N>;
-        const dim3 grid(N, b.n), block(G::BD);
-        const size_t lds = G::lds_elems * sizeof(typename vec2<T>::type);
-        if (mode == SLAB_FWD)
-            hipLaunchKernelGGL((slab_kernel_wl<T, SLAB_FWD, N>), grid, block, lds, g.stream(), b, g.tw<T>(1),
-                               g.tw<T>(2), g.sg.inv_m);
-        else if (mode == SLAB_INV1)
-            hipLaunchKernelGGL((slab_kernel_wl<T, SLAB_INV1, N>), grid, block, lds, g.stream(), b, g.tw<T>(1),
-                               g.tw<T>(2), g.sg.inv_m);
-        else
-            hipLaunchKernelGGL((slab_kernel_wl<T, SLAB_INV2, N>), grid, block, lds, g.stream(), b, g.tw<T>(1),
-                               g.tw<T>(2), g.sg.inv_m);
+        const int bd = g.slab_bd[prec_index<T>()];
+        launch_slab_wl_bd<T, N>(g, bd ? bd : wslab_default_bd(N), mode, b);
         return;
     }
     using G = SlabShape<N>;
@@ -184,10 +245,8 @@ void launch_slab_ct(const Grid& g, int mode, const SlabBatch<T>& b) {
 template <typename T, int N>
 void launch_xpass_ct(const Grid& g, const XBatch<T>& b, const T* bx, const T* by, const T* bz) {
     if (g.wave_local) {
-        using G = WPencil<N>;
-        const dim3 grid(N * G::TILES, b.n), block(G::BD);
-        hipLaunchKernelGGL((xpass_kernel_wl<T, N>), grid, block, G::lds_elems * sizeof(typename vec2<T>::type),
-                           g.stream(), b, g.tw<T>(0), bx, by, bz);
+        const int nl = g.pencil_nl[prec_index<T>()];
+        launch_xpass_wl_nl<T, N>(g, nl ? nl : WPENCIL_DEFAULT_NL, b, bx, by, bz);
         return;
     }
     using G = PencilShape<N>;
@@ -328,6 +387,166 @@ void Grid::setup_fft() {
     // wave-local line transforms for those grids (env FTS_CT_KERNELS=1: the block-synchronous ones)
     wave_local = ct_n != 0 && !std::getenv("FTS_CT_KERNELS");
     if (ct_n) set_lds_attributes_ct();
+    if (wave_local) {
+        tune_fft_schedule<double>();
+        tune_fft_schedule<float>();
+    }
+}
+
+// ------------------------------------------------------------------ FFT schedule autotuner
+// Times every launch-shape candidate of the wave-local slab and pencil kernels for this
+// grid and precision on scratch fields (one discrete-chain step of two propagators:
+// slab inverse+forward with exp(-w ds) factors, pencil with the bond factors), and keeps
+// the fastest; a candidate replaces the default only when it is at least 2% faster.
+// The choice is cached per (N, precision) for the process. FTS_FFT_TUNE=0 keeps the
+// defaults. Candidates run the same per-line transforms; results agree to rounding
+// (bit-identical at 32^3 and 64^3, where the compiler contracts both alike).
+namespace {
+struct TuneChoice {
+    int bd, nl;
+    std::string note;
+};
+std::map<std::pair<int, int>, TuneChoice>& tune_cache() {
+    static std::map<std::pair<int, int>, TuneChoice> c;
+    return c;
+}
+
+std::string us_text(int shape, float ms) {
+    char buf[48];
+    std::snprintf(buf, sizeof(buf), " %d:%.2fus", shape, 1e3 * ms);
+    return buf;
+}
+
+template <typename F>
+float time_launches(hipStream_t st, F&& launch) {
+    hipEvent_t e0, e1;
+    FTS_HIP_CHECK(hipEventCreate(&e0));
+    FTS_HIP_CHECK(hipEventCreate(&e1));
+    for (int i = 0; i < 3; i++) launch();
+    float best = 1e30f;
+    for (int rep = 0; rep < 3; rep++) {
+        FTS_HIP_CHECK(hipEventRecord(e0, st));
+        for (int i = 0; i < 8; i++) launch();
+        FTS_HIP_CHECK(hipEventRecord(e1, st));
+        FTS_HIP_CHECK(hipEventSynchronize(e1));
+        float ms = 0.f;
+        FTS_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+        best = std::min(best, ms / 8.f);
+    }
+    FTS_HIP_CHECK(hipGetLastError());
+    FTS_HIP_CHECK(hipEventDestroy(e0));
+    FTS_HIP_CHECK(hipEventDestroy(e1));
+    return best;
+}
+
+template <typename T, int N>
+TuneChoice tune_wave_local(const Grid& g) {
+    using T2 = typename vec2<T>::type;
+    constexpr int NZC = N / 2 + 1, ITEMS = 2;
+    const size_t M = (size_t)N * N * N, Mc = (size_t)N * N * NZC;
+    DeviceBuffer spec, q, w, tab;
+    spec.allocate(sizeof(T2) * Mc * ITEMS);
+    q.allocate(sizeof(T) * M * ITEMS);
+    w.allocate(sizeof(T) * M);
+    tab.allocate(sizeof(T) * (2 * N + NZC));
+    hipStream_t st = g.stream();
+    FTS_HIP_CHECK(hipMemsetAsync(spec.get(), 0, sizeof(T2) * Mc * ITEMS, st));
+    FTS_HIP_CHECK(hipMemsetAsync(w.get(), 0, sizeof(T) * M, st));
+    FTS_HIP_CHECK(hipMemsetAsync(tab.get(), 0, sizeof(T) * (2 * N + NZC), st));
+    SlabBatch<T> sb{};
+    XBatch<T> xb{};
+    sb.n = xb.n = ITEMS;
+    for (int i = 0; i < ITEMS; i++) {
+        SlabItem<T>& it = sb.it[i];
+        it.qout = q.as<T>() + i * M;
+        it.c0 = spec.as<T2>() + i * Mc;
+        it.w = w.as<T>();
+        // FP64 discrete steps read exp(-w ds) from a table (coef < 0), FP32 steps evaluate it
+        it.post0 = it.pre0 = std::is_same<T, double>::value ? (T)-1 : (T)0.01;
+        it.n_out = 1;
+        it.forward = 1;
+        xb.it[i].c = spec.as<T2>() + i * Mc;
+        xb.it[i].table = 0;
+    }
+    const T* bx = tab.as<T>();
+    TuneChoice c{wslab_default_bd(N), WPENCIL_DEFAULT_NL, ""};
+    std::string note = "slab threads";
+    float t_def = 0.f, t_best = 1e30f;
+    int best = c.bd;
+    for (int bd : SLAB_BD_ALL) {
+        if (!slab_bd_ok(N, bd)) continue;
+        const float t = time_launches(st, [&] { launch_slab_wl_bd<T, N>(g, bd, SLAB_INV1, sb); });
+        note += us_text(bd, t);
+        if (bd == c.bd) t_def = t;
+        if (t < t_best) {
+            t_best = t;
+            best = bd;
+        }
+    }
+    if (t_best < 0.98f * t_def) c.bd = best;
+    note += "; pencil columns";
+    t_best = 1e30f;
+    best = c.nl;
+    for (int nl : PENCIL_NL_ALL) {
+        const float t = time_launches(st, [&] { launch_xpass_wl_nl<T, N>(g, nl, xb, bx, bx + N, bx + 2 * N); });
+        note += us_text(nl, t);
+        if (nl == c.nl) t_def = t;
+        if (t < t_best) {
+            t_best = t;
+            best = nl;
+        }
+    }
+    if (t_best < 0.98f * t_def) c.nl = best;
+    FTS_HIP_CHECK(hipStreamSynchronize(st));
+    c.note = "tuned (" + note + ")";
+    return c;
+}
+}  // namespace
+
+template <typename T>
+void Grid::tune_fft_schedule() {
+    const int p = prec_index<T>();
+    slab_bd[p] = wslab_default_bd(ct_n);
+    pencil_nl[p] = WPENCIL_DEFAULT_NL;
+    const char* env = std::getenv("FTS_FFT_TUNE");
+    if (!wave_local || (env && env[0] == '0')) {
+        fft_schedule[p] = "defaults";
+        return;
+    }
+    auto& cache = tune_cache();
+    const auto key = std::make_pair(ct_n, p);
+    auto hit = cache.find(key);
+    if (hit == cache.end()) {
+        TuneChoice c{slab_bd[p], pencil_nl[p], ""};
+        try {
+            switch (ct_n) {
+                case 32: c = tune_wave_local<T, 32>(*this); break;
+                case 48: c = tune_wave_local<T, 48>(*this); break;
+                case 64: c = tune_wave_local<T, 64>(*this); break;
+                case 96: c = tune_wave_local<T, 96>(*this); break;
+                default: c = tune_wave_local<T, 128>(*this); break;
+            }
+        } catch (const Error& e) {
+            (void)hipGetLastError();
+            c = TuneChoice{wslab_default_bd(ct_n), WPENCIL_DEFAULT_NL, std::string("defaults (tuning failed: ") +
+                                                                       e.what() + ")"};
+        }
+        hit = cache.emplace(key, c).first;
+    }
+    slab_bd[p] = hit->second.bd;
+    pencil_nl[p] = hit->second.nl;
+    fft_schedule[p] = hit->second.note;
+}
+
+void Grid::set_fft_schedule(int precision, int threads, int columns) {
+    FTS_REQUIRE(wave_local, "this grid has no wave-local FFT kernels (cubic 32, 48, 64, 96 or 128 only)");
+    FTS_REQUIRE(precision == 0 || precision == 1, "precision must be FTS_FP64 or FTS_FP32");
+    FTS_REQUIRE(slab_bd_ok(ct_n, threads), "no slab kernel of " + std::to_string(threads) + " threads for N=" +
+                                               std::to_string(ct_n));
+    FTS_REQUIRE(columns == 4 || columns == 8 || columns == 16, "pencil columns must be 4, 8 or 16");
+    slab_bd[precision] = threads;
+    pencil_nl[precision] = columns;
+    fft_schedule[precision] = "set by the caller";
 }
 
 // ComputationBox.cpp:229-262

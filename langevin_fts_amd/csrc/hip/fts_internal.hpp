@@ -101,6 +101,14 @@ public:
     bool big = false;        // some axis needs a radix outside {2, 3, 4, 5, 8}
     int ct_n = 0;        // N of the compile-time specialised kernels for an N^3 grid, 0: runtime kernels
     bool wave_local = false; // N^3 grids: one wavefront per line transform
+    // launch shapes of the wave-local kernels per precision ([0] FP64, [1] FP32):
+    // slab workgroup size and pencil kz columns per wavefront, with how they were chosen
+    int slab_bd[2] = {0, 0};
+    int pencil_nl[2] = {0, 0};
+    std::string fft_schedule[2];
+    template <typename T>
+    void tune_fft_schedule();
+    void set_fft_schedule(int precision, int slab_threads, int pencil_columns);
     DeviceBuffer partials;   // reduction scratch: MAX_BATCH * RED_BLOCKS doubles
     DeviceBuffer scalars;    // reduction results: MAX_BATCH doubles
     DeviceBuffer scratch64;  // general FP64 scratch of 2 fields
