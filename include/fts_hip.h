@@ -183,6 +183,14 @@ int fts_solver_compute_propagators(fts_solver* s, const double* const* w, const 
                                    int on_device);
 /* compute_concentrations (CpuComputationContinuous.cpp:371-443, CpuComputationDiscrete.cpp:584-663) */
 int fts_solver_compute_concentrations(fts_solver* s);
+/* HIP graphs of the evaluation; no counterpart in the reference (its CUDA platform
+ * launches eagerly). enable != 0: compute_propagators with on_device != 0 is captured
+ * once into a HIP graph and replayed while the field (and q_init) addresses, the box
+ * volume and the FFT launch shapes are those of the captured call, else captured
+ * again; compute_concentrations after it likewise. With kernel timing on, calls run
+ * eagerly. Results equal the eager calls'. Environment FTS_HIP_GRAPHS=1 sets it for
+ * every solver at creation. */
+int fts_solver_set_graphs(fts_solver* s, int enable);
 /* compute_stress (CpuComputationContinuous.cpp:643-726, CpuComputationDiscrete.cpp:857-996) */
 int fts_solver_compute_stress(fts_solver* s);
 /* get_total_partition / get_solvent_partition */

@@ -56,6 +56,7 @@ def lib():
             "fts_context_synchronize": (i, [vp]),
             "fts_context_stream": (i, [vp, VPP]),
             "fts_context_info": (i, [vp, ctypes.c_char_p, i]),
+            "fts_context_set_kernel_timing": (i, [vp, i]),
             "fts_grid_create": (i, [vp, i, _IP, _DP, _DP, VPP]),
             "fts_grid_destroy": (i, [vp]),
             "fts_grid_set_lx": (i, [vp, _DP]),
@@ -75,6 +76,7 @@ def lib():
             "fts_solver_update_laplacian": (i, [vp]),
             "fts_solver_compute_propagators": (i, [vp, ctypes.POINTER(_DP), ctypes.POINTER(_DP), i]),
             "fts_solver_compute_concentrations": (i, [vp]),
+            "fts_solver_set_graphs": (i, [vp, i]),
             "fts_solver_compute_stress": (i, [vp]),
             "fts_solver_get_partition": (i, [vp, i, _DP]),
             "fts_solver_get_solvent_partition": (i, [vp, i, _DP]),
@@ -177,6 +179,10 @@ class Context:
 
     def synchronize(self):
         check(lib().fts_context_synchronize(self.h))
+
+    def set_kernel_timing(self, sample_every):
+        """time one launch in sample_every of each kernel family with HIP events (0: off)"""
+        check(lib().fts_context_set_kernel_timing(self.h, sample_every))
 
     def __del__(self):
         if getattr(self, "h", None) and _LIB is not None:
@@ -392,6 +398,10 @@ class Solver:
 
     def compute_concentrations(self):
         check(lib().fts_solver_compute_concentrations(self.h))
+
+    def set_graphs(self, enable=True):
+        """replay device-input evaluations from captured HIP graphs"""
+        check(lib().fts_solver_set_graphs(self.h, 1 if enable else 0))
 
     def compute_statistics(self, w, q_init=None):
         self.compute_propagators(w, q_init)

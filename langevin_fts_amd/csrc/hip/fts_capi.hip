@@ -428,6 +428,13 @@ int fts_solver_compute_concentrations(fts_solver* s) {
     });
 }
 
+int fts_solver_set_graphs(fts_solver* s, int enable) {
+    return guarded([&] {
+        need(s, "solver");
+        s->s->set_graphs(enable != 0);
+    });
+}
+
 int fts_solver_compute_stress(fts_solver* s) {
     return guarded([&] {
         need(s, "solver");

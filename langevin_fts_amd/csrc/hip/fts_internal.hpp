@@ -20,6 +20,7 @@ class KernelTimer {
 public:
     ~KernelTimer();
     void enable(int sample_every);
+    bool active() const { return every_ != 0; }
     // a start event when this launch of the family is sampled, else nullptr
     hipEvent_t begin(hipStream_t st, int family);
     void end(hipStream_t st, hipEvent_t start, int family, double bytes);
@@ -177,6 +178,9 @@ public:
     virtual void propagator(int prop, int n, int half, double* out, bool on_device) = 0;
     // bytes of device memory and of pinned host memory the solver holds
     virtual void memory_usage(size_t* device, size_t* host) const = 0;
+    // replay compute_propagators + compute_concentrations from a captured HIP graph when
+    // the inputs are device fields at the same addresses as the captured call
+    virtual void set_graphs(bool on) = 0;
 };
 
 // bc: NULL for pseudo-spectral steps, else 2 dim FTS_BC_* codes (real-space steps, FP64 continuous)

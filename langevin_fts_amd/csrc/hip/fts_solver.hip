@@ -39,6 +39,11 @@ public:
     void sum_concentrations(int n, const int* kind, const int* idx, const double* scale, double* out,
                             bool on_device) override;
     void propagator(int prop, int n, int half, double* out, bool on_device) override;
+    void set_graphs(bool on) override {
+        graphs_ = on;
+        if (!on) drop_graphs();
+    }
+    ~Solver() override { drop_graphs(); }
     void memory_usage(size_t* device, size_t* host) const override {
         size_t d = 0, h = 0;
         for (const DeviceBuffer* b : {&qstore_, &hstore_, &w64_, &w32_, &ew_, &qinit_, &ws_, &phib_, &phis_, &Qd_,
@@ -94,6 +99,27 @@ private:
     void build_slice(const std::vector<std::array<int, 3>>& jobs);
     void build_stress_terms();
     void run_launches();
+    void issue_propagators(const double* const* w, const double* const* q_init, bool on_device);
+    void issue_concentrations();
+
+    // HIP graphs of the two evaluation calls, valid while their key holds: the input
+    // addresses, the box volume (a by-value kernel argument) and the FFT launch shapes
+    struct GraphKey {
+        std::vector<const void*> ptrs;
+        double volume = 0.0;
+        int bd = 0, nl = 0;
+        bool operator==(const GraphKey& o) const {
+            return ptrs == o.ptrs && volume == o.volume && bd == o.bd && nl == o.nl;
+        }
+    };
+    GraphKey graph_key(const double* const* w, const double* const* q_init) const;
+    bool graph_ok() const;
+    template <typename F>
+    void replay_or_capture(hipGraphExec_t& exec, GraphKey& have, const GraphKey& want, F&& issue);
+    void drop_graphs();
+    bool graphs_ = false;
+    hipGraphExec_t gprop_ = nullptr, gconc_ = nullptr;
+    GraphKey kprop_, kconc_;
 
     Grid* g_;
     int model_;
@@ -782,7 +808,71 @@ void Solver<T>::run_launches() {
 }
 
 template <typename T>
+typename Solver<T>::GraphKey Solver<T>::graph_key(const double* const* w, const double* const* q_init) const {
+    GraphKey k;
+    for (int m = 0; m < n_mon_; m++) k.ptrs.push_back(w ? w[m] : nullptr);
+    for (int i = 0; i < n_qinit_; i++) k.ptrs.push_back(q_init ? q_init[i] : nullptr);
+    k.volume = g_->volume;
+    const int p = std::is_same<T, float>::value ? 1 : 0;
+    k.bd = g_->slab_bd[p];
+    k.nl = g_->pencil_nl[p];
+    return k;
+}
+
+// sampled launch events stay out of graphs: with kernel timing on, calls run eagerly
+template <typename T>
+bool Solver<T>::graph_ok() const {
+    return graphs_ && !g_->ctx->timer.active();
+}
+
+template <typename T>
+void Solver<T>::drop_graphs() {
+    for (hipGraphExec_t* e : {&gprop_, &gconc_}) {
+        if (*e) (void)hipGraphExecDestroy(*e);
+        *e = nullptr;
+    }
+    kprop_ = GraphKey{};
+    kconc_ = GraphKey{};
+}
+
+template <typename T>
+template <typename F>
+void Solver<T>::replay_or_capture(hipGraphExec_t& exec, GraphKey& have, const GraphKey& want, F&& issue) {
+    hipStream_t st = g_->stream();
+    if (!exec || !(have == want)) {
+        if (exec) FTS_HIP_CHECK(hipGraphExecDestroy(exec));
+        exec = nullptr;
+        FTS_HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        hipGraph_t graph = nullptr;
+        try {
+            issue();
+        } catch (...) {
+            (void)hipStreamEndCapture(st, &graph);
+            if (graph) (void)hipGraphDestroy(graph);
+            throw;
+        }
+        FTS_HIP_CHECK(hipStreamEndCapture(st, &graph));
+        const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        FTS_HIP_CHECK(e);
+        have = want;
+    }
+    FTS_HIP_CHECK(hipGraphLaunch(exec, st));
+}
+
+template <typename T>
 void Solver<T>::compute_propagators(const double* const* wf, const double* const* q_init, bool on_device) {
+    for (int m = 0; m < n_mon_; m++) FTS_REQUIRE(wf[m] != nullptr, "missing field for a monomer type");
+    for (int i = 0; i < n_qinit_; i++) FTS_REQUIRE(q_init && q_init[i], "missing q_init field");
+    if (on_device && graph_ok())
+        replay_or_capture(gprop_, kprop_, graph_key(wf, q_init), [&] { issue_propagators(wf, q_init, true); });
+    else
+        issue_propagators(wf, q_init, on_device);
+    Q_valid_ = false;
+}
+
+template <typename T>
+void Solver<T>::issue_propagators(const double* const* wf, const double* const* q_init, bool on_device) {
     const hipMemcpyKind kind = on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
     const int cb = std::min(1024, ceil_div(M_, 256));
     for (int m = 0; m < n_mon_; m++) {
@@ -862,6 +952,15 @@ void Solver<T>::compute_propagators(const double* const* wf, const double* const
 
 template <typename T>
 void Solver<T>::compute_concentrations() {
+    if (graph_ok() && gprop_)
+        replay_or_capture(gconc_, kconc_, graph_key(nullptr, nullptr), [&] { issue_concentrations(); });
+    else
+        issue_concentrations();
+    Q_valid_ = false;
+}
+
+template <typename T>
+void Solver<T>::issue_concentrations() {
     const int gb = std::min(1024, ceil_div(M_, 256));
     for (size_t b0 = 0; b0 < blocks_.size(); b0 += MAX_BATCH) {
         PhiBatch<T> pb{};
@@ -1045,13 +1144,19 @@ void Solver<T>::propagator(int prop, int n, int hf, double* out, bool on_device)
 
 std::unique_ptr<SolverBase> make_solver(Grid* g, const fts_plan* plan, int precision, const int* bc,
                                         bool reduce_memory) {
-    if (precision == FTS_FP64) return std::unique_ptr<SolverBase>(new Solver<double>(g, plan, bc, reduce_memory));
-    if (precision == FTS_FP32) {
+    std::unique_ptr<SolverBase> s;
+    if (precision == FTS_FP64) {
+        s.reset(new Solver<double>(g, plan, bc, reduce_memory));
+    } else if (precision == FTS_FP32) {
         FTS_REQUIRE(!bc, "the real-space method runs in FP64");
         FTS_REQUIRE(!g->big, "FP32 propagators need grid sizes whose prime factors are 2, 3 and 5");
-        return std::unique_ptr<SolverBase>(new Solver<float>(g, plan, nullptr, reduce_memory));
+        s.reset(new Solver<float>(g, plan, nullptr, reduce_memory));
+    } else {
+        throw Error("precision must be FTS_FP64 or FTS_FP32");
     }
-    throw Error("precision must be FTS_FP64 or FTS_FP32");
+    const char* graphs = std::getenv("FTS_HIP_GRAPHS");
+    if (graphs && graphs[0] == '1') s->set_graphs(true);
+    return s;
 }
 
 }  // namespace fts
