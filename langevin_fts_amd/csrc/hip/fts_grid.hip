@@ -394,10 +394,11 @@ void Grid::setup_fft() {
 }
 
 // ------------------------------------------------------------------ FFT schedule autotuner
-// Times every launch-shape candidate of the wave-local slab and pencil kernels for this
-// grid and precision on scratch fields (one discrete-chain step of two propagators:
-// slab inverse+forward with exp(-w ds) factors, pencil with the bond factors), and keeps
-// the fastest; a candidate replaces the default only when it is at least 2% faster.
+// Times every pair of launch shapes of the wave-local slab and pencil kernels for this
+// grid and precision on scratch fields (steps of two propagators: slab inverse+forward
+// with the exp(-w ds) factors, then the pencil pass with the bond factors, alternating
+// as in a propagator), and keeps the fastest pair; it replaces the defaults only when
+// at least 2% faster.
 // The choice is cached per (N, precision) for the process. FTS_FFT_TUNE=0 keeps the
 // defaults. Candidates run the same per-line transforms; results agree to rounding
 // (bit-identical at 32^3 and 64^3, where the compiler contracts both alike).
@@ -411,9 +412,9 @@ std::map<std::pair<int, int>, TuneChoice>& tune_cache() {
     return c;
 }
 
-std::string us_text(int shape, float ms) {
+std::string us_text(int bd, int nl, float ms) {
     char buf[48];
-    std::snprintf(buf, sizeof(buf), " %d:%.2fus", shape, 1e3 * ms);
+    std::snprintf(buf, sizeof(buf), " %d/%d:%.2f", bd, nl, 1e3 * ms);
     return buf;
 }
 
@@ -470,33 +471,30 @@ TuneChoice tune_wave_local(const Grid& g) {
     }
     const T* bx = tab.as<T>();
     TuneChoice c{wslab_default_bd(N), WPENCIL_DEFAULT_NL, ""};
-    std::string note = "slab threads";
+    // whole steps (slab pass then pencil pass, as the propagators alternate them)
+    std::string note = "us per step of 2 propagators, slab threads/pencil columns";
     float t_def = 0.f, t_best = 1e30f;
-    int best = c.bd;
+    TuneChoice best = c;
     for (int bd : SLAB_BD_ALL) {
         if (!slab_bd_ok(N, bd)) continue;
-        const float t = time_launches(st, [&] { launch_slab_wl_bd<T, N>(g, bd, SLAB_INV1, sb); });
-        note += us_text(bd, t);
-        if (bd == c.bd) t_def = t;
-        if (t < t_best) {
-            t_best = t;
-            best = bd;
+        for (int nl : PENCIL_NL_ALL) {
+            const float t = time_launches(st, [&] {
+                launch_slab_wl_bd<T, N>(g, bd, SLAB_INV1, sb);
+                launch_xpass_wl_nl<T, N>(g, nl, xb, bx, bx + N, bx + 2 * N);
+            });
+            note += us_text(bd, nl, t);
+            if (bd == c.bd && nl == c.nl) t_def = t;
+            if (t < t_best) {
+                t_best = t;
+                best.bd = bd;
+                best.nl = nl;
+            }
         }
     }
-    if (t_best < 0.98f * t_def) c.bd = best;
-    note += "; pencil columns";
-    t_best = 1e30f;
-    best = c.nl;
-    for (int nl : PENCIL_NL_ALL) {
-        const float t = time_launches(st, [&] { launch_xpass_wl_nl<T, N>(g, nl, xb, bx, bx + N, bx + 2 * N); });
-        note += us_text(nl, t);
-        if (nl == c.nl) t_def = t;
-        if (t < t_best) {
-            t_best = t;
-            best = nl;
-        }
+    if (t_best < 0.98f * t_def) {
+        c.bd = best.bd;
+        c.nl = best.nl;
     }
-    if (t_best < 0.98f * t_def) c.nl = best;
     FTS_HIP_CHECK(hipStreamSynchronize(st));
     c.note = "tuned (" + note + ")";
     return c;
