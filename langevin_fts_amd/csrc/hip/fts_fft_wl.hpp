@@ -175,6 +175,18 @@ constexpr int wslab_pitch(int N, int NLY) {
     return (N == 48 && NLY == 5) ? P + 8 : P;
 }
 
+// Row pitch (complex elements) of the half spectra that the wave-local passes exchange
+// through HBM: nz/2+1 rounded up to a multiple of 8, so that every spectrum row, and
+// with it every 8-column pencil tile, starts on a 128-byte line in FP64 (64 bytes in
+// FP32). The pad columns are never read or written. FTS_SPEC_PAD=0 (a tuning define)
+// keeps nz/2+1.
+#ifndef FTS_SPEC_PAD
+#define FTS_SPEC_PAD 1
+#endif
+constexpr int wspec_pitch(int N) {
+    return FTS_SPEC_PAD ? (N / 2 + 1 + 7) / 8 * 8 : N / 2 + 1;
+}
+
 template <int N, int BD_ = wslab_default_bd(N)>
 struct WSlab {
     static constexpr int NZC = N / 2 + 1;
@@ -184,7 +196,8 @@ struct WSlab {
     static constexpr int NLY = (NZC + NW - 1) / NW;   // y columns per wavefront
     static constexpr int NLZ = (NP + NW - 1) / NW;    // packed z rows per wavefront
     static constexpr int P = wslab_pitch(N, NLY);     // plane row pitch; packed z line p spans rows 2p, 2p+1
-    static constexpr int NSPEC = N * NZC;
+    static constexpr int SP = wspec_pitch(N);         // spectrum row pitch in HBM
+    static constexpr int NSPEC = N * SP;              // one x-slab of the spectrum
     static constexpr size_t lds_elems = (size_t)N * P + 2 * N;
     static_assert(BD % 64 == 0 && BD <= 1024, "whole wavefronts, at most 1024 threads");
     static_assert(NW * NLY >= NZC && NW * NLZ >= NP, "the wavefronts cover every y column and packed z row");
@@ -251,7 +264,7 @@ __global__ void __launch_bounds__(BD) slab_kernel_wl(SlabBatch<T> batch, const t
             if (MA::ok(g, ncol)) {
                 const int c = col0 + MA::line(g), j = MA::idx(g);
 #pragma unroll
-                for (int r = 0; r < G::YI::r(0); r++) v[t][r] = spec[(j + r * MA::B) * NZC + c];
+                for (int r = 0; r < G::YI::r(0); r++) v[t][r] = spec[(j + r * MA::B) * G::SP + c];
             }
         }
         __syncthreads();   // twiddles stored; the plane's previous users are done
@@ -384,7 +397,7 @@ __global__ void __launch_bounds__(BD) slab_kernel_wl(SlabBatch<T> batch, const t
                 if (MO::ok(g, ncol)) {
                     const int c = col0 + MO::line(g), j = MO::idx(g);
 #pragma unroll
-                    for (int r = 0; r < G::YF::r(G::YF::n - 1); r++) put2(&spec[(j + r * MO::B) * NZC + c], o[t][r]);
+                    for (int r = 0; r < G::YF::r(G::YF::n - 1); r++) put2(&spec[(j + r * MO::B) * G::SP + c], o[t][r]);
                 }
             }
         });
@@ -477,8 +490,9 @@ __global__ void __launch_bounds__(64) xpass_kernel_wl(XBatch<T> batch, const typ
     const int j = blockIdx.x / G::TILES;
     const int k0 = (blockIdx.x - j * G::TILES) * NL;
     const int nl = min(NL, NZC - k0);
-    constexpr size_t ROW = (size_t)N * NZC;
-    T2* c = it.c + (size_t)j * NZC + k0;
+    constexpr int SP = wspec_pitch(N);
+    constexpr size_t ROW = (size_t)N * SP;
+    T2* c = it.c + (size_t)j * SP + k0;
     const int table = it.table;
     for (int m = lane; m < N; m += 64) Wx[m] = twx[m];
 

@@ -297,6 +297,8 @@ Grid::Grid(fts_context* c, int d, const int* n, const double* l, const double* m
     }
     nzc = tnx[2] / 2 + 1;
     Mc = tnx[0] * tnx[1] * nzc;
+    spitch = nzc;
+    Ms = (size_t)Mc;
     // ComputationBox.cpp:47-98: dv = prod dx (times mask), volume = prod lx
     dV = 1.0;
     for (int i = 0; i < dim; i++) dV *= dx[i];
@@ -388,6 +390,8 @@ void Grid::setup_fft() {
     wave_local = ct_n != 0 && !std::getenv("FTS_CT_KERNELS");
     if (ct_n) set_lds_attributes_ct();
     if (wave_local) {
+        spitch = wspec_pitch(ct_n);
+        Ms = (size_t)tnx[0] * tnx[1] * spitch;
         tune_fft_schedule<double>();
         tune_fft_schedule<float>();
     }
@@ -444,7 +448,7 @@ template <typename T, int N>
 TuneChoice tune_wave_local(const Grid& g) {
     using T2 = typename vec2<T>::type;
     constexpr int NZC = N / 2 + 1, ITEMS = 2;
-    const size_t M = (size_t)N * N * N, Mc = (size_t)N * N * NZC;
+    const size_t M = (size_t)N * N * N, Mc = (size_t)N * N * wspec_pitch(N);
     DeviceBuffer spec, q, w, tab;
     spec.allocate(sizeof(T2) * Mc * ITEMS);
     q.allocate(sizeof(T) * M * ITEMS);
@@ -710,21 +714,32 @@ double Grid::integral(const double* a, bool on_device) {
 template <typename T>
 void Grid::fft_forward(const T* in, typename vec2<T>::type* out, int batch) {
     require_fft();
+    using T2 = typename vec2<T>::type;
+    // padded rows: transform into a workspace of the padded layout, then copy the rows out
+    const bool padded = spitch != nzc;
+    DeviceBuffer tmp;
+    if (padded) tmp.allocate(sizeof(T2) * Ms * std::min(batch, MAX_BATCH));
     for (int b0 = 0; b0 < batch; b0 += MAX_BATCH) {
         const int nb = std::min(MAX_BATCH, batch - b0);
         SlabBatch<T> sb{};
         XBatch<T> xb{};
         sb.n = xb.n = nb;
         for (int b = 0; b < nb; b++) {
+            T2* spec = padded ? tmp.as<T2>() + (size_t)b * Ms : out + (size_t)(b0 + b) * Mc;
             SlabItem<T>& it = sb.it[b];
             it.qin = in + (size_t)(b0 + b) * M;
-            it.c0 = out + (size_t)(b0 + b) * Mc;
+            it.c0 = spec;
             it.n_out = 1;
-            xb.it[b] = XItem<T>{out + (size_t)(b0 + b) * Mc, -1};
+            xb.it[b] = XItem<T>{spec, -1};
         }
         launch_slab<T>(SLAB_FWD, sb);
         launch_xpass<T>(xb, nullptr, nullptr, nullptr);
+        for (int b = 0; padded && b < nb; b++)
+            FTS_HIP_CHECK(hipMemcpy2DAsync(out + (size_t)(b0 + b) * Mc, sizeof(T2) * nzc, tmp.as<T2>() + (size_t)b * Ms,
+                                           sizeof(T2) * spitch, sizeof(T2) * nzc, (size_t)tnx[0] * tnx[1],
+                                           hipMemcpyDeviceToDevice, stream()));
     }
+    if (padded) FTS_HIP_CHECK(hipStreamSynchronize(stream()));
 }
 
 template <typename T>
@@ -733,18 +748,20 @@ void Grid::fft_backward(const typename vec2<T>::type* in, T* out, int batch) {
     // the pencil pass transforms in place, so the input is first copied to out's
     // companion workspace: use a temporary spectrum buffer
     using T2 = typename vec2<T>::type;
-    DeviceBuffer tmp(sizeof(T2) * (size_t)Mc * std::min(batch, MAX_BATCH));
+    DeviceBuffer tmp(sizeof(T2) * Ms * std::min(batch, MAX_BATCH));
     for (int b0 = 0; b0 < batch; b0 += MAX_BATCH) {
         const int nb = std::min(MAX_BATCH, batch - b0);
-        FTS_HIP_CHECK(hipMemcpyAsync(tmp.get(), in + (size_t)b0 * Mc, sizeof(T2) * (size_t)Mc * nb,
-                                     hipMemcpyDeviceToDevice, stream()));
+        // the caller's rows of nzc values into the workspace's rows of spitch
+        FTS_HIP_CHECK(hipMemcpy2DAsync(tmp.get(), sizeof(T2) * spitch, in + (size_t)b0 * Mc, sizeof(T2) * nzc,
+                                       sizeof(T2) * nzc, (size_t)tnx[0] * tnx[1] * nb, hipMemcpyDeviceToDevice,
+                                       stream()));
         SlabBatch<T> sb{};
         XBatch<T> xb{};
         sb.n = xb.n = nb;
         for (int b = 0; b < nb; b++) {
-            xb.it[b] = XItem<T>{tmp.as<T2>() + (size_t)b * Mc, -2};
+            xb.it[b] = XItem<T>{tmp.as<T2>() + (size_t)b * Ms, -2};
             SlabItem<T>& it = sb.it[b];
-            it.c0 = tmp.as<T2>() + (size_t)b * Mc;
+            it.c0 = tmp.as<T2>() + (size_t)b * Ms;
             it.qout = out + (size_t)(b0 + b) * M;
         }
         launch_xpass<T>(xb, nullptr, nullptr, nullptr);

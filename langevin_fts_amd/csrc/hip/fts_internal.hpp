@@ -89,6 +89,10 @@ public:
     int tnx[3];     // 3D-mapped sizes (Pseudo.cpp:31-45)
     double tdx[3];
     int M, nzc, Mc;
+    // row pitch and per-field size (complex elements) of the half spectra in device
+    // buffers: nzc and Mc, or the padded pitch of the wave-local kernels
+    int spitch = 0;
+    size_t Ms = 0;
     double volume;
     double dV;
     std::vector<double> mask_h;

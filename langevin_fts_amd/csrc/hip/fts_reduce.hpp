@@ -237,6 +237,7 @@ struct StressBatch {
 
 struct StressGeom {
     int n0, n1, nz, nzc;
+    int sp;              // row pitch of the spectra (nzc, or padded)
     double f0, f1, f2;   // (2 pi / L_d)^2 on the 3D-mapped axes
 };
 
@@ -254,8 +255,9 @@ __global__ void __launch_bounds__(RED_THREADS) stress_stage1(StressBatch<T> b, S
         const int i = r / g.n1;
         const int it0 = i > g.n0 / 2 ? g.n0 - i : i;
         const int jt = j > g.n1 / 2 ? g.n1 - j : j;
-        const auto A = it.a[f];
-        const auto B = it.b[f];
+        const size_t e = (size_t)r * g.sp + k;
+        const auto A = it.a[e];
+        const auto B = it.b[e];
         double coeff = it.bsq * ((double)A.x * (double)B.x + (double)A.y * (double)B.y);
         if (it.table >= 0)
             coeff *= (double)bx[(size_t)it.table * g.n0 + i] * (double)by[(size_t)it.table * g.n1 + j] *

@@ -85,7 +85,7 @@ private:
     bool table() const { return model_ == FTS_DISCRETE && std::is_same<T, double>::value; }
     const T* ew(int m) const { return table() ? ew_.as<T>() + (size_t)m * M_ : w(m); }
     T ew_coef() const { return table() ? (T)-1 : (T)ds_; }   // negative: the array is the factor
-    T2* slot(int s) const { return ws_.as<T2>() + (size_t)s * Mc_; }
+    T2* slot(int s) const { return ws_.as<T2>() + (size_t)s * Ms_; }
     const T* mask() const { return g_->mask_h.empty() ? nullptr : g_->mask<T>(); }
     int full(int m) const { return 2 * m; }
     int half(int m) const { return 2 * m + 1; }
@@ -124,7 +124,8 @@ private:
     Grid* g_;
     int model_;
     double ds_;
-    int M_, Mc_;
+    int M_;
+    size_t Ms_;   // spectrum workspace slot size (the grid's padded layout)
     int n_mon_, n_qinit_;
     std::vector<double> bsq_;
     std::vector<Prop> props_;
@@ -180,7 +181,7 @@ Solver<T>::Solver(Grid* grid, const fts_plan* pl, const int* bc, bool reduce_mem
     }
     ds_ = pl->ds;
     M_ = g_->M;
-    Mc_ = g_->Mc;
+    Ms_ = g_->Ms;
     n_mon_ = pl->n_monomers;
     n_qinit_ = pl->n_qinit;
     FTS_REQUIRE(model_ == FTS_CONTINUOUS || model_ == FTS_DISCRETE, "invalid chain model");
@@ -256,7 +257,7 @@ Solver<T>::Solver(Grid* grid, const fts_plan* pl, const int* bc, bool reduce_mem
     if (table()) ew_.allocate(sizeof(T) * n_mon_ * (size_t)M_);
     if (n_qinit_) qinit_.allocate(sizeof(double) * n_qinit_ * (size_t)M_);
     n_ws_ = std::max<int>(2 * (int)jmax, 16);
-    ws_.allocate(sizeof(T2) * (size_t)n_ws_ * Mc_);
+    ws_.allocate(sizeof(T2) * (size_t)n_ws_ * Ms_);
     phib_.allocate(sizeof(double) * blocks_.size() * (size_t)M_);
     if (!svf_.empty()) phis_.allocate(sizeof(double) * svf_.size() * (size_t)M_);
     Qd_.allocate(sizeof(double) * (vf_.size() + svf_.size() + 1));
@@ -1010,6 +1011,7 @@ void Solver<T>::compute_stress() {
     sg.n1 = g_->tnx[1];
     sg.nz = g_->tnx[2];
     sg.nzc = g_->nzc;
+    sg.sp = g_->spitch;
     const double PI = 3.14159265358979323846;
     sg.f0 = std::pow(2 * PI / (g_->tnx[0] * g_->tdx[0]), 2);
     sg.f1 = std::pow(2 * PI / (g_->tnx[1] * g_->tdx[1]), 2);
