@@ -146,13 +146,6 @@ constexpr int PENCIL_NL_ALL[] = {4, 8, 16};
 
 template <typename T, int N, int BD>
 void launch_slab_wl(const Grid& g, int mode, const SlabBatch<T>& b) {
-    static const bool attr = [] {
-        allow_lds(&slab_kernel_wl<T, SLAB_FWD, N, BD>);
-        allow_lds(&slab_kernel_wl<T, SLAB_INV1, N, BD>);
-        allow_lds(&slab_kernel_wl<T, SLAB_INV2, N, BD>);
-        return true;
-    }();
-    (void)attr;
     using G = WSlab<N, BD>;
     const dim3 grid(N, b.n), block(BD);
     const size_t lds = G::lds_elems * sizeof(typename vec2<T>::type);
@@ -184,11 +177,6 @@ void launch_slab_wl_bd(const Grid& g, int bd, int mode, const SlabBatch<T>& b) {
 template <typename T, int N, int NL>
 void launch_xpass_wl(const Grid& g, const XBatch<T>& b, const T* bx, const T* by, const T* bz) {
     using G = WPencil<N, NL>;
-    static const bool attr = [] {
-        allow_lds(&xpass_kernel_wl<T, N, NL>);
-        return true;
-    }();
-    (void)attr;
     const dim3 grid(N * G::TILES, b.n), block(G::BD);
     hipLaunchKernelGGL((xpass_kernel_wl<T, N, NL>), grid, block, G::lds_elems * sizeof(typename vec2<T>::type),
                        g.stream(), b, g.tw<T>(0), bx, by, bz);
@@ -210,14 +198,45 @@ constexpr int prec_index() {
     return std::is_same<T, float>::value ? 1 : 0;
 }
 
+// LDS limits of every wave-local variant, set before any launch (and so before any
+// graph capture could record the first one)
+template <typename T, int N, int BD>
+void allow_lds_slab_wl() {
+    if constexpr (slab_bd_ok(N, BD)) {
+        allow_lds(&slab_kernel_wl<T, SLAB_FWD, N, BD>);
+        allow_lds(&slab_kernel_wl<T, SLAB_INV1, N, BD>);
+        allow_lds(&slab_kernel_wl<T, SLAB_INV2, N, BD>);
+    }
+}
+
+template <typename T, int N>
+void allow_lds_wl() {
+    allow_lds_slab_wl<T, N, 128>();
+    allow_lds_slab_wl<T, N, 256>();
+    allow_lds_slab_wl<T, N, 320>();
+    allow_lds_slab_wl<T, N, 384>();
+    allow_lds_slab_wl<T, N, 512>();
+    allow_lds_slab_wl<T, N, 1024>();
+    allow_lds(&xpass_kernel_wl<T, N, 4>);
+    allow_lds(&xpass_kernel_wl<T, N, 8>);
+    allow_lds(&xpass_kernel_wl<T, N, 16>);
+}
+
+template <int N>
+void allow_lds_n() {
+    allow_lds_ct<N>();
+    allow_lds_wl<double, N>();
+    allow_lds_wl<float, N>();
+}
+
 void set_lds_attributes_ct() {
     static bool done = false;
     if (done) return;
-    allow_lds_ct<32>();
-    allow_lds_ct<48>();
-    allow_lds_ct<64>();
-    allow_lds_ct<96>();
-    allow_lds_ct<128>();
+    allow_lds_n<32>();
+    allow_lds_n<48>();
+    allow_lds_n<64>();
+    allow_lds_n<96>();
+    allow_lds_n<128>();
     done = true;
 }
 
