@@ -5,6 +5,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <map>
+#include <mutex>
 #include <string>
 
 #include "fts_internal.hpp"
@@ -434,6 +435,10 @@ std::map<std::pair<int, int>, TuneChoice>& tune_cache() {
     static std::map<std::pair<int, int>, TuneChoice> c;
     return c;
 }
+std::mutex& tune_mutex() {   // grids created on several threads tune one (N, precision) once
+    static std::mutex m;
+    return m;
+}
 
 std::string us_text(int bd, int nl, float ms) {
     char buf[48];
@@ -534,6 +539,7 @@ void Grid::tune_fft_schedule() {
         fft_schedule[p] = "defaults";
         return;
     }
+    std::lock_guard<std::mutex> lock(tune_mutex());
     auto& cache = tune_cache();
     const auto key = std::make_pair(ct_n, p);
     auto hit = cache.find(key);
