@@ -81,7 +81,7 @@ private:
     T* slice_ptr(int p, int n, int half) const { return half ? h(p, n) : q(p, n); }
     const T* w(int m) const { return wT_ + (size_t)m * M_; }
     // discrete FP64 chains: exp(-w ds) of monomer m, made once per compute_propagators (the FP32
-    // kernels compute their single-precision exponential on the fly, which costs less than the table)
+    // kernels compute their single-precision exponential on the fly: a table measures the same there)
     bool table() const { return model_ == FTS_DISCRETE && std::is_same<T, double>::value; }
     const T* ew(int m) const { return table() ? ew_.as<T>() + (size_t)m * M_ : w(m); }
     T ew_coef() const { return table() ? (T)-1 : (T)ds_; }   // negative: the array is the factor
