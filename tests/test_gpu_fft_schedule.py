@@ -99,3 +99,20 @@ def test_schedule_errors(capi, ctx):
     assert h.fft_schedule()[:2] == (0, 0)
     with pytest.raises(RuntimeError, match="wave-local"):
         h.set_fft_schedule(256, 8)
+
+
+def test_tuning_off_keeps_defaults(capi, ctx):
+    """FTS_FFT_TUNE=0 at grid creation: the default shapes, no timing"""
+    import os
+
+    defaults = {32: 256, 48: 384, 64: 320, 96: 1024, 128: 1024}
+    os.environ["FTS_FFT_TUNE"] = "0"
+    try:
+        for n in (32, 64):
+            g = capi.Grid(ctx, [n] * 3, [4.0] * 3)
+            for precision in (capi.FTS_FP64, capi.FTS_FP32):
+                bd, nl, note = g.fft_schedule(precision)
+                assert (bd, nl) == (defaults[n], 8)
+                assert note == "defaults"
+    finally:
+        os.environ.pop("FTS_FFT_TUNE", None)
