@@ -100,3 +100,32 @@ def test_noise_stream_keeps_the_draw_order():
     got += [stream.next() for _ in range(3)]
     for a, b in zip(got, expected):
         np.testing.assert_array_equal(a, b)
+
+
+def test_guess_learning_rate_falls_on_a_cosine():
+    """LFTS.train_guess passes lr at the first training step, (lr + lr_final)/2 half-way and lr_final at
+    the last, on a cosine (checked with a stand-in for the device object)"""
+    import types
+
+    from langevin_fts_amd.lfts import LFTS
+
+    seen = []
+
+    class Device:
+        def train_guess(self, guess, lr, iters, replay):
+            seen.append(lr)
+            return 0.0
+
+    T = 100
+    sim = types.SimpleNamespace(guess=object(), guess_trained_steps=0, saddle_device=Device(),
+                                learned={"lr": 1e-3, "lr_final": 1e-4, "train_steps": T, "train_iters": 16,
+                                         "replay": 64})
+    for _ in range(T + 3):
+        LFTS.train_guess(sim)
+    assert seen[0] == pytest.approx(1e-3, rel=1e-12)
+    assert seen[T // 2] == pytest.approx(0.5 * (1e-3 + 1e-4), rel=1e-12)
+    assert seen[T] == pytest.approx(1e-4, rel=1e-12)
+    assert seen[T + 2] == pytest.approx(1e-4, rel=1e-12)   # held after the training window
+    assert all(a >= b for a, b in zip(seen, seen[1:]))
+    k = 30
+    assert seen[k] == pytest.approx(1e-4 + 0.5 * 9e-4 * (1 + np.cos(np.pi * k / T)), rel=1e-12)
