@@ -20,11 +20,14 @@ steps run untimed. Then, with the same tolerance:
                point: FP32 propagators while the error is >= tol, FP64 after; only
                FP64 evaluations can declare convergence, so every saddle point
                meets the same FP64 tolerance;
-  learned      K steps with both (with --precision fp64: the CNN alone).
+  learned      K steps with both (skipped with --precision fp64).
 
-`value` is the last arm's Langevin steps/s; config carries every arm's rate and
-iterations per step, the ratio to the plain arm and the rate of a 10,000-step run
-that trains on its first steps.
+`value` is the learned-guess arm with FP64 saddle iterations (the reference's
+precision); config carries every arm's rate and iterations per step, the ratio
+to the plain arm, the mixed-precision arm with its largest (FP64) saddle error,
+the rate of a 10,000-step run that trains on its first steps, and the agreement
+of the structure functions S_ij(k) (lfts.py:753-766) of the plain and learned
+FP64 arms run from one state with one noise sequence (untimed, --sk-steps).
 
 Printed: one JSON line. `roofline` is the slab pass (the dominant kernel:
 inverse y/z FFT, exp(-w ds) factor, q store, forward z/y FFT), timed during the
@@ -33,8 +36,10 @@ stream; its algorithmic bytes are the spectra, fields and w each launch must mov
 once. `traffic` is the slab launch's HBM bytes from two `rocprofv3 --pmc` child
 runs (FETCH_SIZE, WRITE_SIZE) made before this process touches the GPU, each
 scaled by what the same counter reads for a plain device copy of 256 MiB.
-`cpu_baseline` times the CPU oracle (FP64, OpenMP FFT lines) on one saddle
-iteration of the same 64^3 workload and scales by the GPU run's iterations per step.
+`cpu_baseline` times saddle iterations of the same 64^3 workload on the CPU in
+the reference CPU path's structure (oracle/cpu_pocketfft.py: a slice's propagators
+as parallel jobs, each with sequential pocketfft FFTs), over the whole contour, and
+scales by the GPU run's iterations per step.
 """
 import argparse
 import contextlib
@@ -42,7 +47,6 @@ import csv
 import glob
 import itertools
 import json
-import math
 import os
 import re
 import shutil
@@ -147,30 +151,108 @@ def langevin_steps(sim, n, stats, learned=False):
                 dev.restore()
         stats["iterations"] += it
         stats["failures"] += 0 if ok else 1
+        if ok:
+            stats["max_error"] = max(stats.get("max_error", 0.0), float(err))
+
+
+def baseline_plan(w, ds=None):
+    """The propagator plan of the workload's molecules from the product's analyzer
+    (host code only), in the oracle's plan form."""
+    from langevin_fts_amd import langevinfts as lf
+
+    bonds = w.get("bonds", {"A": 1.0, "B": 1.0})
+    polymers = w.get("polymers", [[1.0, [["A", w.get("f", 0.5), 0, 1], ["B", 1 - w.get("f", 0.5), 1, 2]]]])
+    factory = lf.PlatformSelector.create_factory("hip", False)
+    mol = factory.create_molecules_information(w["model"], ds or w["ds"], bonds)
+    for vf, blocks in polymers:
+        mol.add_polymer(vf, blocks)
+    an = factory.create_propagator_analyzer(mol, True)
+    return plan_dict(mol, an, w["model"]), bonds
+
+
+def baseline_solver(w):
+    """The CPU baseline in the reference CPU path's structure: the propagators of a
+    slice as parallel jobs, each with its own sequential FFTs (pocketfft)."""
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from oracle import cpu_pocketfft as P
+
+    plan, bonds = baseline_plan(w)
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    solver = P.PocketSolver(w["nx"], w["lx"], plan, w["ds"], bonds, threads=threads)
+    jobs = max(len(level) for level in solver.levels)
+    return solver, bonds, min(threads, max(jobs, len(plan["blocks"])))
+
+
+def baseline_note(solver, cores):
+    return (f"FP64, reference CPU structure: the propagators of a schedule slice as parallel jobs (at most "
+            f"{max(len(lv) for lv in solver.levels)} at once here), each job's FFTs sequential (scipy.fft pocketfft, "
+            f"1 worker), block concentrations in parallel; thread pool of {solver.threads} on the host, "
+            f"{cores} busy at most; {os.cpu_count()} host cores visible")
+
+
+def structure_function_agreement(sim, n_steps, seed):
+    """Plain and learned-guess FP64 saddle points from the same state with the same
+    noise sequence: each arm accumulates the structure functions S_ij(k) of
+    lfts.py:753-766 over n_steps Langevin steps. Reports the relative L2 difference of
+    every S_ij(k) (k != 0) between the arms, of their spherical averages, and the
+    largest field difference at the end."""
+    dev = sim.saddle_device
+    R = len(sim.mpt.aux_fields_real_idx)
+    M = sim.cb.get_n_grid()
+    w0, n0 = dev.get_fields(), dev.get_noise_prev()
+    g = np.random.Generator(np.random.PCG64(seed + 1))
+    noises = [g.normal(0.0, sim.langevin["sigma"], [R, M]) for _ in range(n_steps)]
+    acc, fields, iters = {}, {}, {}
+    for arm in ("plain", "learned"):
+        dev.set_fields(np.ascontiguousarray(w0))
+        dev.set_noise_prev(np.ascontiguousarray(n0))
+        total, its = None, 0
+        for noise in noises:
+            if arm == "learned":
+                it, err, ok = sim.learned_step(noise)
+            else:
+                dev.backup()
+                dev.langevin_update(noise, sim.langevin["dt"])
+                it, err, _ = sim._find_saddle_point_device()
+                ok = not (np.isnan(err) or err >= sim.saddle["tolerance"])
+            if not ok:
+                raise RuntimeError("a saddle point of the agreement pass missed the tolerance")
+            its += it
+            terms = sim.structure_function_terms(dev.get_fields(), sim._phi_from_device())
+            total = terms if total is None else {k: total[k] + v for k, v in terms.items()}
+        acc[arm], fields[arm], iters[arm] = total, dev.get_fields(), its / n_steps
+    nx = sim.cb.get_nx()
+    k2 = sum(np.meshgrid(*[np.minimum(np.arange(n), n - np.arange(n)) ** 2 if d < len(nx) - 1
+                           else np.arange(n // 2 + 1) ** 2 for d, n in enumerate(nx)], indexing="ij"))
+    shell = np.rint(np.sqrt(k2)).astype(int).ravel()
+    out = {"steps": n_steps, "saddle_iterations_per_step": iters, "pairs": {}}
+    for key in acc["plain"]:
+        a, b = acc["plain"][key].ravel().copy(), acc["learned"][key].ravel().copy()
+        a[0] = b[0] = 0.0
+        sa = np.bincount(shell, weights=a.real) / np.maximum(np.bincount(shell), 1)
+        sb = np.bincount(shell, weights=b.real) / np.maximum(np.bincount(shell), 1)
+        out["pairs"][key] = {"rel_l2": float(np.linalg.norm(b - a) / np.linalg.norm(a)),
+                             "spherical_rel_l2": float(np.linalg.norm(sb[1:] - sa[1:]) / np.linalg.norm(sa[1:]))}
+    out["max_field_difference"] = float(np.max(np.abs(fields["learned"] - fields["plain"])))
+    return out
 
 
 def cpu_baseline(w, iterations_per_step, budget_s):
-    """FP64 oracle on one or more saddle iterations of the same workload."""
-    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    """Saddle iterations of the same workload on the CPU baseline (propagators,
+    concentrations, residual and the oracle's Anderson mixing), timed over the
+    real contour, times the GPU run's iterations per Langevin step."""
     from oracle import oracle as O
-    from langevin_fts_amd import langevinfts as lf
 
-    # the propagator plan of this molecule from the product's analyzer (host code only)
-    factory = lf.PlatformSelector.create_factory("hip", False)
-    mol = factory.create_molecules_information(w["model"], w["ds"], {"A": 1.0, "B": 1.0})
-    mol.add_polymer(1.0, [["A", w["f"], 0, 1], ["B", 1 - w["f"], 1, 2]])
-    an = factory.create_propagator_analyzer(mol, True)
-    plan = plan_dict(mol, an, w["model"])
+    solver, _, cores = baseline_solver(w)
     M = int(np.prod(w["nx"]))
-    grid = O.Grid(w["nx"], w["lx"])
-    solver = O.Solver(grid, plan, w["ds"], {"A": 1.0, "B": 1.0})
     am = O.AndersonMixing(M, *w["am"])
     rng = np.random.default_rng(1)
     w_aux = np.stack([rng.standard_normal(M), rng.standard_normal(M)])
     n, t0 = 0, time.perf_counter()
     err = 1e20
     while True:
-        solver.compute_statistics({"A": w_aux[0] + w_aux[1], "B": -w_aux[0] + w_aux[1]})
+        solver.compute_propagators({"A": w_aux[0] + w_aux[1], "B": -w_aux[0] + w_aux[1]})
+        solver.compute_concentrations()
         h = -(solver.get_total_concentration("A") + solver.get_total_concentration("B") - 1.0)
         old, err = err, np.std(h)
         w_aux[1] = am.calculate_new_fields(w_aux[1], -h, old, err)
@@ -178,11 +260,10 @@ def cpu_baseline(w, iterations_per_step, budget_s):
         if time.perf_counter() - t0 > budget_s:
             break
     t_iter = (time.perf_counter() - t0) / n
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": 1.0 / (t_iter * iterations_per_step), "unit": "steps/s", "cores": threads, "kind": "port",
-            "sample": f"{n} saddle iteration(s) of the same 64^3 workload on the FP64 C oracle "
-                      f"({t_iter:.2f} s each, FFT lines over {threads} OpenMP threads) x "
-                      f"{iterations_per_step:.1f} iterations per Langevin step measured on the GPU"}
+    return {"value": 1.0 / (t_iter * iterations_per_step), "unit": "steps/s", "cores": cores, "kind": "port",
+            "sample": f"{n} saddle iteration(s) of the same {w['nx'][0]}^3 workload, whole contour "
+                      f"({t_iter:.2f} s each) x {iterations_per_step:.1f} iterations per Langevin step measured "
+                      f"on the GPU; " + baseline_note(solver, cores)}
 
 
 def plan_dict(mol, an, model):
@@ -539,48 +620,28 @@ def make_solver(lf, platform, w, reduce=False):
     return factory, cb, mol, an, factory.create_pseudospectral_solver(cb, mol, an)
 
 
-def oracle_eval_seconds(w, budget_s, with_stress=True):
-    """FP64 oracle time of one evaluation (propagators, concentrations, stress) of the
-    workload: the same polymers with an up to 10x coarser contour time the per-step cost,
-    which is then scaled by the real number of steps (the step cost does not depend
-    on ds)."""
-    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
-    from oracle import oracle as O
-    from langevin_fts_amd import langevinfts as lf
-
-    def plan_for(ds):
-        factory = lf.PlatformSelector.create_factory("hip", False)
-        mol = factory.create_molecules_information(w["model"], ds, w["bonds"])
-        for vf, blocks in w["polymers"]:
-            mol.add_polymer(vf, blocks)
-        an = factory.create_propagator_analyzer(mol, True)
-        steps = sum(e.max_n_segment for e in an.get_computation_propagators().values())
-        return plan_dict(mol, an, w["model"]), steps
-
-    _, steps_full = plan_for(w["ds"])
-    # the coarsening keeps every block an integer number of segments
-    seg = [int(round(b[1] / w["ds"])) for _, blocks in w["polymers"] for b in blocks]
-    g = 0
-    for n in seg:
-        g = math.gcd(g, n)
-    k = max(d for d in range(1, 11) if g % d == 0)
-    plan, steps = plan_for(w["ds"] * k)
+def baseline_eval_seconds(w, budget_s, with_stress=True):
+    """CPU baseline time of one evaluation (propagators, concentrations and, for
+    continuous chains, stress) over the real contour: at least one evaluation, more
+    while the budget lasts."""
+    solver, bonds, cores = baseline_solver(w)
     M = int(np.prod(w["nx"]))
     rng = np.random.default_rng(0)
-    fields = {m: w.get("sigma", 1.0) * rng.standard_normal(M) for m in w["bonds"]}
-    solver = O.Solver(O.Grid(w["nx"], w["lx"]), plan, w["ds"] * k, w["bonds"])
+    fields = {m: w.get("sigma", 1.0) * rng.standard_normal(M) for m in bonds}
+    stress = with_stress and w["model"] == "continuous"
     n, t0 = 0, time.perf_counter()
     while True:
-        solver.compute_statistics(fields)
-        if with_stress:
+        solver.compute_propagators(fields)
+        solver.compute_concentrations()
+        if stress:
             solver.compute_stress()
         n += 1
         if time.perf_counter() - t0 > budget_s:
             break
-    t_step = (time.perf_counter() - t0) / (n * steps)
-    return t_step * steps_full, f"{n * steps} oracle steps of the same grid and polymers at {k}x ds ({1e3 * t_step:.1f} ms " \
-                                f"each, FFT lines over {int(os.environ.get('OMP_NUM_THREADS', os.cpu_count() or 1))} " \
-                                f"OpenMP threads) x {steps_full} steps per evaluation"
+    t = (time.perf_counter() - t0) / n
+    what = "propagators + concentrations" + (" + stress" if stress else " (no stress)")
+    return t, cores, f"{n} evaluation(s) ({what}) of the same grid and polymers, whole contour, {t:.2f} s each; " \
+                     + baseline_note(solver, cores)
 
 
 def eval_bench(args, w, lf):
@@ -636,9 +697,8 @@ def eval_bench(args, w, lf):
                            "bytes_per_launch": kern["bytes"] / kern["launches"], "timed_launches": f"1 in {SAMPLE}"}
     if args.cpu_seconds > 0:
         with stdout_to_stderr():
-            t, sample = oracle_eval_seconds(w, args.cpu_seconds)
-        out["cpu_baseline"] = {"value": 1.0 / t, "unit": "evaluations/s",
-                               "cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)), "kind": "port",
+            t, cores, sample = baseline_eval_seconds(w, args.cpu_seconds)
+        out["cpu_baseline"] = {"value": 1.0 / t, "unit": "evaluations/s", "cores": cores, "kind": "port",
                                "sample": sample}
     print(json.dumps(out), flush=True)
 
@@ -696,9 +756,8 @@ def scft_bench(args, w, lf):
         wl = dict(w, bonds={"A": 1.0, "B": 1.0}, polymers=[[1.0, [["A", w["f"], 0, 1], ["B", 1 - w["f"], 1, 2]]]],
                   sigma=1.0)
         with stdout_to_stderr():
-            tt, sample = oracle_eval_seconds(wl, args.cpu_seconds)
-        out["cpu_baseline"] = {"value": 1.0 / tt, "unit": "iterations/s",
-                               "cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)), "kind": "port",
+            tt, cores, sample = baseline_eval_seconds(wl, args.cpu_seconds)
+        out["cpu_baseline"] = {"value": 1.0 / tt, "unit": "iterations/s", "cores": cores, "kind": "port",
                                "sample": sample + " (one SCFT iteration = one evaluation; the host-side mixing of "
                                                   "[w, L] is not counted)"}
     print(json.dumps(out), flush=True)
@@ -723,7 +782,12 @@ def main():
     ap.add_argument("--guess-train-iters", type=int, default=16, help="Adam iterations per training step")
     ap.add_argument("--guess-lr", type=float, default=1e-3, help="initial Adam learning rate of the guess")
     ap.add_argument("--guess-lr-final", type=float, default=1e-4, help="learning rate at the end of training")
-    ap.add_argument("--guess-replay", type=int, default=64, help="stored (w-, w+) pairs replayed in training")
+    ap.add_argument("--guess-replay", type=int, default=64, help="stored training samples replayed in training")
+    ap.add_argument("--guess-inputs", default="state", choices=["state", "delta", "absolute"],
+                    help="state: the CNN sees w-, its Langevin change and the previous w+ and predicts the change "
+                         "of w+; delta: w+ += f(w-_new) - f(w-_old)")
+    ap.add_argument("--sk-steps", type=int, default=20,
+                    help="steps of the untimed structure-function agreement pass (0: skip)")
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"],
                     help="mixed: saddle iterations on FP32 propagators until the error < tol, then FP64")
     args = ap.parse_args()
@@ -766,7 +830,7 @@ def main():
     params["saddle"]["mixed_precision"] = mixed
     params["learned_guess"] = {"train_steps": args.guess_train_steps, "lr": args.guess_lr,
                                "lr_final": args.guess_lr_final, "train_iters": args.guess_train_iters,
-                               "replay": args.guess_replay, "delta": True, "seed": 0}
+                               "replay": args.guess_replay, "inputs": args.guess_inputs, "seed": 0}
     with stdout_to_stderr():
         sim = lfts.LFTS(params, random_seed=args.seed)
         tol = sim.saddle["tolerance"]
@@ -819,8 +883,8 @@ def main():
             learned64 = arm(True, False)    # saddle points from the CNN's prediction, FP64 iterations
         if mixed:
             plain_mixed = arm(False, True)  # the reference's start, FP32 iterations until err < tol
-        if args.guess == "learned":
-            learned = arm(True, mixed)
+        if args.guess == "learned" and mixed:
+            learned = arm(True, True)       # both: the CNN's start and FP32 iterations until err < tol
             learned_s = learned["s"]
 
         # roofline pass: plain FP64 steps with one launch in SAMPLE timed by HIP events
@@ -837,10 +901,15 @@ def main():
             kern = {f: lf.get_kernel_timing(f) for f in ("slab", "pencil")}
             lf.set_kernel_timing(0)
 
-    if learned is None and plain_mixed is not None:
-        learned, learned_s = plain_mixed, plain_mixed["s"]
-    stats = learned if learned is not None else plain
-    elapsed = learned_s if learned is not None else plain_s
+        # structure-function agreement (untimed): from one state and with one noise
+        # sequence, the plain and learned-guess FP64 arms each run n_sf steps
+        sk = None
+        if args.guess == "learned" and args.sk_steps > 0:
+            sk = structure_function_agreement(sim, args.sk_steps, args.seed)
+
+    # value: the learned guess with FP64 saddle iterations (the reference's precision)
+    stats = learned64 if learned64 is not None else plain
+    elapsed = learned64["s"] if learned64 is not None else plain_s
     steps_per_s = args.steps / elapsed
     iters = stats["iterations"] / args.steps
     plain_iters = plain["iterations"] / args.steps
@@ -849,14 +918,13 @@ def main():
         "value": steps_per_s, "unit": "steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": 1e3 * elapsed / args.steps, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 propagators until err < tol, then f64; f64 Q/phi/AM" if mixed else "f64",
+        "dtype": "f64",
         "data": "synthetic: w_A, w_B ~ N(0,1) from PCG64(seed); Langevin noise from numpy PCG64",
         "config": {"workload": describe(args.workload, w),
-                   "guess": ("learned CNN (delta form) + Anderson mixing" if args.guess == "learned"
-                             else "plain Anderson mixing")
-                            + ("; FP32 saddle iterations until the error < tol, FP64 after (only FP64 "
-                               "evaluations declare convergence)" if mixed else ""),
+                   "guess": (f"learned CNN ({args.guess_inputs} form) + Anderson mixing, FP64 saddle iterations"
+                             if args.guess == "learned" else "plain Anderson mixing, FP64"),
                    "saddle_iterations_per_step": iters, "saddle_failures": stats["failures"],
+                   "max_saddle_error": stats.get("max_error"),
                    "plain_steps_per_s": args.steps / plain_s, "plain_saddle_iterations_per_step": plain_iters,
                    "plain_saddle_failures": plain["failures"],
                    "propagator_evaluations_per_s": plain["iterations"] / plain_s},
@@ -868,15 +936,25 @@ def main():
     if plain_mixed is not None:
         out["config"]["plain_mixed_steps_per_s"] = args.steps / plain_mixed["s"]
         out["config"]["plain_mixed_saddle_iterations_per_step"] = plain_mixed["iterations"] / args.steps
+    if learned is not None and mixed:
+        # FP32 propagators while the error >= tol; the converging iteration (and the
+        # last permitted one) is always FP64, so max_saddle_error is an FP64 error
+        out["config"]["mixed_precision_arm"] = {
+            "steps_per_s": args.steps / learned_s, "saddle_iterations_per_step": learned["iterations"] / args.steps,
+            "saddle_failures": learned["failures"], "max_saddle_error": learned.get("max_error"),
+            "vs_plain_fp64": (args.steps / learned_s) / (args.steps / plain_s),
+            "dtype": "f32 propagators until err < tol, then f64; f64 Q/phi/AM"}
+    if sk is not None:
+        out["config"]["structure_function_agreement"] = sk
     if args.guess == "learned":
         n_train, n_run = args.guess_train_steps, 10000
         train_step_s = train_s / max(n_train, 1)
         out["config"].update({
-            "learned_vs_plain": (args.steps / learned_s) / (args.steps / plain_s),
+            "learned_vs_plain": (args.steps / learned64["s"]) / (args.steps / plain_s),
             "guess_train_steps": n_train, "train_step_ms": 1e3 * train_step_s,
             "train_saddle_iterations_per_step": train["iterations"] / max(n_train, 1),
             # a 10,000-step run that trains on its first n_train steps, at the measured rates
-            "run10k_steps_per_s": n_run / (n_train * train_step_s + (n_run - n_train) * learned_s / args.steps)})
+            "run10k_steps_per_s": n_run / (n_train * train_step_s + (n_run - n_train) * learned64["s"] / args.steps)})
     if kern:
         sl, pe = kern["slab"], kern["pencil"]
         achieved = sl["bytes"] / (sl["milliseconds"] * 1e-3) / 1e9

@@ -293,13 +293,19 @@ int fts_saddle_field_moments(fts_saddle* s, double* mean, double* mean_sq);
 
 /* ---- Learned initial guess of the saddle point (no counterpart in the
  *      reference, whose drivers start each saddle point from the previous step's
- *      w+, examples/fts/lfts.py:701-717). A periodic 3D CNN maps w- to w+:
- *      conv(1->32) ReLU, 4 x [conv(32->32) ReLU], conv(32->1), 3x3x3 kernels with
+ *      w+, examples/fts/lfts.py:701-717). A periodic 3D CNN of n_in input fields:
+ *      conv(n_in->32) ReLU, 4 x [conv(32->32) ReLU], conv(32->1), 3x3x3 kernels with
  *      circular padding; bf16 weights and activations, FP32 accumulation (MFMA);
- *      FP32 master weights trained with Adam. Inputs are divided by in_scale and
- *      outputs multiplied by out_scale. The grid size must be a multiple of 32. */
+ *      FP32 master weights trained with Adam, laid out W0[n_in][27][32], b0[32],
+ *      4 x (W[27][32][32], b[32]), WL[27][32], bL. Input channel c is divided by
+ *      in_scales[c] and the output multiplied by out_scale; inputs are n_in
+ *      consecutive fields. The grid size must be a multiple of 32.
+ * n_params: the count of a one-input network; param_count: this network's count. */
 int fts_cnn_n_params(void);
 int fts_cnn_create(fts_context* ctx, const int* nx, double in_scale, double out_scale, fts_cnn** out);
+int fts_cnn_create_inputs(fts_context* ctx, const int* nx, int n_in, const double* in_scales, double out_scale,
+                          fts_cnn** out);
+int fts_cnn_param_count(fts_cnn* c, int* n);
 int fts_cnn_destroy(fts_cnn* c);
 /* FP32 master parameters, fts_cnn_n_params() floats, host memory; setting resets Adam */
 int fts_cnn_set_params(fts_cnn* c, const float* params);
@@ -320,15 +326,21 @@ int fts_cnn_train_step(fts_cnn* c, const double* w_minus, const double* w_plus_t
  * solver's precision. lo = NULL switches it off. lo must outlive its use here. */
 int fts_saddle_set_low_precision(fts_saddle* s, fts_solver* lo, double switch_error);
 
-/* Field theories with one real and one imaginary field (AB melts), on the device:
- * prepare: remember f(w_aux[real]) before the Langevin update;
- * predict: w_aux[imag] = f(w_aux[real]) (delta = 0), or w_aux[imag] += f(w_aux[real]) - the
- *          prepared prediction (delta = 1: the change of w+ the network expects from the update);
- * train:   push the current (w_aux[real], w_aux[imag]) pair into a ring of `replay` pairs, then
- *          `iters` Adam steps, the first on the newest pair and the rest on pairs drawn from the
- *          ring; loss is the newest pair's loss before its step. */
+/* Field theories with one real and one imaginary field (AB melts), on the device.
+ * With a one-input network f(w-):
+ *   prepare: remember f(w_aux[real]) before the Langevin update;
+ *   predict: w_aux[imag] = f(w_aux[real]) (mode 0), or w_aux[imag] += f(w_aux[real]) - the
+ *            prepared prediction (mode 1: the change of w+ the network expects from the update);
+ *   train:   push the current (w_aux[real], w_aux[imag]) pair into a ring of `replay` samples,
+ *            then `iters` Adam steps, the first on the newest sample and the rest on samples
+ *            drawn from the ring; loss is the newest sample's loss before its step.
+ * With a three-input network g (mode 2, "state"): the inputs are the new w-, its change
+ * in the Langevin update (w- minus the w- of the last fts_saddle_backup) and the w+ of the
+ * last backup, and g predicts the change of w+: predict sets w_aux[imag] = w+ of the backup
+ * + g(...); train pushes (inputs, converged w+ minus the w+ of the backup) of the current
+ * step. Both need a backup made before the Langevin update of this step. */
 int fts_saddle_prepare_guess(fts_saddle* s, fts_cnn* c);
-int fts_saddle_predict_guess(fts_saddle* s, fts_cnn* c, int delta);
+int fts_saddle_predict_guess(fts_saddle* s, fts_cnn* c, int mode);
 int fts_saddle_train_guess(fts_saddle* s, fts_cnn* c, double lr, int iters, int replay, double* loss);
 
 #ifdef __cplusplus

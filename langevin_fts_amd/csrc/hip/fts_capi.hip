@@ -708,22 +708,36 @@ int fts_saddle_field_moments(fts_saddle* s, double* mean, double* mean_sq) {
     });
 }
 
-int fts_cnn_n_params(void) { return fts::cnn_n_params(); }
+int fts_cnn_n_params(void) { return fts::cnn_n_params(1); }
 
-int fts_cnn_create(fts_context* ctx, const int* nx, double in_scale, double out_scale, fts_cnn** out) {
+int fts_cnn_create_inputs(fts_context* ctx, const int* nx, int n_in, const double* in_scales, double out_scale,
+                          fts_cnn** out) {
     return guarded([&] {
         need(ctx, "ctx");
         need(nx, "nx");
+        need(in_scales, "in_scales");
         need(out, "out");
         FTS_HIP_CHECK(hipSetDevice(ctx->device));
         auto* c = new fts_cnn;
         try {
-            c->c = std::make_unique<fts::Cnn>(ctx, nx[0], nx[1], nx[2], in_scale, out_scale);
+            c->c = std::make_unique<fts::Cnn>(ctx, nx[0], nx[1], nx[2], n_in, in_scales, out_scale);
         } catch (...) {
             delete c;
             throw;
         }
         *out = c;
+    });
+}
+
+int fts_cnn_create(fts_context* ctx, const int* nx, double in_scale, double out_scale, fts_cnn** out) {
+    return fts_cnn_create_inputs(ctx, nx, 1, &in_scale, out_scale, out);
+}
+
+int fts_cnn_param_count(fts_cnn* c, int* n) {
+    return guarded([&] {
+        need(c, "cnn");
+        need(n, "n");
+        *n = c->c->n_params();
     });
 }
 
@@ -796,11 +810,12 @@ int fts_saddle_prepare_guess(fts_saddle* s, fts_cnn* c) {
     });
 }
 
-int fts_saddle_predict_guess(fts_saddle* s, fts_cnn* c, int delta) {
+int fts_saddle_predict_guess(fts_saddle* s, fts_cnn* c, int mode) {
     return guarded([&] {
         need(s, "saddle");
         need(c, "cnn");
-        s->s->predict_guess(*c->c, delta != 0);
+        FTS_REQUIRE(mode >= 0 && mode <= 2, "guess mode must be 0 (absolute), 1 (delta) or 2 (state)");
+        s->s->predict_guess(*c->c, mode);
     });
 }
 

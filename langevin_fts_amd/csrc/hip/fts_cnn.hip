@@ -3,14 +3,15 @@
 // counterpart in the reference, whose drivers start each saddle point from the
 // previous step's w+, examples/fts/lfts.py:701-717).
 //
-// Network: conv(1->C) ReLU, 4 x [conv(C->C) ReLU], conv(C->1); 3x3x3 kernels with
-// circular padding, C = 32. Hidden activations are bf16, channels-last [voxel][C];
+// Network: conv(n_in->C) ReLU, 4 x [conv(C->C) ReLU], conv(C->1); 3x3x3 kernels with
+// circular padding, C = 32, n_in = 1 (w- alone) or 3 (w-, its change in the Langevin
+// update and the previous w+). Hidden activations are bf16, channels-last [voxel][C];
 // every product accumulates in FP32.
 //
 //   hidden layers    implicit GEMM on v_mfma_f32_32x32x16_bf16: one wavefront owns
 //                    32 voxels x 32 output channels and accumulates 27 taps x 2
 //                    input-channel chunks of 16
-//   first / last     FP32 FMA (1 input or 1 output channel)
+//   first / last     FP32 FMA (n_in input or 1 output channel)
 //   backward         data gradients: the same MFMA kernel with flipped, transposed
 //                    bf16 weights and the ReLU mask of the layer input; hidden weight
 //                    gradients: MFMA GEMMs over voxels, split over voxel chunks and
@@ -31,14 +32,22 @@ namespace {
 constexpr int C = FTS_CNN_CHANNELS;
 constexpr int TAPS = 27;
 constexpr int NHID = 4;                                 // hidden C->C layers
-constexpr int P_W0 = 0;                                 // [27][C]
-constexpr int P_B0 = P_W0 + TAPS * C;                   // [C]
-constexpr int P_WH = P_B0 + C;                          // NHID x ([27][C co][C ci] + [C])
 constexpr int HSZ = TAPS * C * C + C;
-constexpr int P_WL = P_WH + NHID * HSZ;                 // [27][C]
-constexpr int P_BL = P_WL + TAPS * C;                   // [1]
-constexpr int N_PARAMS = P_BL + 1;
 constexpr int WG_CHUNK = 1024;                          // voxels per block in the weight-gradient kernels
+
+// Offsets in the FP32 master vector for n_in input channels:
+// W0[n_in][27][C], b0[C], NHID x (W[27][C co][C ci], b[C]), WL[27][C], bL[1]
+struct Layout {
+    int w0, b0, wh, wl, bl, n;
+    explicit Layout(int n_in) {
+        w0 = 0;
+        b0 = w0 + n_in * TAPS * C;
+        wh = b0 + C;
+        wl = wh + NHID * HSZ;
+        bl = wl + TAPS * C;
+        n = bl + 1;
+    }
+};
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
@@ -80,17 +89,23 @@ __device__ __forceinline__ int neighbour(const Geo& g, Vox c, int t) {
 }
 
 // ---------------------------------------------------------------- forward
-__global__ void __launch_bounds__(256) conv_first(Geo g, const float* x, const float* W, const float* b, __bf16* out) {
+// x: [n_in][M], W: [n_in][27][C]
+__global__ void __launch_bounds__(256) conv_first(Geo g, int n_in, const float* x, const float* W, const float* b,
+                                                  __bf16* out) {
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= g.M) return;
     float acc[C];
 #pragma unroll
     for (int c = 0; c < C; c++) acc[c] = b[c];
     const Vox cv = coords(g, v);
-    for (int t = 0; t < TAPS; t++) {
-        const float xv = x[neighbour(g, cv, t)];
+    for (int ci = 0; ci < n_in; ci++) {
+        const float* xc = x + (size_t)ci * g.M;
+        const float* Wc = W + (size_t)ci * TAPS * C;
+        for (int t = 0; t < TAPS; t++) {
+            const float xv = xc[neighbour(g, cv, t)];
 #pragma unroll
-        for (int c = 0; c < C; c++) acc[c] += W[t * C + c] * xv;
+            for (int c = 0; c < C; c++) acc[c] += Wc[t * C + c] * xv;
+        }
     }
     bf16x8* o = reinterpret_cast<bf16x8*>(out + (size_t)v * C);
 #pragma unroll
@@ -316,12 +331,12 @@ __global__ void __launch_bounds__(256) adam_step(float* p, const float* gr, floa
 
 // bf16 copies of the hidden weights: forward [t][co][ci] and the backward form
 // [t'][ci][co] with t' = 26 - t
-__global__ void __launch_bounds__(256) hidden_weights(const float* p, __bf16* wf, __bf16* wb) {
+__global__ void __launch_bounds__(256) hidden_weights(const float* wh, __bf16* wf, __bf16* wb) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= NHID * TAPS * C * C) return;
     const int l = i / (TAPS * C * C), r = i % (TAPS * C * C);
     const int t = r / (C * C), co = (r / C) % C, ci = r % C;
-    const __bf16 w = (__bf16)p[P_WH + l * HSZ + r];
+    const __bf16 w = (__bf16)wh[l * HSZ + r];
     wf[i] = w;
     wb[(size_t)l * TAPS * C * C + ((size_t)(26 - t) * C + ci) * C + co] = w;
 }
@@ -340,22 +355,25 @@ int blocks(int n, int b = 256) { return (n + b - 1) / b; }
 
 }  // namespace
 
-int cnn_n_params() { return N_PARAMS; }
+int cnn_n_params(int n_in) { return Layout(n_in).n; }
 
-Cnn::Cnn(fts_context* ctx, int nx, int ny, int nz, double in_scale, double out_scale)
-    : ctx_(ctx), nx_(nx), ny_(ny), nz_(nz), in_scale_(in_scale), out_scale_(out_scale) {
+Cnn::Cnn(fts_context* ctx, int nx, int ny, int nz, int n_in, const double* in_scales, double out_scale)
+    : ctx_(ctx), nx_(nx), ny_(ny), nz_(nz), n_in_(n_in), n_params_(Layout(n_in).n), out_scale_(out_scale) {
     FTS_REQUIRE(nx >= 3 && ny >= 3 && nz >= 3, "the CNN needs at least 3 points per axis");
+    FTS_REQUIRE(n_in >= 1 && n_in <= 4, "the CNN takes 1 to 4 input channels");
     M_ = nx * ny * nz;
     FTS_REQUIRE(M_ % 32 == 0, "the CNN needs a grid size that is a multiple of 32");
-    FTS_REQUIRE(in_scale > 0 && out_scale > 0, "scales must be positive");
+    for (int c = 0; c < n_in; c++) FTS_REQUIRE(in_scales[c] > 0, "scales must be positive");
+    FTS_REQUIRE(out_scale > 0, "scales must be positive");
+    in_scales_.assign(in_scales, in_scales + n_in);
     n_chunks_ = (M_ + WG_CHUNK - 1) / WG_CHUNK;
-    params_.allocate(sizeof(float) * N_PARAMS);
-    grads_.allocate(sizeof(float) * N_PARAMS);
-    adam_m_.allocate(sizeof(float) * N_PARAMS);
-    adam_v_.allocate(sizeof(float) * N_PARAMS);
+    params_.allocate(sizeof(float) * n_params_);
+    grads_.allocate(sizeof(float) * n_params_);
+    adam_m_.allocate(sizeof(float) * n_params_);
+    adam_v_.allocate(sizeof(float) * n_params_);
     wf_.allocate(sizeof(__bf16) * NHID * TAPS * C * C);
     wb_.allocate(sizeof(__bf16) * NHID * TAPS * C * C);
-    x_.allocate(sizeof(float) * M_);
+    x_.allocate(sizeof(float) * (size_t)n_in * M_);
     target_.allocate(sizeof(float) * M_);
     y_.allocate(sizeof(float) * M_);
     dy_.allocate(sizeof(float) * M_);
@@ -365,36 +383,36 @@ Cnn::Cnn(fts_context* ctx, int nx, int ny, int nz, double in_scale, double out_s
     partial_.allocate(sizeof(float) * (size_t)std::max(TAPS * n_chunks_ * C * C, n_chunks_ * TAPS * C));
     loss_partials_.allocate(sizeof(double) * 256);
     loss_.allocate(sizeof(double));
-    stage_in_.allocate(sizeof(double) * M_);
+    stage_in_.allocate(sizeof(double) * (size_t)n_in * M_);
     stage_out_.allocate(sizeof(double) * M_);
-    std::vector<float> zero(N_PARAMS, 0.f);
+    std::vector<float> zero(n_params_, 0.f);
     set_params(zero.data());
 }
 
 void Cnn::set_params(const float* host) {
     version_++;
     hipStream_t st = ctx_->stream;
-    FTS_HIP_CHECK(hipMemcpyAsync(params_.get(), host, sizeof(float) * N_PARAMS, hipMemcpyHostToDevice, st));
-    FTS_HIP_CHECK(hipMemsetAsync(adam_m_.get(), 0, sizeof(float) * N_PARAMS, st));
-    FTS_HIP_CHECK(hipMemsetAsync(adam_v_.get(), 0, sizeof(float) * N_PARAMS, st));
+    FTS_HIP_CHECK(hipMemcpyAsync(params_.get(), host, sizeof(float) * n_params_, hipMemcpyHostToDevice, st));
+    FTS_HIP_CHECK(hipMemsetAsync(adam_m_.get(), 0, sizeof(float) * n_params_, st));
+    FTS_HIP_CHECK(hipMemsetAsync(adam_v_.get(), 0, sizeof(float) * n_params_, st));
     step_ = 0;
     refresh_weights();
     FTS_HIP_CHECK(hipStreamSynchronize(st));
 }
 
 void Cnn::get_params(float* host) {
-    FTS_HIP_CHECK(hipMemcpyAsync(host, params_.get(), sizeof(float) * N_PARAMS, hipMemcpyDeviceToHost, ctx_->stream));
+    FTS_HIP_CHECK(hipMemcpyAsync(host, params_.get(), sizeof(float) * n_params_, hipMemcpyDeviceToHost, ctx_->stream));
     FTS_HIP_CHECK(hipStreamSynchronize(ctx_->stream));
 }
 
 void Cnn::get_grads(float* host) {
-    FTS_HIP_CHECK(hipMemcpyAsync(host, grads_.get(), sizeof(float) * N_PARAMS, hipMemcpyDeviceToHost, ctx_->stream));
+    FTS_HIP_CHECK(hipMemcpyAsync(host, grads_.get(), sizeof(float) * n_params_, hipMemcpyDeviceToHost, ctx_->stream));
     FTS_HIP_CHECK(hipStreamSynchronize(ctx_->stream));
 }
 
 void Cnn::refresh_weights() {
     hipLaunchKernelGGL(hidden_weights, dim3(blocks(NHID * TAPS * C * C)), dim3(256), 0, ctx_->stream,
-                       params_.as<float>(), wf_.as<__bf16>(), wb_.as<__bf16>());
+                       params_.as<float>() + Layout(n_in_).wh, wf_.as<__bf16>(), wb_.as<__bf16>());
     FTS_HIP_CHECK(hipGetLastError());
 }
 
@@ -402,10 +420,13 @@ void Cnn::load_input(const double* w_minus, bool on_device) {
     hipStream_t st = ctx_->stream;
     const double* src = w_minus;
     if (!on_device) {
-        FTS_HIP_CHECK(hipMemcpyAsync(stage_in_.get(), w_minus, sizeof(double) * M_, hipMemcpyHostToDevice, st));
+        FTS_HIP_CHECK(hipMemcpyAsync(stage_in_.get(), w_minus, sizeof(double) * n_in_ * M_, hipMemcpyHostToDevice,
+                                     st));
         src = stage_in_.as<double>();
     }
-    hipLaunchKernelGGL(to_float, dim3(blocks(M_)), dim3(256), 0, st, src, 1.0 / in_scale_, M_, x_.as<float>());
+    for (int c = 0; c < n_in_; c++)
+        hipLaunchKernelGGL(to_float, dim3(blocks(M_)), dim3(256), 0, st, src + (size_t)c * M_, 1.0 / in_scales_[c], M_,
+                           x_.as<float>() + (size_t)c * M_);
     FTS_HIP_CHECK(hipGetLastError());
 }
 
@@ -413,14 +434,16 @@ void Cnn::forward() {
     hipStream_t st = ctx_->stream;
     const Geo g{nx_, ny_, nz_, M_};
     const float* p = params_.as<float>();
+    const Layout L(n_in_);
     __bf16* a = acts_.as<__bf16>();
     const size_t AS = (size_t)M_ * C;
-    hipLaunchKernelGGL(conv_first, dim3(blocks(M_)), dim3(256), 0, st, g, x_.as<float>(), p + P_W0, p + P_B0, a);
+    hipLaunchKernelGGL(conv_first, dim3(blocks(M_)), dim3(256), 0, st, g, n_in_, x_.as<float>(), p + L.w0, p + L.b0,
+                       a);
     for (int l = 0; l < NHID; l++)
         hipLaunchKernelGGL(conv_hidden, dim3(blocks(M_ / 32, 4)), dim3(256), 0, st, g, a + l * AS,
-                           wf_.as<__bf16>() + (size_t)l * TAPS * C * C, p + P_WH + l * HSZ + TAPS * C * C,
+                           wf_.as<__bf16>() + (size_t)l * TAPS * C * C, p + L.wh + l * HSZ + TAPS * C * C,
                            (const __bf16*)nullptr, 1, a + (l + 1) * AS);
-    hipLaunchKernelGGL(conv_last, dim3(blocks(M_)), dim3(256), 0, st, g, a + NHID * AS, p + P_WL, p + P_BL,
+    hipLaunchKernelGGL(conv_last, dim3(blocks(M_)), dim3(256), 0, st, g, a + NHID * AS, p + L.wl, p + L.bl,
                        y_.as<float>());
     FTS_HIP_CHECK(hipGetLastError());
 }
@@ -458,6 +481,7 @@ double Cnn::gradients(const double* w_minus, const double* w_plus_target, bool o
     hipLaunchKernelGGL(sum_partials, dim3(1), dim3(256), 0, st, loss_partials_.as<double>(), lb, loss_.as<double>());
 
     const float* p = params_.as<float>();
+    const Layout L(n_in_);
     float* gr = grads_.as<float>();
     float* part = partial_.as<float>();
     __bf16* a = acts_.as<__bf16>();
@@ -465,10 +489,10 @@ double Cnn::gradients(const double* w_minus, const double* w_plus_target, bool o
     // last layer: weights, bias, then dZ of the last hidden activation
     hipLaunchKernelGGL(wgrad_edge, dim3(n_chunks_), dim3(256), 0, st, g, 0, a + NHID * AS, dy_.as<float>(), part);
     hipLaunchKernelGGL(reduce_chunks, dim3(blocks(TAPS * C)), dim3(256), 0, st, part, n_chunks_, TAPS * C, TAPS * C,
-                       gr + P_WL);
+                       gr + L.wl);
     hipLaunchKernelGGL(bias_grad, dim3(n_chunks_), dim3(256), 0, st, g, (const __bf16*)nullptr, dy_.as<float>(), part);
-    hipLaunchKernelGGL(reduce_chunks, dim3(1), dim3(256), 0, st, part, n_chunks_, C, 1, gr + P_BL);
-    hipLaunchKernelGGL(back_last, dim3(blocks(M_)), dim3(256), 0, st, g, dy_.as<float>(), p + P_WL, a + NHID * AS,
+    hipLaunchKernelGGL(reduce_chunks, dim3(1), dim3(256), 0, st, part, n_chunks_, C, 1, gr + L.bl);
+    hipLaunchKernelGGL(back_last, dim3(blocks(M_)), dim3(256), 0, st, g, dy_.as<float>(), p + L.wl, a + NHID * AS,
                        grad_a_.as<__bf16>());
     __bf16* dz = grad_a_.as<__bf16>();
     __bf16* other = grad_b_.as<__bf16>();
@@ -476,20 +500,23 @@ double Cnn::gradients(const double* w_minus, const double* w_plus_target, bool o
         // layer l maps a[l] -> a[l + 1]; dz = dL/d(pre-activation of a[l + 1])
         hipLaunchKernelGGL(wgrad_hidden, dim3(n_chunks_, TAPS), dim3(256), 0, st, g, dz, a + l * AS, part);
         hipLaunchKernelGGL(reduce_chunks, dim3(blocks(TAPS * C * C)), dim3(256), 0, st, part, n_chunks_,
-                           TAPS * C * C, TAPS * C * C, gr + P_WH + l * HSZ);
+                           TAPS * C * C, TAPS * C * C, gr + L.wh + l * HSZ);
         hipLaunchKernelGGL(bias_grad, dim3(n_chunks_), dim3(256), 0, st, g, dz, (const float*)nullptr, part);
         hipLaunchKernelGGL(reduce_chunks, dim3(1), dim3(256), 0, st, part, n_chunks_, C, C,
-                           gr + P_WH + l * HSZ + TAPS * C * C);
+                           gr + L.wh + l * HSZ + TAPS * C * C);
         hipLaunchKernelGGL(conv_hidden, dim3(blocks(M_ / 32, 4)), dim3(256), 0, st, g, dz,
                            wb_.as<__bf16>() + (size_t)l * TAPS * C * C, (const float*)nullptr, a + l * AS, 0, other);
         std::swap(dz, other);
     }
     // first layer
-    hipLaunchKernelGGL(wgrad_edge, dim3(n_chunks_), dim3(256), 0, st, g, 1, dz, x_.as<float>(), part);
-    hipLaunchKernelGGL(reduce_chunks, dim3(blocks(TAPS * C)), dim3(256), 0, st, part, n_chunks_, TAPS * C, TAPS * C,
-                       gr + P_W0);
+    for (int c = 0; c < n_in_; c++) {
+        hipLaunchKernelGGL(wgrad_edge, dim3(n_chunks_), dim3(256), 0, st, g, 1, dz, x_.as<float>() + (size_t)c * M_,
+                           part);
+        hipLaunchKernelGGL(reduce_chunks, dim3(blocks(TAPS * C)), dim3(256), 0, st, part, n_chunks_, TAPS * C,
+                           TAPS * C, gr + L.w0 + c * TAPS * C);
+    }
     hipLaunchKernelGGL(bias_grad, dim3(n_chunks_), dim3(256), 0, st, g, dz, (const float*)nullptr, part);
-    hipLaunchKernelGGL(reduce_chunks, dim3(1), dim3(256), 0, st, part, n_chunks_, C, C, gr + P_B0);
+    hipLaunchKernelGGL(reduce_chunks, dim3(1), dim3(256), 0, st, part, n_chunks_, C, C, gr + L.b0);
     FTS_HIP_CHECK(hipGetLastError());
     double loss = 0.0;
     FTS_HIP_CHECK(hipMemcpyAsync(&loss, loss_.get(), sizeof(double), hipMemcpyDeviceToHost, st));
@@ -503,8 +530,8 @@ double Cnn::train_step(const double* w_minus, const double* w_plus_target, bool 
     version_++;
     const float b1 = 0.9f, b2 = 0.999f;
     const float c1 = 1.f - std::pow(b1, (float)step_), c2 = 1.f - std::pow(b2, (float)step_);
-    hipLaunchKernelGGL(adam_step, dim3(blocks(N_PARAMS)), dim3(256), 0, ctx_->stream, params_.as<float>(),
-                       grads_.as<float>(), adam_m_.as<float>(), adam_v_.as<float>(), N_PARAMS, (float)lr, b1, b2,
+    hipLaunchKernelGGL(adam_step, dim3(blocks(n_params_)), dim3(256), 0, ctx_->stream, params_.as<float>(),
+                       grads_.as<float>(), adam_m_.as<float>(), adam_v_.as<float>(), n_params_, (float)lr, b1, b2,
                        1e-8f, c1, c2);
     FTS_HIP_CHECK(hipGetLastError());
     refresh_weights();

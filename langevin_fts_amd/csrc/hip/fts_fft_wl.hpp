@@ -235,52 +235,73 @@ __global__ void __launch_bounds__(BD) slab_kernel_wl(SlabBatch<T> batch, const t
     const int p0 = wave * G::NLZ;
     const int nrow = max(0, min(G::NLZ, G::NP - p0));
 
-    // w at this wavefront's real positions: rows 2p (e) and 2p + 1 (o), column k
-    T we[MZ::NT][RZ], wo[MZ::NT][RZ];
+    // w at this wavefront's real positions: rows 2p (e) and 2p + 1 (o), column k.
+    // Up to N = 64 it is loaded once and stays in registers for the whole kernel. A
+    // 96^3 or 128^3 real plane takes 24-32 VGPRs per thread at 1024 threads, and
+    // holding w beside the propagator's own plane and a transform's working set
+    // spilled to scratch; there each phase that applies a factor reloads w (an L2
+    // hit after the first read), so w is live only while the phase runs.
+    constexpr bool W_RESIDENT = N <= 64;
+    // The y phase of a wavefront runs its NLY columns in YNCH chunks of YCH columns, so
+    // that a chunk's first-stage butterflies fill at most one register slot per lane:
+    // at 128^3 with 1024 threads all 5 columns at once take two slots (80 butterflies
+    // for 64 lanes) and double the phase's registers. One chunk up to N = 64.
+    constexpr int YB0 = N / G::YI::r(0);
+    constexpr int YNCH = W_RESIDENT ? 1 : (G::NLY * YB0 + 63) / 64;
+    constexpr int YCH = (G::NLY + YNCH - 1) / YNCH;
+    auto load_w = [&](T (&we)[W_RESIDENT ? MZ::NT : 1][RZ], T (&wo)[W_RESIDENT ? MZ::NT : 1][RZ]) {
 #pragma unroll
-    for (int t = 0; t < MZ::NT; t++) {
-        const int g = lane + 64 * t;
-        const bool ok = MZ::ok(g, nrow);
-        const int p = p0 + MZ::line(g);
+        for (int t = 0; t < MZ::NT; t++) {
+            const int g = lane + 64 * t;
+            const bool ok = MZ::ok(g, nrow);
+            const int p = p0 + MZ::line(g);
 #pragma unroll
-        for (int r = 0; r < RZ; r++) {
-            const int k = MZ::idx(g) + r * MZ::B;
-            we[t][r] = (has_w && ok) ? it.w[roff + (size_t)(2 * p) * N + k] : (T)0;
-            wo[t][r] = (has_w && ok) ? it.w[roff + (size_t)(2 * p + 1) * N + k] : (T)0;
+            for (int r = 0; r < RZ; r++) {
+                const int k = MZ::idx(g) + r * MZ::B;
+                we[t][r] = (has_w && ok) ? it.w[roff + (size_t)(2 * p) * N + k] : (T)0;
+                wo[t][r] = (has_w && ok) ? it.w[roff + (size_t)(2 * p + 1) * N + k] : (T)0;
+            }
         }
-    }
+    };
+    T we_r[W_RESIDENT ? MZ::NT : 1][RZ], wo_r[W_RESIDENT ? MZ::NT : 1][RZ];
+    if constexpr (W_RESIDENT) load_w(we_r, wo_r);
     for (int m = threadIdx.x; m < N; m += G::BD) {
         Wy[m] = twy[m];
         Wz[m] = twz[m];
     }
 
     // spectrum -> real rows times exp(-w coef) / M, in registers
-    auto inverse = [&](const T2* spec, T coef, T (&qe)[MZ::NT][RZ], T (&qo)[MZ::NT][RZ]) {
-        using MA = WStageMap<N, false, 0, G::NLY, true>;
-        T2 v[MA::NT][G::YI::r(0)];
+    auto inverse = [&](const T2* spec, T coef, T (&qe)[MZ::NT][RZ], T (&qo)[MZ::NT][RZ], auto comb) {
+        using MA = WStageMap<N, false, 0, YCH, true>;
 #pragma unroll
-        for (int t = 0; t < MA::NT; t++) {
-            const int g = lane + 64 * t;
-            if (MA::ok(g, ncol)) {
-                const int c = col0 + MA::line(g), j = MA::idx(g);
+        for (int h = 0; h < YNCH; h++) {
+            const int cb = col0 + h * YCH;
+            const int nc = max(0, min(YCH, ncol - h * YCH));
+            T2 v[MA::NT][G::YI::r(0)];
 #pragma unroll
-                for (int r = 0; r < G::YI::r(0); r++) v[t][r] = spec[(j + r * MA::B) * G::SP + c];
-            }
-        }
-        __syncthreads();   // twiddles stored; the plane's previous users are done
-        wl_stages<+1, T, T2, N, false, G::NLY, true, 1, P, false, 0>(v, S + col0, Wy, lane, ncol, [&](auto& o) {
-            using MO = WStageMap<N, false, G::YI::n - 1, G::NLY, true>;
-            wave_sync();
-#pragma unroll
-            for (int t = 0; t < MO::NT; t++) {
+            for (int t = 0; t < MA::NT; t++) {
                 const int g = lane + 64 * t;
-                if (MO::ok(g, ncol)) {
-                    const int c = col0 + MO::line(g), j = MO::idx(g);
+                if (MA::ok(g, nc)) {
+                    const int c = cb + MA::line(g), j = MA::idx(g);
 #pragma unroll
-                    for (int r = 0; r < G::YI::r(G::YI::n - 1); r++) S[(j + r * MO::B) * P + c] = o[t][r];
+                    for (int r = 0; r < G::YI::r(0); r++) v[t][r] = spec[(j + r * MA::B) * G::SP + c];
                 }
             }
-        });
+            if (h == 0) __syncthreads();   // twiddles stored; the plane's previous users are done
+            wl_stages<+1, T, T2, N, false, YCH, true, 1, P, false, 0>(v, S + cb, Wy, lane, nc, [&](auto& o) {
+                using MO = WStageMap<N, false, G::YI::n - 1, YCH, true>;
+                wave_sync();
+#pragma unroll
+                for (int t = 0; t < MO::NT; t++) {
+                    const int g = lane + 64 * t;
+                    if (MO::ok(g, nc)) {
+                        const int c = cb + MO::line(g), j = MO::idx(g);
+#pragma unroll
+                        for (int r = 0; r < G::YI::r(G::YI::n - 1); r++) S[(j + r * MO::B) * P + c] = o[t][r];
+                    }
+                }
+            });
+        }
         __syncthreads();
         // packed rows: Z[k] = X_{2p}[k] + i X_{2p+1}[k] from the half spectra (Hermitian fill)
         using MB = WStageMap<N, false, 0, G::NLZ, false>;
@@ -309,19 +330,54 @@ __global__ void __launch_bounds__(BD) slab_kernel_wl(SlabBatch<T> batch, const t
                 }
             }
         }
-        wl_stages<+1, T, T2, N, false, G::NLZ, false, 2 * P, 1, WSLAB_SKEW, 0>(u, S + 2 * p0 * P, Wz, lane, nrow, [&](auto& o) {
+        // q = o exp(-w coef) / M; with COMB the Richardson combination (4 q - q_prev) / 3
+        // against the q_prev already in qe/qo. Without resident w, each register slot
+        // loads its w just before use.
+        auto finish = [&](auto& o) {
 #pragma unroll
             for (int t = 0; t < MZ::NT; t++) {
                 const int g = lane + 64 * t;
                 if (MZ::ok(g, nrow)) {
+                    T we[RZ], wo[RZ];
+                    if constexpr (W_RESIDENT) {
+#pragma unroll
+                        for (int r = 0; r < RZ; r++) {
+                            we[r] = we_r[t][r];
+                            wo[r] = wo_r[t][r];
+                        }
+                    } else {
+                        const int p = p0 + MZ::line(g);
+#pragma unroll
+                        for (int r = 0; r < RZ; r++) {
+                            const int k = MZ::idx(g) + r * MZ::B;
+                            we[r] = has_w ? it.w[roff + (size_t)(2 * p) * N + k] : (T)0;
+                            wo[r] = has_w ? it.w[roff + (size_t)(2 * p + 1) * N + k] : (T)0;
+                        }
+                    }
 #pragma unroll
                     for (int r = 0; r < RZ; r++) {
-                        qe[t][r] = o[t][r].x * factor<T>(we[t][r], has_w, coef) * inv_m;
-                        qo[t][r] = o[t][r].y * factor<T>(wo[t][r], has_w, coef) * inv_m;
+                        const T ve = o[t][r].x * factor<T>(we[r], has_w, coef) * inv_m;
+                        const T vo = o[t][r].y * factor<T>(wo[r], has_w, coef) * inv_m;
+                        const size_t ie = roff + (size_t)(2 * (p0 + MZ::line(g))) * N + MZ::idx(g) + r * MZ::B;
+                        if constexpr (decltype(comb)::value == 1) {
+                            qe[t][r] = ((T)4.0 * ve - qe[t][r]) / (T)3.0;
+                            qo[t][r] = ((T)4.0 * vo - qo[t][r]) / (T)3.0;
+                        } else if constexpr (decltype(comb)::value == 2) {
+                            // q1 of the Richardson step parked in the output slice
+                            it.qout[ie] = ve;
+                            it.qout[ie + N] = vo;
+                        } else if constexpr (decltype(comb)::value == 3) {
+                            qe[t][r] = ((T)4.0 * ve - it.qout[ie]) / (T)3.0;
+                            qo[t][r] = ((T)4.0 * vo - it.qout[ie + N]) / (T)3.0;
+                        } else {
+                            qe[t][r] = ve;
+                            qo[t][r] = vo;
+                        }
                     }
                 }
             }
-        });
+        };
+        wl_stages<+1, T, T2, N, false, G::NLZ, false, 2 * P, 1, WSLAB_SKEW, 0>(u, S + 2 * p0 * P, Wz, lane, nrow, finish);
     };
 
     // real rows times exp(-w coef) -> spectrum
@@ -331,9 +387,25 @@ __global__ void __launch_bounds__(BD) slab_kernel_wl(SlabBatch<T> batch, const t
         for (int t = 0; t < MZ::NT; t++) {
             const int g = lane + 64 * t;
             if (MZ::ok(g, nrow)) {
+                T we[RZ], wo[RZ];
+                if constexpr (W_RESIDENT) {
+#pragma unroll
+                    for (int r = 0; r < RZ; r++) {
+                        we[r] = we_r[t][r];
+                        wo[r] = wo_r[t][r];
+                    }
+                } else {
+                    const int p = p0 + MZ::line(g);
+#pragma unroll
+                    for (int r = 0; r < RZ; r++) {
+                        const int k = MZ::idx(g) + r * MZ::B;
+                        we[r] = has_w ? it.w[roff + (size_t)(2 * p) * N + k] : (T)0;
+                        wo[r] = has_w ? it.w[roff + (size_t)(2 * p + 1) * N + k] : (T)0;
+                    }
+                }
 #pragma unroll
                 for (int r = 0; r < RZ; r++)
-                    u[t][r] = T2{qe[t][r] * factor<T>(we[t][r], has_w, coef), qo[t][r] * factor<T>(wo[t][r], has_w, coef)};
+                    u[t][r] = T2{qe[t][r] * factor<T>(we[r], has_w, coef), qo[t][r] * factor<T>(wo[r], has_w, coef)};
             }
         }
         wl_stages<-1, T, T2, N, true, G::NLZ, false, 2 * P, 1, WSLAB_SKEW, 0>(u, S + 2 * p0 * P, Wz, lane, nrow, [&](auto& o) {
@@ -378,29 +450,35 @@ __global__ void __launch_bounds__(BD) slab_kernel_wl(SlabBatch<T> batch, const t
             }
         }
         __syncthreads();
-        using MD = WStageMap<N, false, 0, G::NLY, true>;
-        T2 v[MD::NT][G::YF::r(0)];
+        using MD = WStageMap<N, false, 0, YCH, true>;
 #pragma unroll
-        for (int t = 0; t < MD::NT; t++) {
-            const int g = lane + 64 * t;
-            if (MD::ok(g, ncol)) {
-                const int c = col0 + MD::line(g), j = MD::idx(g);
+        for (int h = 0; h < YNCH; h++) {
+            const int cb = col0 + h * YCH;
+            const int nc = max(0, min(YCH, ncol - h * YCH));
+            T2 v[MD::NT][G::YF::r(0)];
 #pragma unroll
-                for (int r = 0; r < G::YF::r(0); r++) v[t][r] = S[(j + r * MD::B) * P + c];
-            }
-        }
-        wl_stages<-1, T, T2, N, false, G::NLY, true, 1, P, false, 0>(v, S + col0, Wy, lane, ncol, [&](auto& o) {
-            using MO = WStageMap<N, false, G::YF::n - 1, G::NLY, true>;
-#pragma unroll
-            for (int t = 0; t < MO::NT; t++) {
+            for (int t = 0; t < MD::NT; t++) {
                 const int g = lane + 64 * t;
-                if (MO::ok(g, ncol)) {
-                    const int c = col0 + MO::line(g), j = MO::idx(g);
+                if (MD::ok(g, nc)) {
+                    const int c = cb + MD::line(g), j = MD::idx(g);
 #pragma unroll
-                    for (int r = 0; r < G::YF::r(G::YF::n - 1); r++) put2(&spec[(j + r * MO::B) * G::SP + c], o[t][r]);
+                    for (int r = 0; r < G::YF::r(0); r++) v[t][r] = S[(j + r * MD::B) * P + c];
                 }
             }
-        });
+            wl_stages<-1, T, T2, N, false, YCH, true, 1, P, false, 0>(v, S + cb, Wy, lane, nc, [&](auto& o) {
+                using MO = WStageMap<N, false, G::YF::n - 1, YCH, true>;
+#pragma unroll
+                for (int t = 0; t < MO::NT; t++) {
+                    const int g = lane + 64 * t;
+                    if (MO::ok(g, nc)) {
+                        const int c = cb + MO::line(g), j = MO::idx(g);
+#pragma unroll
+                        for (int r = 0; r < G::YF::r(G::YF::n - 1); r++)
+                            put2(&spec[(j + r * MO::B) * G::SP + c], o[t][r]);
+                    }
+                }
+            });
+        }
     };
 
     T qe[MZ::NT][RZ], qo[MZ::NT][RZ];
@@ -420,18 +498,13 @@ __global__ void __launch_bounds__(BD) slab_kernel_wl(SlabBatch<T> batch, const t
         }
         __syncthreads();   // twiddles
     } else {
-        inverse(it.c0 + coff, it.post0, qe, qo);
-        if constexpr (MODE == SLAB_INV2) {
-            T q2e[MZ::NT][RZ], q2o[MZ::NT][RZ];
-            inverse(it.c1 + coff, it.post1, q2e, q2o);
-#pragma unroll
-            for (int t = 0; t < MZ::NT; t++)
-#pragma unroll
-                for (int r = 0; r < RZ; r++) {
-                    qe[t][r] = ((T)4.0 * q2e[t][r] - qe[t][r]) / (T)3.0;
-                    qo[t][r] = ((T)4.0 * q2o[t][r] - qo[t][r]) / (T)3.0;
-                }
-        }
+        // Richardson step: (4 q2 - q1) / 3 with q2 from the second spectrum. Without
+        // resident w, q1 waits in the output slice (always set for INV2 items at these
+        // sizes; the host checks it) instead of in registers through the second inverse.
+        using Comb = std::integral_constant<int, W_RESIDENT ? 1 : 3>;
+        using First = std::integral_constant<int, (MODE == SLAB_INV2 && !W_RESIDENT) ? 2 : 0>;
+        inverse(it.c0 + coff, it.post0, qe, qo, First{});
+        if constexpr (MODE == SLAB_INV2) inverse(it.c1 + coff, it.post1, qe, qo, Comb{});
 #pragma unroll
         for (int t = 0; t < MZ::NT; t++) {
             const int g = lane + 64 * t;
@@ -456,10 +529,28 @@ __global__ void __launch_bounds__(BD) slab_kernel_wl(SlabBatch<T> batch, const t
     }
     if (!do_fwd) return;
     const int n_out = (MODE == SLAB_FWD) ? it.n_out : (MODE == SLAB_INV2 ? 2 : 1);
-    for (int o = 0; o < n_out; o++) {
-        if (o > 0) __syncthreads();   // the plane of the previous spectrum is fully read
-        forward(qe, qo, o == 0 ? it.pre0 : it.pre1, (o == 0 ? it.c0 : it.c1) + coff);
+    forward(qe, qo, it.pre0, it.c0 + coff);
+    if (n_out < 2) return;
+    __syncthreads();   // the plane of the first spectrum is fully read
+    if constexpr (!W_RESIDENT) {
+        // the second forward transform starts from the real plane this thread read
+        // (FWD: qin) or wrote (INV2: qout, always set for INV2 items at these sizes;
+        // the host checks it), so the plane is not held in registers through the first
+#pragma unroll
+        for (int t = 0; t < MZ::NT; t++) {
+            const int g = lane + 64 * t;
+            const bool ok = MZ::ok(g, nrow);
+            const int p = p0 + MZ::line(g);
+            const T* src = MODE == SLAB_FWD ? it.qin : it.qout;
+#pragma unroll
+            for (int r = 0; r < RZ; r++) {
+                const int k = MZ::idx(g) + r * MZ::B;
+                qe[t][r] = ok ? src[roff + (size_t)(2 * p) * N + k] : (T)0;
+                qo[t][r] = ok ? src[roff + (size_t)(2 * p + 1) * N + k] : (T)0;
+            }
+        }
     }
+    forward(qe, qo, it.pre1, it.c1 + coff);
 }
 
 // ------------------------------------------------------------------ pencil pass

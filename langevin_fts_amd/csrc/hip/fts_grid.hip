@@ -244,6 +244,15 @@ void set_lds_attributes_ct() {
 template <typename T, int N>
 void launch_slab_ct(const Grid& g, int mode, const SlabBatch<T>& b) {
     if (g.wave_local) {
+        if constexpr (N > 64) {
+            // at these sizes the wave-local kernel parks the Richardson q1 in the output
+            // slice and starts a second forward transform from qin / qout
+            for (int i = 0; i < b.n; i++) {
+                const SlabItem<T>& it = b.it[i];
+                if ((mode == SLAB_INV2 && !it.qout) || (mode == SLAB_FWD && it.n_out > 1 && !it.qin))
+                    throw Error("slab item without the real slice its second pass reads (N=" + std::to_string(N) + ")");
+            }
+        }
         const int bd = g.slab_bd[prec_index<T>()];
         launch_slab_wl_bd<T, N>(g, bd ? bd : wslab_default_bd(N), mode, b);
         return;

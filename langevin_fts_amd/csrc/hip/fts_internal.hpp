@@ -246,7 +246,9 @@ public:
     }
     // learned guess of the imaginary field from the real one (single replica)
     void prepare_guess(Cnn& cnn);               // f(w_real) before the Langevin update (delta mode)
-    void predict_guess(Cnn& cnn, bool delta);   // w_imag = f(w_real), or w_imag += f(w_real) - prepared
+    // mode 0: w_imag = f(w_real); 1: w_imag += f(w_real) - prepared; 2 (three-input network):
+    // w_imag = w_imag of the backup + g(w_real, w_real - backup's, backup's w_imag)
+    void predict_guess(Cnn& cnn, int mode);
     double train_guess(Cnn& cnn, double lr, int iters, int replay);
 
 private:
@@ -275,7 +277,11 @@ private:
     DeviceBuffer w_aux_, phi_, w_solver_, h_, lam_, am_w_, am_d_, noise_prev_, noise_cur_, w_backup_, phi_backup_,
         partials_, stats_;
     // learned guess: replay ring of (w_real, w_imag) pairs and the prepared prediction
-    DeviceBuffer replay_, guess_old_, guess_new_;
+    // the three inputs of the state guess (and its target) from the fields and the backup
+    void state_inputs(double* in, double* target);
+    DeviceBuffer replay_, guess_old_, guess_new_, guess_in_;
+    bool has_backup_ = false;
+    int replay_per_ = 0;   // fields per ring sample: the network's inputs and its target
     // which network (and parameter version) and which real field a guess buffer holds f of:
     // the real field's version changes with every update of it and returns with a restore
     struct GuessTag {
@@ -292,12 +298,14 @@ private:
     bool prepared_ = false;
 };
 
-int cnn_n_params();
+int cnn_n_params(int n_in);
 
 // Periodic 3D CNN of the learned saddle-point guess (fts_cnn.hip)
 class Cnn {
 public:
-    Cnn(fts_context* ctx, int nx, int ny, int nz, double in_scale, double out_scale);
+    // n_in input channels, in_scales[n_in]: inputs are divided by their scale, outputs
+    // multiplied by out_scale
+    Cnn(fts_context* ctx, int nx, int ny, int nz, int n_in, const double* in_scales, double out_scale);
     // changes whenever the parameters do
     long long version() const { return version_; }
     void set_params(const float* host);
@@ -308,14 +316,17 @@ public:
     double gradients(const double* w_minus, const double* w_plus_target, bool on_device);
     double train_step(const double* w_minus, const double* w_plus_target, bool on_device, double lr);
     int n_grid() const { return M_; }
+    int n_in() const { return n_in_; }
+    int n_params() const { return n_params_; }
 
 private:
     void load_input(const double* w_minus, bool on_device);
     void forward();
     void refresh_weights();
     fts_context* ctx_;
-    int nx_, ny_, nz_, M_, n_chunks_;
-    double in_scale_, out_scale_;
+    int nx_, ny_, nz_, n_in_, n_params_, M_, n_chunks_;
+    std::vector<double> in_scales_;
+    double out_scale_;
     long long step_ = 0;
     long long version_ = 0;
     DeviceBuffer params_, grads_, adam_m_, adam_v_, wf_, wb_, x_, target_, y_, dy_, acts_, grad_a_, grad_b_, partial_,

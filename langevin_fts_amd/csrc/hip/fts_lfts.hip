@@ -138,6 +138,23 @@ __global__ void add_difference_kernel(double* x, const double* a, const double* 
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < M; i += gridDim.x * blockDim.x) x[i] += a[i] - b[i];
 }
 
+// inputs of the state guess: [w-, w- - w-_backup, w+_backup] (3 M values), and the
+// target w+ - w+_backup when target is given
+__global__ void state_inputs_kernel(const double* w_real, const double* b_real, const double* w_imag,
+                                    const double* b_imag, double* in, double* target, int M) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < M; i += gridDim.x * blockDim.x) {
+        in[i] = w_real[i];
+        in[M + i] = w_real[i] - b_real[i];
+        in[2 * (size_t)M + i] = b_imag[i];
+        if (target) target[i] = w_imag[i] - b_imag[i];
+    }
+}
+
+// w+ = w+_backup + predicted change
+__global__ void add_kernel(double* out, const double* a, const double* b, int M) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < M; i += gridDim.x * blockDim.x) out[i] = a[i] + b[i];
+}
+
 }  // namespace
 
 SaddlePoint::SaddlePoint(fts_context* ctx, SolverBase* solver, std::vector<AndersonMixingDevice*> ams, int M,
@@ -243,6 +260,7 @@ void SaddlePoint::get_noise(double* noise, bool on_device) {
 
 void SaddlePoint::backup() {
     backup_field_version_ = field_version_;
+    has_backup_ = true;
     const size_t b = sizeof(double) * M_ * S_ * NR_;
     FTS_HIP_CHECK(hipMemcpyAsync(w_backup_.get(), w_aux_.get(), b, hipMemcpyDeviceToDevice, ctx_->stream));
     FTS_HIP_CHECK(hipMemcpyAsync(phi_backup_.get(), phi_.get(), b, hipMemcpyDeviceToDevice, ctx_->stream));
@@ -370,6 +388,9 @@ void SaddlePoint::find(double tolerance, int max_iter, int verbose, double* log,
     // low-precision phase while the errors are large (none when no low-precision solver is set)
     bool low = lo_ != nullptr;
     for (int it = 1; it <= max_iter; it++) {
+        // the last permitted iteration always evaluates in FP64: an error measured by
+        // the low-precision solver never ends the loop as a converged saddle point
+        if (it == max_iter) low = false;
         SolverBase* solver = low ? lo_ : solver_;
         concentrations(solver);
         // dH/dw of the imaginary fields, sign already flipped (lfts.py:219-245)
@@ -504,6 +525,7 @@ void SaddlePoint::prepare_guess(Cnn& cnn) {
     FTS_REQUIRE(NR_ == 1, "the learned guess runs on a single replica");
     FTS_REQUIRE(R_ == 1 && I_ == 1, "the learned guess maps one real field to one imaginary field");
     FTS_REQUIRE(cnn.n_grid() == M_, "the network's grid differs from the simulation grid");
+    FTS_REQUIRE(cnn.n_in() == 1, "prepare_guess is for the one-input (delta) guess");
     if (guess_old_.bytes() == 0) guess_old_.allocate(sizeof(double) * M_);
     // f(w_real) is often already there: the previous step's prediction was made from this
     // same field when that step was accepted, and a restored step keeps the prepared one
@@ -520,13 +542,36 @@ void SaddlePoint::prepare_guess(Cnn& cnn) {
     prepared_ = true;
 }
 
-void SaddlePoint::predict_guess(Cnn& cnn, bool delta) {
+void SaddlePoint::state_inputs(double* in, double* target) {
+    FTS_REQUIRE(has_backup_, "the state guess needs a backup of the previous step's fields");
+    const double* w = w_aux_.as<const double>();
+    const double* b = w_backup_.as<const double>();
+    const size_t re = (size_t)real_idx_[0] * M_, im = (size_t)imag_idx_[0] * M_;
+    hipLaunchKernelGGL(state_inputs_kernel, dim3(grid_blocks(M_)), dim3(256), 0, ctx_->stream, w + re, b + re, w + im,
+                       b + im, in, target, M_);
+    FTS_HIP_CHECK(hipGetLastError());
+}
+
+void SaddlePoint::predict_guess(Cnn& cnn, int mode) {
     FTS_REQUIRE(NR_ == 1, "the learned guess runs on a single replica");
     FTS_REQUIRE(R_ == 1 && I_ == 1, "the learned guess maps one real field to one imaginary field");
     FTS_REQUIRE(cnn.n_grid() == M_, "the network's grid differs from the simulation grid");
     double* w_real = w_aux_.as<double>() + (size_t)real_idx_[0] * M_;
     double* w_imag = w_aux_.as<double>() + (size_t)imag_idx_[0] * M_;
-    if (!delta) {
+    if (mode == 2) {
+        FTS_REQUIRE(cnn.n_in() == 3, "the state guess needs a three-input network");
+        if (guess_in_.bytes() == 0) guess_in_.allocate(sizeof(double) * 3 * M_);
+        if (guess_new_.bytes() == 0) guess_new_.allocate(sizeof(double) * M_);
+        state_inputs(guess_in_.as<double>(), nullptr);
+        cnn.predict(guess_in_.as<const double>(), guess_new_.as<double>(), true);
+        hipLaunchKernelGGL(add_kernel, dim3(grid_blocks(M_)), dim3(256), 0, ctx_->stream, w_imag,
+                           w_backup_.as<const double>() + (size_t)imag_idx_[0] * M_, guess_new_.as<const double>(),
+                           M_);
+        FTS_HIP_CHECK(hipGetLastError());
+        return;
+    }
+    FTS_REQUIRE(cnn.n_in() == 1, "the absolute and delta guesses need a one-input network");
+    if (mode == 0) {
         cnn.predict(w_real, w_imag, true);
         return;
     }
@@ -544,19 +589,28 @@ double SaddlePoint::train_guess(Cnn& cnn, double lr, int iters, int replay) {
     FTS_REQUIRE(NR_ == 1, "the learned guess runs on a single replica");
     FTS_REQUIRE(R_ == 1 && I_ == 1, "the learned guess maps one real field to one imaginary field");
     FTS_REQUIRE(cnn.n_grid() == M_, "the network's grid differs from the simulation grid");
+    FTS_REQUIRE(cnn.n_in() == 1 || cnn.n_in() == 3, "the guess trains one- or three-input networks");
     FTS_REQUIRE(iters >= 1 && replay >= 1, "iterations and replay size must be positive");
     hipStream_t st = ctx_->stream;
-    if (replay != replay_cap_) {
-        replay_.allocate(sizeof(double) * 2 * (size_t)M_ * replay);
+    // a sample: the network's inputs, then its target
+    const int per = cnn.n_in() + 1;
+    if (replay != replay_cap_ || per != replay_per_) {
+        replay_.allocate(sizeof(double) * per * (size_t)M_ * replay);
         replay_cap_ = replay;
+        replay_per_ = per;
         replay_n_ = replay_head_ = 0;
     }
-    // newest pair into the ring
-    double* slot = replay_.as<double>() + (size_t)replay_head_ * 2 * M_;
-    FTS_HIP_CHECK(hipMemcpyAsync(slot, w_aux_.as<double>() + (size_t)real_idx_[0] * M_, sizeof(double) * M_,
-                                 hipMemcpyDeviceToDevice, st));
-    FTS_HIP_CHECK(hipMemcpyAsync(slot + M_, w_aux_.as<double>() + (size_t)imag_idx_[0] * M_, sizeof(double) * M_,
-                                 hipMemcpyDeviceToDevice, st));
+    // newest sample into the ring
+    double* slot = replay_.as<double>() + (size_t)replay_head_ * per * M_;
+    if (per == 4) {
+        // inputs [w-, w- - w-_backup, w+_backup], target w+ - w+_backup
+        state_inputs(slot, slot + 3 * (size_t)M_);
+    } else {
+        FTS_HIP_CHECK(hipMemcpyAsync(slot, w_aux_.as<double>() + (size_t)real_idx_[0] * M_, sizeof(double) * M_,
+                                     hipMemcpyDeviceToDevice, st));
+        FTS_HIP_CHECK(hipMemcpyAsync(slot + M_, w_aux_.as<double>() + (size_t)imag_idx_[0] * M_,
+                                     sizeof(double) * M_, hipMemcpyDeviceToDevice, st));
+    }
     const int newest = replay_head_;
     replay_head_ = (replay_head_ + 1) % replay_cap_;
     replay_n_ = std::min(replay_n_ + 1, replay_cap_);
@@ -564,14 +618,14 @@ double SaddlePoint::train_guess(Cnn& cnn, double lr, int iters, int replay) {
     for (int i = 0; i < iters; i++) {
         int k = newest;
         if (i > 0) {
-            // xorshift64 pick among the stored pairs
+            // xorshift64 pick among the stored samples
             replay_rng_ ^= replay_rng_ << 13;
             replay_rng_ ^= replay_rng_ >> 7;
             replay_rng_ ^= replay_rng_ << 17;
             k = (int)(replay_rng_ % (unsigned long long)replay_n_);
         }
-        const double* pair = replay_.as<double>() + (size_t)k * 2 * M_;
-        const double l = cnn.train_step(pair, pair + M_, true, lr);
+        const double* sample = replay_.as<double>() + (size_t)k * per * M_;
+        const double l = cnn.train_step(sample, sample + (size_t)(per - 1) * M_, true, lr);
         if (i == 0) loss = l;
     }
     return loss;

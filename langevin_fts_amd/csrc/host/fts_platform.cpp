@@ -726,8 +726,8 @@ void DeviceSaddlePoint::set_low_precision_solver(PropagatorComputation* lo, doub
 }
 
 void DeviceSaddlePoint::prepare_guess(LearnedGuess& g) { check_fts(fts_saddle_prepare_guess(saddle_, g.handle())); }
-void DeviceSaddlePoint::predict_guess(LearnedGuess& g, bool delta) {
-    check_fts(fts_saddle_predict_guess(saddle_, g.handle(), delta ? 1 : 0));
+void DeviceSaddlePoint::predict_guess(LearnedGuess& g, int mode) {
+    check_fts(fts_saddle_predict_guess(saddle_, g.handle(), mode));
 }
 double DeviceSaddlePoint::train_guess(LearnedGuess& g, double lr, int iters, int replay) {
     double loss = 0.0;
@@ -736,10 +736,13 @@ double DeviceSaddlePoint::train_guess(LearnedGuess& g, double lr, int iters, int
 }
 
 // ------------------------------------------------------------------ learned guess
-LearnedGuess::LearnedGuess(std::vector<int> nx, double in_scale, double out_scale) {
+LearnedGuess::LearnedGuess(std::vector<int> nx, std::vector<double> in_scales, double out_scale) {
     if (nx.size() != 3) LFTS_THROW("the learned guess needs a 3D grid");
+    if (in_scales.empty()) LFTS_THROW("the learned guess needs at least one input scale");
     M_ = nx[0] * nx[1] * nx[2];
-    check_fts(fts_cnn_create(hip_context(), nx.data(), in_scale, out_scale, &cnn_));
+    n_in_ = (int)in_scales.size();
+    check_fts(fts_cnn_create_inputs(hip_context(), nx.data(), n_in_, in_scales.data(), out_scale, &cnn_));
+    check_fts(fts_cnn_param_count(cnn_, &n_params_));
 }
 LearnedGuess::~LearnedGuess() {
     if (cnn_) fts_cnn_destroy(cnn_);

@@ -157,14 +157,17 @@ struct FieldTheoryTables {
     double h_const = 0.0;
 };
 
-// Periodic 3D CNN of the learned saddle-point guess (w- -> w+); weights on the device.
+// Periodic 3D CNN of the learned saddle-point guess; weights on the device. One input
+// (w- -> w+) or several (in_scales.size() channels, e.g. the state guess's w-, its
+// change and the previous w+ -> the change of w+).
 class LearnedGuess {
 public:
-    LearnedGuess(std::vector<int> nx, double in_scale, double out_scale);
+    LearnedGuess(std::vector<int> nx, std::vector<double> in_scales, double out_scale);
     ~LearnedGuess();
     LearnedGuess(const LearnedGuess&) = delete;
     LearnedGuess& operator=(const LearnedGuess&) = delete;
-    static int n_params() { return fts_cnn_n_params(); }
+    int n_params() const { return n_params_; }
+    int n_inputs() const { return n_in_; }
     int n_grid() const { return M_; }
     void set_params(const float* p);
     void get_params(float* p);
@@ -174,7 +177,7 @@ public:
     fts_cnn* handle() const { return cnn_; }
 
 private:
-    int M_;
+    int M_, n_in_, n_params_ = 0;
     fts_cnn* cnn_ = nullptr;
 };
 
@@ -206,7 +209,8 @@ public:
     void langevin_update(const double* noise, double dt);
     void field_moments(double* mean, double* mean_sq);
     void prepare_guess(LearnedGuess& g);
-    void predict_guess(LearnedGuess& g, bool delta);
+    // mode 0: absolute, 1: delta (one-input networks), 2: state (three-input network)
+    void predict_guess(LearnedGuess& g, int mode);
     // iterations on `lo` (an FP32 solver of the same molecules) while the error is at or above switch_error
     void set_low_precision_solver(PropagatorComputation* lo, double switch_error);
     double train_guess(LearnedGuess& g, double lr, int iters, int replay);

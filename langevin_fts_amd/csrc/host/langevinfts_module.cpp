@@ -374,7 +374,7 @@ PYBIND11_MODULE(langevinfts, m) {
         .def("set_low_precision_solver", &DeviceSaddlePoint::set_low_precision_solver, py::arg("solver"),
              py::arg("switch_error"), py::keep_alive<1, 2>())
         .def("prepare_guess", &DeviceSaddlePoint::prepare_guess)
-        .def("predict_guess", &DeviceSaddlePoint::predict_guess, py::arg("guess"), py::arg("delta") = false)
+        .def("predict_guess", &DeviceSaddlePoint::predict_guess, py::arg("guess"), py::arg("mode") = 0)
         .def("train_guess", &DeviceSaddlePoint::train_guess, py::arg("guess"), py::arg("lr"), py::arg("iters") = 1,
              py::arg("replay") = 1);
 
@@ -382,37 +382,48 @@ PYBIND11_MODULE(langevinfts, m) {
     // the reference's surface).
     using farr = py::array_t<float, py::array::c_style | py::array::forcecast>;
     py::class_<LearnedGuess>(m, "LearnedGuess")
-        .def(py::init<std::vector<int>, double, double>(), py::arg("nx"), py::arg("in_scale") = 1.0,
-             py::arg("out_scale") = 1.0)
-        .def_static("n_params", &LearnedGuess::n_params)
+        .def(py::init([](std::vector<int> nx, py::object in_scale, double out_scale) {
+                 std::vector<double> scales;
+                 if (py::isinstance<py::float_>(in_scale) || py::isinstance<py::int_>(in_scale))
+                     scales.push_back(in_scale.cast<double>());
+                 else
+                     scales = in_scale.cast<std::vector<double>>();
+                 return new LearnedGuess(nx, scales, out_scale);
+             }),
+             py::arg("nx"), py::arg("in_scale") = 1.0, py::arg("out_scale") = 1.0)
+        .def_static("n_params_one_input", []() { return fts_cnn_n_params(); })
+        .def("n_params", &LearnedGuess::n_params)
+        .def("n_inputs", &LearnedGuess::n_inputs)
         .def("set_params",
              [](LearnedGuess& o, const farr& p) {
-                 if (p.size() != LearnedGuess::n_params())
-                     LFTS_THROW("expected " + std::to_string(LearnedGuess::n_params()) + " parameters");
+                 if (p.size() != o.n_params()) LFTS_THROW("expected " + std::to_string(o.n_params()) + " parameters");
                  o.set_params(p.data());
              })
         .def("get_params",
              [](LearnedGuess& o) {
-                 farr p(LearnedGuess::n_params());
+                 farr p(o.n_params());
                  o.get_params(p.mutable_data());
                  return p;
              })
         .def("predict",
              [](LearnedGuess& o, const darr& w) {
-                 if (w.size() != o.n_grid()) LFTS_THROW("w_minus must have n_grid values");
+                 if (w.size() != (py::ssize_t)o.n_inputs() * o.n_grid())
+                     LFTS_THROW("the inputs must have n_inputs * n_grid values");
                  py::array_t<double> out(o.n_grid());
                  o.predict(w.data(), out.mutable_data());
                  return out;
              })
         .def("gradients",
              [](LearnedGuess& o, const darr& w, const darr& t) {
-                 if (w.size() != o.n_grid() || t.size() != o.n_grid()) LFTS_THROW("fields must have n_grid values");
-                 farr g(LearnedGuess::n_params());
+                 if (w.size() != (py::ssize_t)o.n_inputs() * o.n_grid() || t.size() != o.n_grid())
+                     LFTS_THROW("the inputs must have n_inputs * n_grid values and the target n_grid");
+                 farr g(o.n_params());
                  const double loss = o.gradients(w.data(), t.data(), g.mutable_data());
                  return py::make_tuple(loss, g);
              })
         .def("train_step", [](LearnedGuess& o, const darr& w, const darr& t, double lr) {
-            if (w.size() != o.n_grid() || t.size() != o.n_grid()) LFTS_THROW("fields must have n_grid values");
+            if (w.size() != (py::ssize_t)o.n_inputs() * o.n_grid() || t.size() != o.n_grid())
+                LFTS_THROW("the inputs must have n_inputs * n_grid values and the target n_grid");
             return o.train_step(w.data(), t.data(), lr);
         });
 

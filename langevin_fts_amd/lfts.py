@@ -204,18 +204,19 @@ class SymmetricPolymerTheory:
         return dH
 
 
-def guess_initial_params(seed):
+def guess_initial_params(seed, n_in=1):
     """He-normal weights and zero biases for the learned-guess network (layout of
-    LearnedGuess: W0[27][32], b0[32], 4 x (W[27][32][32], b[32]), WL[27][32], bL)."""
+    LearnedGuess: W0[n_in][27][32], b0[32], 4 x (W[27][32][32], b[32]), WL[27][32], bL)."""
     C, T = 32, 27
     rng = np.random.default_rng(seed)
-    parts = [rng.normal(0.0, np.sqrt(2.0 / T), T * C), np.zeros(C)]
+    parts = [rng.normal(0.0, np.sqrt(2.0 / (T * n_in)), n_in * T * C), np.zeros(C)]
     for _ in range(4):
         parts += [rng.normal(0.0, np.sqrt(2.0 / (T * C)), T * C * C), np.zeros(C)]
     parts += [rng.normal(0.0, np.sqrt(1.0 / (T * C)), T * C), np.zeros(1)]
-    p = np.concatenate(parts).astype(np.float32)
-    assert p.size == lf.LearnedGuess.n_params()
-    return p
+    return np.concatenate(parts).astype(np.float32)
+
+
+GUESS_MODES = {"absolute": 0, "delta": 1, "state": 2}
 
 
 class NoiseStream:
@@ -257,9 +258,12 @@ class NoiseStream:
 
 class LFTS:
     """L-FTS driver; `params` has the keys of the reference examples (fts/Lamella.py).
-    An optional "learned_guess" entry {"train_steps", "lr", "lr_final", "train_iters", "seed"}
-    trains a periodic 3D CNN on the (w-, w+) saddle points of the first train_steps
-    accepted Langevin steps and then starts every saddle point from its prediction."""
+    An optional "learned_guess" entry {"train_steps", "lr", "lr_final", "train_iters",
+    "replay", "inputs", "seed"} trains a periodic 3D CNN on the saddle points of the
+    first train_steps accepted Langevin steps and then starts every saddle point from
+    its prediction. "inputs": "state" (default: the network sees w-, its change in the
+    Langevin update and the previous w+, and predicts the change of w+), "delta" (a
+    network f(w-) ~ w+, used as w+ += f(w-_new) - f(w-_old)) or "absolute" (w+ = f(w-))."""
 
     def __init__(self, params, random_seed=None, n_replicas=1):
         """n_replicas > 1: that many independent runs of the same system advance together,
@@ -418,7 +422,7 @@ class LFTS:
         self.langevin.update({"sigma": langevin_sigma})
         self.verbose_level = params["verbose_level"]
         # learned initial guess of w+ (optional):
-        # {"train_steps", "lr", "lr_final", "train_iters", "replay", "delta", "seed"}
+        # {"train_steps", "lr", "lr_final", "train_iters", "replay", "inputs", "seed"}
         self.learned = dict(params["learned_guess"]) if params.get("learned_guess") else None
         if self.learned is not None:
             assert len(self.mpt.aux_fields_real_idx) == 1 and len(self.mpt.aux_fields_imag_idx) == 1, \
@@ -429,9 +433,14 @@ class LFTS:
             self.learned.setdefault("lr_final", self.learned["lr"] / 10)
             self.learned.setdefault("train_iters", 16)
             self.learned.setdefault("replay", 64)
-            self.learned.setdefault("delta", True)
+            if "inputs" not in self.learned:
+                # the earlier boolean form: "delta": True / False
+                self.learned["inputs"] = "delta" if self.learned.get("delta", True) is True else \
+                    ("absolute" if "delta" in self.learned else "state")
+            assert self.learned["inputs"] in GUESS_MODES, "inputs must be state, delta or absolute"
             self.learned.setdefault("seed", 0)
         self.guess = None
+        self._guess_prev_fields = None
         self.saddle = params["saddle"].copy()
         self.recording = params["recording"].copy()
         self.cb = cb
@@ -473,17 +482,40 @@ class LFTS:
 
     # ---------------------------------------------------------------- learned guess
     def init_guess(self):
-        """Network for w+ = f(w-), scaled by the current fields' standard deviations,
-        with He-normal weights from the configured seed."""
+        """The network and its scales, with He-normal weights from the configured seed.
+        One input (delta, absolute): w- -> w+, scaled by the current fields' standard
+        deviations. State: [w-, w- - w-_prev, w+_prev] -> w+ - w+_prev, scaled by the
+        standard deviations of the first training sample (the current converged step
+        and the previous step's fields, kept by the driver for this)."""
         w_aux = self.saddle_device.get_fields()
-        S = len(self.monomer_types)
         ri, ii = self.mpt.aux_fields_real_idx[0], self.mpt.aux_fields_imag_idx[0]
-        in_scale = float(np.std(w_aux[ri])) or 1.0
-        out_scale = float(np.std(w_aux[ii])) or 1.0
-        self.guess = lf.LearnedGuess(self.cb.get_nx(), in_scale, out_scale)
-        self.guess.set_params(guess_initial_params(self.learned["seed"]))
+        if self.learned["inputs"] == "state":
+            prev = self._guess_prev_fields
+            assert prev is not None, "the state guess trains on a step whose previous fields were kept"
+            in_scales = [float(np.std(w_aux[ri])) or 1.0, float(np.std(w_aux[ri] - prev[ri])) or 1.0,
+                         float(np.std(prev[ii])) or 1.0]
+            out_scale = float(np.std(w_aux[ii] - prev[ii])) or 1.0
+        else:
+            in_scales = [float(np.std(w_aux[ri])) or 1.0]
+            out_scale = float(np.std(w_aux[ii])) or 1.0
+        self.guess = lf.LearnedGuess(self.cb.get_nx(), in_scales, out_scale)
+        self.guess.set_params(guess_initial_params(self.learned["seed"], len(in_scales)))
         self.guess_trained_steps = 0
-        return in_scale, out_scale
+        return in_scales, out_scale
+
+    def _keep_fields_for_guess(self):
+        """Before the first training step's Langevin update: the fields the state
+        guess's scales are measured against."""
+        if self.learned is not None and self.guess is None and self.learned["inputs"] == "state":
+            self._guess_prev_fields = self.saddle_device.get_fields()
+
+    def _guess_before_update(self, ready):
+        if ready and self.learned["inputs"] == "delta":
+            self.saddle_device.prepare_guess(self.guess)
+
+    def _guess_after_update(self, ready):
+        if ready:
+            self.saddle_device.predict_guess(self.guess, GUESS_MODES[self.learned["inputs"]])
 
     def train_guess(self):
         """Adam steps on the current (w-, w+*) pair and on pairs replayed from a device
@@ -540,11 +572,10 @@ class LFTS:
         dev = self.saddle_device
         dev.backup()
         ready = self.guess_ready()
-        if ready and self.learned["delta"]:
-            dev.prepare_guess(self.guess)
+        self._keep_fields_for_guess()
+        self._guess_before_update(ready)
         dev.langevin_update(noise, self.langevin["dt"])
-        if ready:
-            dev.predict_guess(self.guess, self.learned["delta"])
+        self._guess_after_update(ready)
         n, err, _ = self._find_saddle_point_device()
         ok = not (np.isnan(err) or err >= self.saddle["tolerance"])
         if not ok:
@@ -572,6 +603,28 @@ class LFTS:
                 print("%13.7E" % (e), end=" ")
             print("]")
         return n_iter, error_level, hamiltonian
+
+    def structure_function_terms(self, w_aux, phi):
+        """One sample of the structure functions, S_ij(k) += mu_i(k) conj(phi_j(k)) for
+        every monomer pair i <= j (lfts.py:753-766): mu_i(k) = sum over the real-field
+        eigenmodes k < S-1 of FFT(w_aux[k]) A_inv[k, i] / eigenvalue_k / M and
+        phi_j(k) = FFT(phi_j) / M, on the half spectrum of the grid."""
+        S = len(self.monomer_types)
+        nx = self.cb.get_nx()
+        M = self.cb.get_n_grid()
+        mu_k, phi_k = {}, {}
+        w_aux_k = [np.fft.rfftn(np.reshape(w_aux[k], nx)) for k in range(S - 1)]
+        for i, t in enumerate(self.monomer_types):
+            phi_k[t] = np.fft.rfftn(np.reshape(phi[t], nx)) / M
+            mu_k[t] = np.zeros_like(phi_k[t], np.complex128)
+            for k in range(S - 1):
+                mu_k[t] += w_aux_k[k] * self.mpt.matrix_a_inv[k, i] / self.mpt.eigenvalues[k] / M
+        out = {}
+        for a, b in itertools.combinations_with_replacement(range(S), 2):
+            key = self.monomer_types[a] + "," + self.monomer_types[b]
+            x, y = sorted(key.split(","))
+            out[key] = mu_k[x] * np.conj(phi_k[y])
+        return out
 
     def save_simulation_data(self, path, w, phi, langevin_step, normal_noise_prev):
         """`.mat` checkpoint with the keys of lfts.py:569-598."""
@@ -662,11 +715,10 @@ class LFTS:
             dev.backup()
             noise = self.next_noise()
             ready = self.guess_ready()
-            if ready and self.learned["delta"]:
-                dev.prepare_guess(self.guess)
+            self._keep_fields_for_guess()
+            self._guess_before_update(ready)
             dev.langevin_update(noise, self.langevin["dt"])
-            if ready:
-                dev.predict_guess(self.guess, self.learned["delta"])
+            self._guess_after_update(ready)
             saddle_iter, error_level, hamiltonian = self._find_saddle_point_device()
             total_saddle_iter += saddle_iter
             total_error_level += error_level
@@ -705,19 +757,9 @@ class LFTS:
                 dH_history = {key: [] for key in self.chi_n}
 
             if computing:
-                # S_ij(k) += mu_i(k) conj(phi_j(k)) (lfts.py:753-766)
-                w_aux = dev.get_fields()
-                phi = self._phi_from_device()
-                mu_k, phi_k = {}, {}
-                w_aux_k = [np.fft.rfftn(np.reshape(w_aux[k], nx)) for k in range(S - 1)]
-                for i, t in enumerate(self.monomer_types):
-                    phi_k[t] = np.fft.rfftn(np.reshape(phi[t], nx)) / M
-                    mu_k[t] = np.zeros_like(phi_k[t], np.complex128)
-                    for k in range(S - 1):
-                        mu_k[t] += w_aux_k[k] * self.mpt.matrix_a_inv[k, i] / self.mpt.eigenvalues[k] / M
+                terms = self.structure_function_terms(dev.get_fields(), self._phi_from_device())
                 for key in sf_average:
-                    a, b = sorted(key.split(","))
-                    sf_average[key] += mu_k[a] * np.conj(phi_k[b])
+                    sf_average[key] += terms[key]
             if recording_sf:
                 from scipy.io import savemat
 

@@ -262,6 +262,12 @@ def molecule_set():
                                     "polymers": [[1.0, [["A", 0.36, 0, 1], ["B", 0.64, 1, 2]], {}]]}
     mols["diblock_f0.5_ds1_90"] = {"ds": 1.0 / 90, "bonds": {"A": 1.0, "B": 1.0},
                                    "polymers": [[1.0, [["A", 0.5, 0, 1], ["B", 0.5, 1, 2]], {}]]}
+    mols["diblock_f0.4_ds0.01"] = {"ds": 0.01, "bonds": {"A": 1.0, "B": 1.0},
+                                   "polymers": [[1.0, [["A", 0.4, 0, 1], ["B", 0.6, 1, 2]], {}]]}
+    mols["diblock_f0.4_ds1_90"] = {"ds": 1.0 / 90, "bonds": {"A": 1.0, "B": 1.0},
+                                   "polymers": [[1.0, [["A", 0.4, 0, 1], ["B", 0.6, 1, 2]], {}]]}
+    mols["diblock_f0.5_ds1_64"] = {"ds": 1.0 / 64, "bonds": {"A": 1.0, "B": 1.0},
+                                   "polymers": [[1.0, [["A", 0.5, 0, 1], ["B", 0.5, 1, 2]], {}]]}
     mols["star_abc_homo_a"] = {"ds": 0.01, "bonds": {"A": 1.0, "B": 1.0, "C": 1.2},
                                "polymers": [[0.8, [["A", 0.3, 0, 1], ["B", 0.3, 0, 2], ["C", 0.4, 0, 3]], {}],
                                             [0.2, [["A", 0.5, 0, 1]], {}]]}

@@ -20,18 +20,21 @@ def test_bf16_rounding():
         assert abs(g - v) <= abs(v) * 2.0 ** -8
 
 
-def test_forward_matches_explicit_loops():
+@pytest.mark.parametrize("n_in", [1, 3])
+def test_forward_matches_explicit_loops(n_in):
     rng = np.random.default_rng(0)
-    p = K.init_params(1).astype(np.float64)
-    p[K.P_B0:K.P_WH] = rng.normal(0, 0.1, K.C)
-    x = rng.standard_normal((4, 5, 6))
-    y, acts = K.forward(p, x, rounding=False)
-    w0, b0, hid, wl, bl = K.unpack(p)
+    p = K.init_params(1, n_in).astype(np.float64)
+    w0o, b0o, who, _, _, _ = K.layout(n_in)
+    p[b0o:who] = rng.normal(0, 0.1, K.C)
+    x = rng.standard_normal((n_in, 4, 5, 6))
+    y, acts = K.forward(p, x if n_in > 1 else x[0], rounding=False)
+    w0, b0, hid, wl, bl = K.unpack(p, n_in)
     v = (1, 4, 0)
-    # first layer at one voxel
+    # first layer at one voxel, every input channel
     z0 = b0.copy()
-    for t, (dx, dy, dz) in enumerate(K.OFFSETS):
-        z0 += w0[t] * x[(v[0] + dx) % 4, (v[1] + dy) % 5, (v[2] + dz) % 6]
+    for c in range(n_in):
+        for t, (dx, dy, dz) in enumerate(K.OFFSETS):
+            z0 += w0[c, t] * x[c, (v[0] + dx) % 4, (v[1] + dy) % 5, (v[2] + dz) % 6]
     np.testing.assert_allclose(acts[0][v], np.maximum(z0, 0), rtol=1e-13, atol=1e-13)
     # one hidden layer at one voxel
     w, b = hid[2]
@@ -49,24 +52,26 @@ def test_forward_matches_explicit_loops():
     assert abs(out - y[v]) < 1e-12
 
 
-@pytest.mark.parametrize("block", ["W0", "b0", "W1", "b1", "W4", "b4", "WL", "bL"])
-def test_gradients_match_finite_differences(block):
+@pytest.mark.parametrize("block,n_in", [("W0", 1), ("b0", 1), ("W1", 1), ("b1", 1), ("W4", 1), ("b4", 1),
+                                        ("WL", 1), ("bL", 1), ("W0", 3), ("b0", 3), ("W1", 3), ("WL", 3)])
+def test_gradients_match_finite_differences(block, n_in):
     rng = np.random.default_rng(2)
-    p = K.init_params(3).astype(np.float64)
-    p[K.P_B0:K.P_WH] = 0.05
+    w0o, b0o, who, wlo, blo, _ = K.layout(n_in)
+    p = K.init_params(3, n_in).astype(np.float64)
+    p[b0o:who] = 0.05
     for l in range(K.NHID):
-        p[K.P_WH + l * K.HSZ + K.TAPS * K.C * K.C:K.P_WH + (l + 1) * K.HSZ] = 0.05
-    x = rng.standard_normal((4, 4, 3))
+        p[who + l * K.HSZ + K.TAPS * K.C * K.C:who + (l + 1) * K.HSZ] = 0.05
+    x = rng.standard_normal((n_in, 4, 4, 3))
     target = rng.standard_normal((4, 4, 3))
     _, g = K.loss_and_gradients(p, x, target, rounding=False)
-    lo = {"W0": K.P_W0, "b0": K.P_B0, "WL": K.P_WL, "bL": K.P_BL}
+    lo = {"W0": w0o, "b0": b0o, "WL": wlo, "bL": blo}
     hid = {"W1": (0, 0), "b1": (0, 1), "W4": (3, 0), "b4": (3, 1)}
     if block in lo:
         start = lo[block]
-        size = {"W0": K.TAPS * K.C, "b0": K.C, "WL": K.TAPS * K.C, "bL": 1}[block]
+        size = {"W0": n_in * K.TAPS * K.C, "b0": K.C, "WL": K.TAPS * K.C, "bL": 1}[block]
     else:
         l, part = hid[block]
-        start = K.P_WH + l * K.HSZ + (K.TAPS * K.C * K.C if part else 0)
+        start = who + l * K.HSZ + (K.TAPS * K.C * K.C if part else 0)
         size = K.C if part else K.TAPS * K.C * K.C
     idx = start + rng.choice(size, size=min(6, size), replace=False)
     h = 1e-6

@@ -49,7 +49,7 @@ def rel(a, b):
 
 
 @pytest.mark.parametrize("shape", [(5, 4, 3), (32, 32, 32), (64, 64, 64), (48, 40, 36), (7, 11, 13),
-                                   (23, 27, 25), (12, 10), (16,), (96, 96, 96)])
+                                   (23, 27, 25), (12, 10), (16,), (96, 96, 96), (128, 128, 128)])
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
 def test_fft_matches_numpy(ctx, capi, shape, prec):
     if prec == "fp32" and any(f > 5 for n in shape for f in _prime_factors(n)):
@@ -151,11 +151,13 @@ def test_fp64_matches_oracle(ctx, capi, name, model, agg):
     assert rel(s.get_stress(), ref.get_stress()) < 1e-10
 
 
-@pytest.mark.parametrize("n", [32, 48])
+@pytest.mark.parametrize("n", [32, 48, 64, 96, 128])
 @pytest.mark.parametrize("name,model", [("diblock_f0.5_ds1_90", "discrete"), ("branched", "continuous"),
                                         ("star_abc_homo_a", "continuous")])
 def test_specialised_cubic_kernels_match_oracle(ctx, capi, n, name, model):
-    """N^3 grids run the compile-time specialised slab/pencil kernels."""
+    """N^3 grids run the compile-time specialised slab/pencil kernels of that N."""
+    if name == "star_abc_homo_a" and n > 64:
+        pytest.skip("the star's 220 contour steps at 96^3/128^3 are left to the diblock cases (oracle time)")
     say(f"cubic {n} {name} {model}")
     plan, ref, s = _pair(ctx, capi, name, model, 0, [n, n, n], [4.0, 4.0, 4.0], capi.FTS_FP64)
     for p in range(len(plan["polymers"])):
@@ -205,9 +207,9 @@ def test_schedule_width_does_not_change_results(ctx, capi, model):
         assert np.array_equal(o, outs[0])
 
 
-def test_mask_matches_oracle(ctx, capi):
-    nx, lx = [12, 10, 8], [3.0, 2.5, 2.0]
-    say("mask")
+@pytest.mark.parametrize("nx,lx", [([12, 10, 8], [3.0, 2.5, 2.0]), ([96, 96, 96], [4.5, 4.5, 4.5])])
+def test_mask_matches_oracle(ctx, capi, nx, lx):
+    say(f"mask {nx}")
     rng = np.random.default_rng(3)
     mask = (rng.random(int(np.prod(nx))) > 0.15).astype(np.float64)
     for model in ["continuous", "discrete"]:

@@ -18,14 +18,15 @@ def lf():
     return langevinfts
 
 
-def params(seed):
+def params(seed, n_in=1):
     rng = np.random.default_rng(seed + 100)
-    p = K.init_params(seed)
-    p[K.P_B0:K.P_WH] = rng.normal(0, 0.1, K.C)
+    w0o, b0o, who, wlo, blo, _ = K.layout(n_in)
+    p = K.init_params(seed, n_in)
+    p[b0o:who] = rng.normal(0, 0.1, K.C)
     for l in range(K.NHID):
-        b = K.P_WH + l * K.HSZ + K.TAPS * K.C * K.C
+        b = who + l * K.HSZ + K.TAPS * K.C * K.C
         p[b:b + K.C] = rng.normal(0, 0.1, K.C)
-    p[K.P_BL] = 0.05
+    p[blo] = 0.05
     return p
 
 
@@ -57,21 +58,36 @@ def test_scales(lf):
     assert rel_l2(net.predict(x.ravel()), 0.5 * y_ref.ravel()) < 3e-3
 
 
-@pytest.mark.parametrize("nx", [[16, 16, 16], [8, 12, 16]])
-def test_gradients_match_oracle(lf, nx):
-    say(f"cnn gradients {nx}")
-    p = params(5)
+def test_three_input_forward_and_scales(lf):
+    """The state guess's network: three input channels, each with its own scale."""
+    say("cnn three inputs")
+    nx = [16, 8, 16]
+    p = params(11, 3)
+    scales = [2.0, 0.5, 3.0]
+    net = lf.LearnedGuess(nx, scales, 0.25)
+    assert net.n_inputs() == 3 and net.n_params() == K.n_params(3)
+    net.set_params(p)
+    x = np.random.default_rng(12).standard_normal([3] + nx)
+    y_ref, _ = K.forward(p, x / np.array(scales)[:, None, None, None])
+    assert rel_l2(net.predict(x.ravel()), 0.25 * y_ref.ravel()) < 3e-3
+
+
+@pytest.mark.parametrize("nx,n_in", [([16, 16, 16], 1), ([8, 12, 16], 1), ([8, 12, 16], 3)])
+def test_gradients_match_oracle(lf, nx, n_in):
+    say(f"cnn gradients {nx} inputs {n_in}")
+    p = params(5, n_in)
     rng = np.random.default_rng(6)
-    x = rng.standard_normal(nx)
-    t = 0.5 * np.roll(x, 1, axis=0) - 0.3 * x + 0.1 * rng.standard_normal(nx)
-    net = lf.LearnedGuess(nx, 1.0, 1.0)
+    x = rng.standard_normal([n_in] + nx)
+    t = 0.5 * np.roll(x[0], 1, axis=0) - 0.3 * x[-1] + 0.1 * rng.standard_normal(nx)
+    net = lf.LearnedGuess(nx, [1.0] * n_in, 1.0)
     net.set_params(p)
     loss, g = net.gradients(x.ravel(), t.ravel())
     loss_ref, g_ref = K.loss_and_gradients(p, x, t)
     assert abs(loss / loss_ref - 1) < 2e-3
-    blocks = {"W0": (K.P_W0, K.P_B0), "b0": (K.P_B0, K.P_WH), "WL": (K.P_WL, K.P_BL), "bL": (K.P_BL, K.N_PARAMS)}
+    w0o, b0o, who, wlo, blo, n = K.layout(n_in)
+    blocks = {"W0": (w0o, b0o), "b0": (b0o, who), "WL": (wlo, blo), "bL": (blo, n)}
     for l in range(K.NHID):
-        b = K.P_WH + l * K.HSZ
+        b = who + l * K.HSZ
         blocks[f"W{l + 1}"] = (b, b + K.TAPS * K.C * K.C)
         blocks[f"b{l + 1}"] = (b + K.TAPS * K.C * K.C, b + K.HSZ)
     for name, (lo, hi) in blocks.items():

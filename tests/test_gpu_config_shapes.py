@@ -1,0 +1,162 @@
+"""The HIP path against the FP64 oracle on the grids, boxes and molecules of the
+benchmark configurations, where N^3 grids run the compile-time specialised
+slab/pencil kernels of that N (each with its own radix plan, LDS pitch and
+launch shape):
+
+  C1   AB continuous f=0.5, ds=0.01, 32^3, L=4.0, w ~ N(0, 1)
+  C2   AB continuous f=0.36, ds=0.01, 64^3, L=3.3, 8-cosine fields (chiN=20); discrete N=100
+  C3   AB continuous f=0.4, ds=0.01 and discrete N=90, 96^3 (L=4.5) and 128^3 (L=6.0), w ~ N(0, 2^2)
+  C4   ABC 3-arm star + A homopolymer, ds=0.01, 64^3, L=4.0, w ~ N(0, 1)
+  C4b  AB discrete N=64, f=0.5, 48^3, L=4.0, w ~ N(0, 1)
+  C5   AB discrete N=90, f=0.5, 64^3, L=6.0, w ~ N(0, 1)
+
+FP64 bar (CpuComputationContinuous.cpp:148-726; tolerances as the reference's
+TestPseudoLinearContinuous3D.cpp:247-285 but tightened to the FP64 restatement):
+Q, phi and stress within 1e-10 relative, and Q(n) constant along every block
+within 1e-7 (check_total_partition, CpuComputationContinuous.cpp:755-817).
+The precision form ("hip-fp32": propagators and FFTs in FP32, Q/phi/stress in
+FP64) is measured against the same oracle result at the C3 shapes.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.conftest import golden
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+def rel(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def cosine_fields(n, L, chin, seed):
+    """w_A = -w_B = chiN/2 sum_{j=1..8} cos(k_j.r + phi_j), |k_j| = 2 pi/L sqrt(6)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    vecs = [np.array(v) for v in
+            {(sx * a, sy * b, sz * c) for a, b, c in [(1, 1, 2), (1, 2, 1), (2, 1, 1)]
+             for sx in (1, -1) for sy in (1, -1) for sz in (1, -1)}]
+    vecs.sort(key=tuple)
+    pick = rng.choice(len(vecs), 8, replace=False)
+    phase = rng.uniform(0, 2 * np.pi, 8)
+    x = np.arange(n) * (L / n)
+    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+    s = np.zeros((n, n, n))
+    for j, p in zip(pick, phase):
+        k = 2 * np.pi / L * vecs[j]
+        s += np.cos(k[0] * X + k[1] * Y + k[2] * Z + p)
+    wa = (chin / 2 * s).reshape(-1)
+    return {"A": wa, "B": -wa}
+
+
+# name: (plan, n, L, fields)
+CONFIGS = {
+    "C1": ("diblock_f0.5_ds0.01|continuous|1", 32, 4.0, ("normal", 1.0)),
+    "C2": ("diblock_f0.36_ds0.01|continuous|1", 64, 3.3, ("cosine", 20.0)),
+    "C2-discrete": ("diblock_f0.36_ds0.01|discrete|1", 64, 3.3, ("cosine", 20.0)),
+    "C3-96": ("diblock_f0.4_ds0.01|continuous|1", 96, 4.5, ("normal", 2.0)),
+    "C3-128": ("diblock_f0.4_ds0.01|continuous|1", 128, 6.0, ("normal", 2.0)),
+    "C3-discrete-96": ("diblock_f0.4_ds1_90|discrete|1", 96, 4.5, ("normal", 2.0)),
+    "C3-discrete-128": ("diblock_f0.4_ds1_90|discrete|1", 128, 6.0, ("normal", 2.0)),
+    "C4": ("star_abc_homo_a|continuous|1", 64, 4.0, ("normal", 1.0)),
+    "C4b": ("diblock_f0.5_ds1_64|discrete|1", 48, 4.0, ("normal", 1.0)),
+    "C5": ("diblock_f0.5_ds1_90|discrete|1", 64, 6.0, ("normal", 1.0)),
+}
+
+
+def fields(name, seed=2024):
+    key, n, L, (kind, scale) = CONFIGS[name]
+    bonds = golden("plans")[key]["input"]["bonds"]
+    if kind == "cosine":
+        return cosine_fields(n, L, scale, seed)
+    rng = np.random.Generator(np.random.PCG64(seed))
+    return {m: rng.normal(0.0, scale, n ** 3) for m in bonds}
+
+
+_oracle = {}
+
+
+def oracle_result(name):
+    """Q, phi per monomer and stress of the FP64 oracle (cached per config)."""
+    if name not in _oracle:
+        key, n, L, _ = CONFIGS[name]
+        plan = golden("plans")[key]
+        inp = plan["input"]
+        ref = O.Solver(O.Grid([n] * 3, [L] * 3), plan, inp["ds"], inp["bonds"], inp["solvents"])
+        ref.compute_statistics(fields(name))
+        ref.compute_stress()
+        _oracle[name] = {"Q": [ref.get_total_partition(p) for p in range(len(plan["polymers"]))],
+                         "phi": {m: ref.get_total_concentration(m) for m in inp["bonds"]},
+                         "stress": np.array(ref.get_stress())}
+    return _oracle[name]
+
+
+def hip_result(name, platform):
+    """The same evaluation through the reference's Python surface on a "hip" platform."""
+    from langevin_fts_amd import langevinfts as lf
+
+    key, n, L, _ = CONFIGS[name]
+    plan = golden("plans")[key]
+    inp = plan["input"]
+    factory = lf.PlatformSelector.create_factory(platform, False)
+    cb = factory.create_computation_box([n] * 3, [L] * 3)
+    mol = factory.create_molecules_information(plan["model"], inp["ds"], inp["bonds"])
+    for vf, blocks, _ in inp["polymers"]:
+        mol.add_polymer(vf, blocks)
+    solver = factory.create_pseudospectral_solver(cb, mol, factory.create_propagator_analyzer(mol, True))
+    solver.compute_statistics(fields(name))
+    out = {"Q": [solver.get_total_partition(p) for p in range(len(inp["polymers"]))],
+           "phi": {m: solver.get_total_concentration(m) for m in inp["bonds"]}}
+    out["partition_constant"] = solver.check_total_partition() if platform == "hip" else None
+    solver.compute_stress()
+    out["stress"] = np.array(solver.get_stress())
+    return out
+
+
+def errors(h, r):
+    q = max(abs(a / b - 1) for a, b in zip(h["Q"], r["Q"]))
+    phi_rel = max(rel(h["phi"][m], r["phi"][m]) for m in r["phi"])
+    phi_abs = max(float(np.max(np.abs(h["phi"][m] - r["phi"][m]))) for m in r["phi"])
+    stress = rel(h["stress"], r["stress"])
+    return q, phi_rel, phi_abs, stress
+
+
+@pytest.mark.parametrize("name", list(CONFIGS))
+def test_fp64_matches_oracle_at_config_shape(name):
+    key, n, L, _ = CONFIGS[name]
+    say(f"config {name}: {key} {n}^3 L={L}")
+    r = oracle_result(name)
+    h = hip_result(name, "hip")
+    q, phi_rel, _, stress = errors(h, r)
+    say(f"  Q rel {q:.1e}  phi rel {phi_rel:.1e}  stress rel {stress:.1e}  Q(n) constant {h['partition_constant']}")
+    assert q < 1e-10 and phi_rel < 1e-10 and stress < 1e-10
+    assert h["partition_constant"]
+
+
+def free_energy(res, w, chin):
+    """Mean-field free energy per chain of the AB melt: -ln Q + <chiN phi_A phi_B - w_A phi_A - w_B phi_B>."""
+    a, b = res["phi"]["A"], res["phi"]["B"]
+    return -np.log(res["Q"][0]) + np.mean(chin * a * b - w["A"] * a - w["B"] * b)
+
+
+@pytest.mark.parametrize("name", ["C3-96", "C3-128", "C3-discrete-96", "C3-discrete-128"])
+def test_fp32_form_against_oracle(name):
+    """The precision form at its own shapes, measured against the FP64 oracle (chiN = 18)."""
+    say(f"fp32 form {name}")
+    r = oracle_result(name)
+    h = hip_result(name, "hip-fp32")
+    q, phi_rel, phi_abs, stress = errors(h, r)
+    w = fields(name)
+    f_ref, f32 = free_energy(r, w, 18.0), free_energy(h, w, 18.0)
+    phi_max = max(float(np.max(v)) for v in r["phi"].values())
+    say(f"  Q rel {q:.1e}  max|dphi| {phi_abs:.1e} (phi max {phi_max:.2f})  stress rel {stress:.1e}  "
+        f"F {f_ref:.6f} |dF| {abs(f32 - f_ref):.1e}")
+    assert q < 1e-5
+    assert phi_abs < 1e-5 * phi_max
+    assert stress < 1e-5
+    assert abs(f32 - f_ref) < 1e-5 * max(1.0, abs(f_ref))

@@ -11,11 +11,12 @@ def say(msg):
     print(msg, flush=True)
 
 
-def test_learned_guess_reduces_saddle_iterations(tmp_path):
+@pytest.mark.parametrize("inputs", ["state", "delta"])
+def test_learned_guess_reduces_saddle_iterations(tmp_path, inputs):
     from langevin_fts_amd import lfts
 
     nx, lx = [32, 32, 32], [4.0, 4.0, 4.0]
-    say(f"learned guess gain {nx}")
+    say(f"learned guess gain {nx} ({inputs})")
     params = {
         "nx": nx, "lx": lx, "chain_model": "discrete", "ds": 1 / 90,
         "segment_lengths": {"A": 1.0, "B": 1.0}, "chi_n": {"A,B": 17.0},
@@ -27,7 +28,8 @@ def test_learned_guess_reduces_saddle_iterations(tmp_path):
         "saddle": {"max_iter": 200, "tolerance": 1e-4},
         "am": {"max_hist": 20, "start_error": 5e-1, "mix_min": 0.01, "mix_init": 0.01},
         "verbose_level": 0,
-        "learned_guess": {"train_steps": 150, "lr": 1e-3, "train_iters": 4, "replay": 32, "delta": True, "seed": 0},
+        "learned_guess": {"train_steps": 150, "lr": 1e-3, "train_iters": 4, "replay": 32, "inputs": inputs,
+                          "seed": 0},
     }
     sim = lfts.LFTS(params, random_seed=5)
     M = int(np.prod(nx))

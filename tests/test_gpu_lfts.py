@@ -41,14 +41,16 @@ def lfts():
     return m
 
 
-def test_saddle_and_langevin_match_oracle(lfts, tmp_path):
-    """C5's chain (discrete N=90, f=0.5, chiN=17) on a small box: every saddle point
+@pytest.mark.parametrize("nx,lx,am,tol,n_steps", [
+    ([12, 10, 8], [3.0, 2.6, 2.2], (20, 1e-1, 0.1, 0.1), 1e-6, 3),
+    # C5 itself: 64^3, L=6.0, the reference's L-FTS mixing (20, 0.5, 0.01, 0.01), tol 1e-4
+    ([64, 64, 64], [6.0, 6.0, 6.0], (20, 5e-1, 0.01, 0.01), 1e-4, 2)])
+def test_saddle_and_langevin_match_oracle(lfts, tmp_path, nx, lx, am, tol, n_steps):
+    """C5's chain (discrete N=90, f=0.5, chiN=17, nbar=1e4, dt=0.8): every saddle point
     and Langevin step of the device path against the oracle with the same noise."""
-    nx, lx = [12, 10, 8], [3.0, 2.6, 2.2]
     say(f"lfts vs oracle {nx}")
     chi_n, dt, nbar = 17.0, 0.8, 1e4
-    am = (20, 1e-1, 0.1, 0.1)
-    tol, max_iter = 1e-6, 200
+    max_iter = 200
     params = diblock_params(nx, lx, "discrete", 1 / 90, chi_n, 0.5, nbar, dt, am, tol, max_iter, 3,
                             str(tmp_path), verbose=0)
     sim = lfts.LFTS(params, random_seed=3)
@@ -66,7 +68,7 @@ def test_saddle_and_langevin_match_oracle(lfts, tmp_path):
     sigma = lfts.calculate_sigma(nbar, dt, M, np.prod(lx))
     noise_prev = np.zeros(M)
     dev.set_noise_prev(noise_prev[None, :])
-    for step in range(4):
+    for step in range(n_steps + 1):
         if step > 0:
             noise = rng.normal(0.0, sigma, [1, M])
             dev.langevin_update(noise, dt)

@@ -80,3 +80,31 @@ def test_mixed_precision_saddle_points_meet_fp64_tolerance(tmp_path):
     diff = np.max(np.abs(runs[False][1] - runs[True][1]))
     say(f"  max |w_aux(mixed) - w_aux(plain)| after 12 steps: {diff:.2e}")
     assert diff < 0.05
+
+
+def test_no_fp32_convergence_on_the_last_iteration(tmp_path):
+    """With max_iter ending where the FP32 phase first gets below the tolerance, the
+    last iteration still evaluates in FP64: a saddle point reported as converged is
+    one whose fields meet the tolerance in FP64."""
+    from langevin_fts_amd import lfts
+    from langevin_fts_amd import langevinfts as lf
+
+    nx, lx = (32, 32, 32), (4.0, 4.0, 4.0)
+    M = int(np.prod(nx))
+    init = np.random.default_rng(21).standard_normal((2, M))
+    sim = lfts.LFTS(params(tmp_path, True), random_seed=7)
+    dev = sim.saddle_device
+    w0 = np.ascontiguousarray(sim.mpt.to_aux_fields(init))
+    dev.set_noise_prev(np.zeros((1, M)))
+    dev.set_fields(w0)
+    n_full, err_full, _ = sim._find_saddle_point_device()
+    say(f"last-iteration switch: full run {n_full} iterations, error {err_full:.2e}")
+    assert err_full < 1e-4
+    for k in range(max(1, n_full - 3), n_full + 1):
+        dev.set_fields(w0)
+        sim.saddle["max_iter"] = k
+        n, err, _ = sim._find_saddle_point_device()
+        check = fp64_error(lf, nx, lx, dev.get_fields(), sim.mpt)
+        say(f"  max_iter {k}: {n} iterations, reported error {err:.2e}, independent FP64 error {check:.2e}")
+        if err < 1e-4:
+            assert check < 1e-4

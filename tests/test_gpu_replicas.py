@@ -64,3 +64,77 @@ def test_replicas_match_separate_runs(tmp_path):
         say(f"  replica {r}: iterations {[x[r] for x in rep_its]} / separate {its}")
         assert [x[r] for x in rep_its] == its
         np.testing.assert_allclose(rep_w[r], dev.get_fields(), rtol=0, atol=1e-10)
+
+
+def test_c4b_replicas_at_config_shape(tmp_path):
+    """C4b: 32 replicas of the discrete N=64, f=0.5, chiN=17 diblock at 48^3, L=4.0,
+    nbar 1e4, dt 0.8, tolerance 1e-4 with the reference's L-FTS mixing: two batched
+    Langevin steps equal 32 separate runs (replica r seeded seed + r), and replica
+    0's saddle points and Langevin step equal the CPU oracle's."""
+    from langevin_fts_amd import lfts
+    from oracle import oracle as O
+    from oracle.lfts_oracle import ABLangevinOracle
+    from tests.conftest import golden
+
+    nx, lx = [48, 48, 48], [4.0, 4.0, 4.0]
+    n_rep, seed, n_steps, tol = 32, 100, 2, 1e-4
+    say(f"C4b replicas {n_rep} x {nx}")
+    M = int(np.prod(nx))
+
+    def c4b_params():
+        p = params(nx, lx, tmp_path)
+        p["ds"] = 1 / 64
+        p["chi_n"] = {"A,B": 17.0}
+        p["langevin"] = {"max_step": 0, "dt": 0.8, "nbar": 1e4}
+        p["saddle"] = {"max_iter": 200, "tolerance": tol}
+        p["am"] = {"max_hist": 20, "start_error": 5e-1, "mix_min": 0.01, "mix_init": 0.01}
+        return p
+
+    rng = np.random.default_rng(48)
+    init = rng.standard_normal((n_rep, 2, M))
+    rep = lfts.LFTS(c4b_params(), random_seed=seed, n_replicas=n_rep)
+    rep.saddle_device.set_fields(np.ascontiguousarray([rep.mpt.to_aux_fields(init[r]) for r in range(n_rep)]))
+    rep.saddle_device.set_noise_prev(np.zeros((n_rep, 1, M)))
+    its0, errs0, _, _ = rep.saddle_device.find_saddle_point(tol, 200, 0)
+    assert all(e < tol for e in errs0)
+    rep_its, rep_fields = [its0], [rep.saddle_device.get_fields()]
+    for _ in range(n_steps):
+        its, errs, ok = rep.replica_step()
+        assert all(ok)
+        rep_its.append(its)
+        rep_fields.append(rep.saddle_device.get_fields())
+
+    for r in range(n_rep):
+        one = lfts.LFTS(c4b_params(), random_seed=seed + r)
+        dev = one.saddle_device
+        dev.set_fields(np.ascontiguousarray(one.mpt.to_aux_fields(init[r])))
+        dev.set_noise_prev(np.zeros((1, M)))
+        n0, _, _, _ = dev.find_saddle_point(tol, 200, 0)
+        its = [n0]
+        for _ in range(n_steps):
+            noise = one.random.normal(0.0, one.langevin["sigma"], [1, M])
+            dev.backup()
+            dev.langevin_update(noise, one.langevin["dt"])
+            n, err, _, _ = dev.find_saddle_point(tol, 200, 0)
+            assert err < tol
+            its.append(n)
+        assert [x[r] for x in rep_its] == its, (r, [x[r] for x in rep_its], its)
+        np.testing.assert_allclose(rep_fields[-1][r], dev.get_fields(), rtol=0, atol=1e-10)
+    say(f"  32 replicas equal separate runs; replica 0 iterations {[x[0] for x in rep_its]}")
+
+    # replica 0 against the oracle: initial saddle point, then one Langevin step with its noise
+    plan = golden("plans")["diblock_f0.5_ds1_64|discrete|1"]
+    inp = plan["input"]
+    oracle = ABLangevinOracle(O.Solver(O.Grid(nx, lx), plan, inp["ds"], inp["bonds"]),
+                              O.AndersonMixing(M, 20, 5e-1, 0.01, 0.01), 17.0, [(1.0, 1.0)], tol, 200)
+    w_ref = np.ascontiguousarray(rep.mpt.to_aux_fields(init[0]))
+    phi_ref, _, n_ref, _ = oracle.find_saddle_point(w_ref)
+    assert n_ref == rep_its[0][0]
+    np.testing.assert_allclose(w_ref, rep_fields[0][0], rtol=0, atol=1e-8)
+    g = np.random.Generator(np.random.PCG64(seed))
+    noise = g.normal(0.0, rep.langevin["sigma"], [1, M])[0]
+    oracle.langevin_step(w_ref, phi_ref, np.zeros(M), noise, 0.8)
+    _, _, n_ref, _ = oracle.find_saddle_point(w_ref)
+    say(f"  replica 0 vs oracle: iterations {rep_its[1][0]} / {n_ref}")
+    assert n_ref == rep_its[1][0]
+    np.testing.assert_allclose(w_ref, rep_fields[1][0], rtol=0, atol=1e-8)

@@ -108,3 +108,85 @@ def test_scft_iterations_match_oracle(capsys):
             w[i] -= grid.integral(w[i]) / V
     np.testing.assert_allclose(calc.w, w, rtol=0, atol=1e-9)
     np.testing.assert_allclose(calc.phi["A"], phi[0], rtol=0, atol=1e-9)
+
+
+def c2_fields(n, L, chi_n, seed):
+    from tests.test_gpu_config_shapes import cosine_fields
+
+    return cosine_fields(n, L, chi_n, seed)
+
+
+@pytest.mark.parametrize("model,plan_key", [("continuous", "diblock_f0.36_ds0.01|continuous|1"),
+                                            ("discrete", "diblock_f0.36_ds0.01|discrete|1")])
+def test_c2_scft_at_config_shape(capsys, model, plan_key):
+    """C2: AB diblock f=0.36, chiN=20, ds=0.01 (discrete: N=100), 64^3, L=3.3, the
+    8-cosine start, AM(history 20, start 1e-2, mix_min 0.01, mix_init 0.1) on [w, L]
+    with box relaxation (scft.py:617-757). The first three iterations (fields and box)
+    against the oracle with the reference's residual, stress, mixing and 10 % box
+    clip restated in numpy; then the run to 1e-8, whose final fields and box meet the
+    tolerance when the oracle evaluates them again."""
+    from langevin_fts_amd import scft
+
+    n, L, chi_n = 64, 3.3, 20.0
+    nx = [n] * 3
+    say(f"C2 scft {model} {nx}")
+    fields = c2_fields(n, L, chi_n, 7)
+    p = params(nx, [L] * 3, True, 3, f=0.36, model=model, ds=0.01, chi_n=chi_n)
+    p["optimizer"] = {"name": "am", "max_hist": 20, "start_error": 1e-2, "mix_min": 0.01, "mix_init": 0.1}
+    calc = scft.SCFT(p)
+    calc.run(initial_fields=fields)
+    capsys.readouterr()
+    w_dev, lx_dev = np.array(calc.w), np.array(calc.cb.get_lx())
+
+    plan = golden("plans")[plan_key]
+    inp = plan["input"]
+    M = n ** 3
+    S = 2
+    am = O.AndersonMixing(S * M + 3, 20, 1e-2, 0.01, 0.1)
+    chi = np.array([[0.0, chi_n], [chi_n, 0.0]])
+    P = np.identity(2) - np.ones((2, 2)) @ np.linalg.inv(chi) / np.sum(np.linalg.inv(chi))
+
+    def evaluate(w, lx):
+        grid = O.Grid(nx, lx)
+        solver = O.Solver(grid, plan, inp["ds"], inp["bonds"])
+        solver.compute_statistics({"A": w[0], "B": w[1]})
+        solver.compute_stress()
+        phi = np.stack([solver.get_total_concentration("A"), solver.get_total_concentration("B")])
+        wd = chi @ phi - P @ w
+        for i in range(S):
+            wd[i] -= grid.integral(wd[i]) / grid.volume
+        err = np.sqrt(sum(grid.inner_product(wd[i], wd[i]) for i in range(S)) /
+                      (1.0 + sum(grid.inner_product(w[i], w[i]) for i in range(S))))
+        stress = np.array(solver.get_stress())
+        return grid, wd, err + np.sqrt(np.sum(stress ** 2)), stress
+
+    w = np.stack([fields["A"].ravel(), fields["B"].ravel()])
+    lx = np.array([L] * 3)
+    grid = O.Grid(nx, lx)
+    for i in range(S):
+        w[i] -= grid.integral(w[i]) / grid.volume
+    err = 1e20
+    for it in range(3):
+        grid, wd, e, stress = evaluate(w, lx)
+        old, err = err, e
+        new = am.calculate_new_fields(np.concatenate([w.ravel(), lx]), np.concatenate([wd.ravel(), -stress]), old, err)
+        w = np.reshape(new[:S * M], (S, M))
+        lx = (1 + np.clip((new[-3:] - lx) / lx, -0.1, 0.1)) * lx
+        grid = O.Grid(nx, lx)
+        for i in range(S):
+            w[i] -= grid.integral(w[i]) / grid.volume
+    say(f"  3 iterations: |dw| {np.max(np.abs(w_dev - w)):.1e}, box {lx_dev} / oracle {lx}")
+    np.testing.assert_allclose(w_dev, w, rtol=0, atol=1e-9)
+    np.testing.assert_allclose(lx_dev, lx, rtol=0, atol=1e-12)
+
+    p["max_iter"] = 3000
+    calc = scft.SCFT(p)
+    calc.run(initial_fields=fields)
+    out = capsys.readouterr().out
+    n_iter = sum(1 for line in out.split("\n") if re.match(r"\s+\d+\s+[-+]?\d", line))
+    w_fin, lx_fin = np.array(calc.w), np.array(calc.cb.get_lx())
+    _, _, e_fin, stress_fin = evaluate(w_fin, lx_fin)
+    say(f"  converged in {n_iter} iterations, box {lx_fin}, oracle error of the result {e_fin:.2e}, "
+        f"stress {stress_fin}")
+    assert n_iter < 3000
+    assert e_fin < 1e-8
