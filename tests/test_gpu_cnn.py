@@ -83,6 +83,8 @@ def test_gradients_match_oracle(lf, nx, n_in):
     net.set_params(p)
     loss, g = net.gradients(x.ravel(), t.ravel())
     loss_ref, g_ref = K.loss_and_gradients(p, x, t)
+    # how far the bf16 rounding points alone move each block: the exact network's gradients
+    _, g_exact = K.loss_and_gradients(p, x, t, rounding=False)
     assert abs(loss / loss_ref - 1) < 2e-3
     w0o, b0o, who, wlo, blo, n = K.layout(n_in)
     blocks = {"W0": (w0o, b0o), "b0": (b0o, who), "WL": (wlo, blo), "bL": (blo, n)}
@@ -92,8 +94,13 @@ def test_gradients_match_oracle(lf, nx, n_in):
         blocks[f"b{l + 1}"] = (b + K.TAPS * K.C * K.C, b + K.HSZ)
     for name, (lo, hi) in blocks.items():
         err = rel_l2(g[lo:hi], g_ref[lo:hi])
-        say(f"  {name}: relative L2 {err:.2e}")
-        assert err < 1e-2, name
+        rounding = rel_l2(g_ref[lo:hi], g_exact[lo:hi])
+        say(f"  {name}: relative L2 {err:.2e} (bf16 rounding moves it by {rounding:.2e})")
+        # the device rounds at the same points but sums in FP32 in another order, so a
+        # value near a bf16 tie can round the other way: bounded by a fraction of the
+        # rounding's own effect where that exceeds 1e-2 (the first layer of a
+        # three-input network: 5.6e-2)
+        assert err < max(1e-2, 0.3 * rounding), name
 
 
 def test_adam_step_matches_oracle(lf):
