@@ -334,11 +334,17 @@ int fts_saddle_set_low_precision(fts_saddle* s, fts_solver* lo, double switch_er
  *   train:   push the current (w_aux[real], w_aux[imag]) pair into a ring of `replay` samples,
  *            then `iters` Adam steps, the first on the newest sample and the rest on samples
  *            drawn from the ring; loss is the newest sample's loss before its step.
- * With a three-input network g (mode 2, "state"): the inputs are the new w-, its change
- * in the Langevin update (w- minus the w- of the last fts_saddle_backup) and the w+ of the
- * last backup, and g predicts the change of w+: predict sets w_aux[imag] = w+ of the backup
- * + g(...); train pushes (inputs, converged w+ minus the w+ of the backup) of the current
- * step. Both need a backup made before the Langevin update of this step. */
+ * Residual guess, a three-input network g:
+ *   set_residual_guess: arms the next fts_saddle_find (one search only): with record != 0
+ *            it keeps the residual of its first evaluation (the pressure-field derivative at
+ *            the starting w+); with c set, that first iteration's mixing is replaced by
+ *            w_aux[imag] += g(w_aux[real], first residual, w_aux[imag]) and the Anderson
+ *            mixing starts from there. The first evaluation counts as an iteration.
+ *   train:   (a three-input network) pushes ([w_aux[real], first residual, w_aux[imag] of the
+ *            last fts_saddle_backup], converged w_aux[imag] minus that backup's) into the ring;
+ *   get_guess_sample: that sample (4 n_grid doubles, host) without training. */
+int fts_saddle_set_residual_guess(fts_saddle* s, fts_cnn* c, int record);
+int fts_saddle_get_guess_sample(fts_saddle* s, double* out);
 int fts_saddle_prepare_guess(fts_saddle* s, fts_cnn* c);
 int fts_saddle_predict_guess(fts_saddle* s, fts_cnn* c, int mode);
 int fts_saddle_train_guess(fts_saddle* s, fts_cnn* c, double lr, int iters, int replay, double* loss);

@@ -814,8 +814,23 @@ int fts_saddle_predict_guess(fts_saddle* s, fts_cnn* c, int mode) {
     return guarded([&] {
         need(s, "saddle");
         need(c, "cnn");
-        FTS_REQUIRE(mode >= 0 && mode <= 2, "guess mode must be 0 (absolute), 1 (delta) or 2 (state)");
+        FTS_REQUIRE(mode == 0 || mode == 1, "guess mode must be 0 (absolute) or 1 (delta)");
         s->s->predict_guess(*c->c, mode);
+    });
+}
+
+int fts_saddle_set_residual_guess(fts_saddle* s, fts_cnn* c, int record) {
+    return guarded([&] {
+        need(s, "saddle");
+        s->s->set_residual_guess(c ? c->c.get() : nullptr, record != 0);
+    });
+}
+
+int fts_saddle_get_guess_sample(fts_saddle* s, double* out) {
+    return guarded([&] {
+        need(s, "saddle");
+        need(out, "out");
+        s->s->guess_sample(out);
     });
 }
 

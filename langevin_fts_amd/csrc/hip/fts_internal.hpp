@@ -246,9 +246,14 @@ public:
     }
     // learned guess of the imaginary field from the real one (single replica)
     void prepare_guess(Cnn& cnn);               // f(w_real) before the Langevin update (delta mode)
-    // mode 0: w_imag = f(w_real); 1: w_imag += f(w_real) - prepared; 2 (three-input network):
-    // w_imag = w_imag of the backup + g(w_real, w_real - backup's, backup's w_imag)
+    // mode 0: w_imag = f(w_real); 1: w_imag += f(w_real) - prepared
     void predict_guess(Cnn& cnn, int mode);
+    // residual guess (three-input network g), armed for the next search only: it keeps the
+    // residual of its first evaluation (record), and with cnn set replaces the first
+    // iteration's mixing by w_imag += g(w_real, first residual, w_imag)
+    void set_residual_guess(Cnn* cnn, bool record);
+    // the current training sample of the residual guess (4 M doubles: inputs, target), host
+    void guess_sample(double* out);
     double train_guess(Cnn& cnn, double lr, int iters, int replay);
 
 private:
@@ -277,9 +282,13 @@ private:
     DeviceBuffer w_aux_, phi_, w_solver_, h_, lam_, am_w_, am_d_, noise_prev_, noise_cur_, w_backup_, phi_backup_,
         partials_, stats_;
     // learned guess: replay ring of (w_real, w_imag) pairs and the prepared prediction
-    // the three inputs of the state guess (and its target) from the fields and the backup
-    void state_inputs(double* in, double* target);
-    DeviceBuffer replay_, guess_old_, guess_new_, guess_in_;
+    // the three inputs of the residual guess (and its target): the w+ the first residual
+    // was evaluated at is the backup's (from_backup) or the current one
+    void residual_inputs(double* in, double* target, bool from_backup);
+    void apply_residual_guess(Cnn& cnn);
+    DeviceBuffer replay_, guess_old_, guess_new_, guess_in_, res_first_;
+    Cnn* res_cnn_ = nullptr;
+    bool res_record_ = false, has_first_res_ = false;
     bool has_backup_ = false;
     int replay_per_ = 0;   // fields per ring sample: the network's inputs and its target
     // which network (and parameter version) and which real field a guess buffer holds f of:

@@ -138,21 +138,20 @@ __global__ void add_difference_kernel(double* x, const double* a, const double* 
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < M; i += gridDim.x * blockDim.x) x[i] += a[i] - b[i];
 }
 
-// inputs of the state guess: [w-, w- - w-_backup, w+_backup] (3 M values), and the
-// target w+ - w+_backup when target is given
-__global__ void state_inputs_kernel(const double* w_real, const double* b_real, const double* w_imag,
-                                    const double* b_imag, double* in, double* target, int M) {
+// inputs of the residual guess: [w-, first residual, w+ it was evaluated at] (3 M
+// values), and the target w+ - that w+ when target is given
+__global__ void residual_inputs_kernel(const double* w_real, const double* res, const double* w_imag,
+                                       const double* w_start, double* in, double* target, int M) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < M; i += gridDim.x * blockDim.x) {
         in[i] = w_real[i];
-        in[M + i] = w_real[i] - b_real[i];
-        in[2 * (size_t)M + i] = b_imag[i];
-        if (target) target[i] = w_imag[i] - b_imag[i];
+        in[M + i] = res[i];
+        in[2 * (size_t)M + i] = w_start[i];
+        if (target) target[i] = w_imag[i] - w_start[i];
     }
 }
 
-// w+ = w+_backup + predicted change
-__global__ void add_kernel(double* out, const double* a, const double* b, int M) {
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < M; i += gridDim.x * blockDim.x) out[i] = a[i] + b[i];
+__global__ void add_in_place_kernel(double* x, const double* a, int M) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < M; i += gridDim.x * blockDim.x) x[i] += a[i];
 }
 
 }  // namespace
@@ -436,6 +435,18 @@ void SaddlePoint::find(double tolerance, int max_iter, int verbose, double* log,
                 continue;
             }
             active++;
+            if (it == 1 && NR_ == 1 && (res_record_ || res_cnn_)) {
+                // the residual of the first evaluation: kept for training, and with a
+                // trained network its correction replaces this iteration's mixing
+                const double* hr0 = h_.as<double>();
+                if (res_first_.bytes() == 0) res_first_.allocate(sizeof(double) * M_);
+                FTS_HIP_CHECK(hipMemcpyAsync(res_first_.get(), hr0, sizeof(double) * M_, hipMemcpyDeviceToDevice, st));
+                has_first_res_ = true;
+                if (res_cnn_) {
+                    apply_residual_guess(*res_cnn_);
+                    continue;
+                }
+            }
             // Anderson mixing of the imaginary fields with d = -dt_scaling * dH/dw (lfts.py:854-860)
             double* am_w = am_w_.as<double>() + (size_t)r * I_ * M_;
             double* am_d = am_d_.as<double>() + (size_t)r * I_ * M_;
@@ -476,6 +487,9 @@ void SaddlePoint::find(double tolerance, int max_iter, int verbose, double* log,
             low = !below;
         }
     }
+    // the residual guess is armed for one search at a time
+    res_cnn_ = nullptr;
+    res_record_ = false;
     // zero mean of the pressure field (lfts.py:865)
     std::vector<const double*> last_rows;
     for (int r = 0; r < NR_; r++) last_rows.push_back(wa(r) + (size_t)(S_ - 1) * M_);
@@ -542,14 +556,45 @@ void SaddlePoint::prepare_guess(Cnn& cnn) {
     prepared_ = true;
 }
 
-void SaddlePoint::state_inputs(double* in, double* target) {
-    FTS_REQUIRE(has_backup_, "the state guess needs a backup of the previous step's fields");
+void SaddlePoint::residual_inputs(double* in, double* target, bool from_backup) {
+    FTS_REQUIRE(has_first_res_, "the residual guess needs the first residual of a saddle-point search");
     const double* w = w_aux_.as<const double>();
-    const double* b = w_backup_.as<const double>();
     const size_t re = (size_t)real_idx_[0] * M_, im = (size_t)imag_idx_[0] * M_;
-    hipLaunchKernelGGL(state_inputs_kernel, dim3(grid_blocks(M_)), dim3(256), 0, ctx_->stream, w + re, b + re, w + im,
-                       b + im, in, target, M_);
+    // the w+ the first residual was evaluated at: the current one during the search,
+    // the backup's (the step's starting point) once the search has moved it
+    FTS_REQUIRE(!from_backup || has_backup_, "training the residual guess needs the step's backup");
+    const double* start = from_backup ? w_backup_.as<const double>() + im : w + im;
+    hipLaunchKernelGGL(residual_inputs_kernel, dim3(grid_blocks(M_)), dim3(256), 0, ctx_->stream, w + re,
+                       res_first_.as<const double>(), w + im, start, in, target, M_);
     FTS_HIP_CHECK(hipGetLastError());
+}
+
+void SaddlePoint::set_residual_guess(Cnn* cnn, bool record) {
+    FTS_REQUIRE(NR_ == 1, "the learned guess runs on a single replica");
+    FTS_REQUIRE(R_ == 1 && I_ == 1, "the learned guess maps one real field to one imaginary field");
+    FTS_REQUIRE(!cnn || (cnn->n_in() == 3 && cnn->n_grid() == M_),
+                "the residual guess needs a three-input network on the simulation grid");
+    res_cnn_ = cnn;
+    res_record_ = record;
+}
+
+void SaddlePoint::apply_residual_guess(Cnn& cnn) {
+    if (guess_in_.bytes() == 0) guess_in_.allocate(sizeof(double) * 3 * M_);
+    if (guess_new_.bytes() == 0) guess_new_.allocate(sizeof(double) * M_);
+    residual_inputs(guess_in_.as<double>(), nullptr, false);
+    cnn.predict(guess_in_.as<const double>(), guess_new_.as<double>(), true);
+    hipLaunchKernelGGL(add_in_place_kernel, dim3(grid_blocks(M_)), dim3(256), 0, ctx_->stream,
+                       w_aux_.as<double>() + (size_t)imag_idx_[0] * M_, guess_new_.as<const double>(), M_);
+    FTS_HIP_CHECK(hipGetLastError());
+}
+
+void SaddlePoint::guess_sample(double* out) {
+    FTS_REQUIRE(NR_ == 1 && R_ == 1 && I_ == 1, "the residual guess runs on one real and one imaginary field");
+    DeviceBuffer tmp(sizeof(double) * 4 * (size_t)M_);
+    residual_inputs(tmp.as<double>(), tmp.as<double>() + 3 * (size_t)M_, true);
+    FTS_HIP_CHECK(hipMemcpyAsync(out, tmp.get(), sizeof(double) * 4 * (size_t)M_, hipMemcpyDeviceToHost,
+                                 ctx_->stream));
+    FTS_HIP_CHECK(hipStreamSynchronize(ctx_->stream));
 }
 
 void SaddlePoint::predict_guess(Cnn& cnn, int mode) {
@@ -558,18 +603,6 @@ void SaddlePoint::predict_guess(Cnn& cnn, int mode) {
     FTS_REQUIRE(cnn.n_grid() == M_, "the network's grid differs from the simulation grid");
     double* w_real = w_aux_.as<double>() + (size_t)real_idx_[0] * M_;
     double* w_imag = w_aux_.as<double>() + (size_t)imag_idx_[0] * M_;
-    if (mode == 2) {
-        FTS_REQUIRE(cnn.n_in() == 3, "the state guess needs a three-input network");
-        if (guess_in_.bytes() == 0) guess_in_.allocate(sizeof(double) * 3 * M_);
-        if (guess_new_.bytes() == 0) guess_new_.allocate(sizeof(double) * M_);
-        state_inputs(guess_in_.as<double>(), nullptr);
-        cnn.predict(guess_in_.as<const double>(), guess_new_.as<double>(), true);
-        hipLaunchKernelGGL(add_kernel, dim3(grid_blocks(M_)), dim3(256), 0, ctx_->stream, w_imag,
-                           w_backup_.as<const double>() + (size_t)imag_idx_[0] * M_, guess_new_.as<const double>(),
-                           M_);
-        FTS_HIP_CHECK(hipGetLastError());
-        return;
-    }
     FTS_REQUIRE(cnn.n_in() == 1, "the absolute and delta guesses need a one-input network");
     if (mode == 0) {
         cnn.predict(w_real, w_imag, true);
@@ -603,8 +636,8 @@ double SaddlePoint::train_guess(Cnn& cnn, double lr, int iters, int replay) {
     // newest sample into the ring
     double* slot = replay_.as<double>() + (size_t)replay_head_ * per * M_;
     if (per == 4) {
-        // inputs [w-, w- - w-_backup, w+_backup], target w+ - w+_backup
-        state_inputs(slot, slot + 3 * (size_t)M_);
+        // inputs [w-, first residual, w+ of the step's start], target: the converged w+ minus that start
+        residual_inputs(slot, slot + 3 * (size_t)M_, true);
     } else {
         FTS_HIP_CHECK(hipMemcpyAsync(slot, w_aux_.as<double>() + (size_t)real_idx_[0] * M_, sizeof(double) * M_,
                                      hipMemcpyDeviceToDevice, st));

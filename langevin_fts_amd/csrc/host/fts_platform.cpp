@@ -729,6 +729,14 @@ void DeviceSaddlePoint::prepare_guess(LearnedGuess& g) { check_fts(fts_saddle_pr
 void DeviceSaddlePoint::predict_guess(LearnedGuess& g, int mode) {
     check_fts(fts_saddle_predict_guess(saddle_, g.handle(), mode));
 }
+void DeviceSaddlePoint::set_residual_guess(LearnedGuess* g, bool record) {
+    check_fts(fts_saddle_set_residual_guess(saddle_, g ? g->handle() : nullptr, record ? 1 : 0));
+}
+std::vector<double> DeviceSaddlePoint::guess_sample() {
+    std::vector<double> out(4 * (size_t)M_);
+    check_fts(fts_saddle_get_guess_sample(saddle_, out.data()));
+    return out;
+}
 double DeviceSaddlePoint::train_guess(LearnedGuess& g, double lr, int iters, int replay) {
     double loss = 0.0;
     check_fts(fts_saddle_train_guess(saddle_, g.handle(), lr, iters, replay, &loss));

@@ -209,8 +209,13 @@ public:
     void langevin_update(const double* noise, double dt);
     void field_moments(double* mean, double* mean_sq);
     void prepare_guess(LearnedGuess& g);
-    // mode 0: absolute, 1: delta (one-input networks), 2: state (three-input network)
+    // mode 0: absolute, 1: delta (one-input networks)
     void predict_guess(LearnedGuess& g, int mode);
+    // residual guess: record the first residual of each search; with g (three inputs),
+    // apply its correction at the first iteration (g = null: record only or off)
+    void set_residual_guess(LearnedGuess* g, bool record);
+    // the residual guess's current sample: [w-, first residual, starting w+, target], host
+    std::vector<double> guess_sample();
     // iterations on `lo` (an FP32 solver of the same molecules) while the error is at or above switch_error
     void set_low_precision_solver(PropagatorComputation* lo, double switch_error);
     double train_guess(LearnedGuess& g, double lr, int iters, int replay);

@@ -375,6 +375,17 @@ PYBIND11_MODULE(langevinfts, m) {
              py::arg("switch_error"), py::keep_alive<1, 2>())
         .def("prepare_guess", &DeviceSaddlePoint::prepare_guess)
         .def("predict_guess", &DeviceSaddlePoint::predict_guess, py::arg("guess"), py::arg("mode") = 0)
+        .def(
+            "set_residual_guess",
+            [](DeviceSaddlePoint& o, LearnedGuess* g, bool record) { o.set_residual_guess(g, record); },
+            py::arg("guess"), py::arg("record") = true)
+        .def("guess_sample",
+             [](DeviceSaddlePoint& o) {
+                 const std::vector<double> v = o.guess_sample();
+                 py::array_t<double> out({(py::ssize_t)4, (py::ssize_t)o.n_grid()});
+                 std::copy(v.begin(), v.end(), out.mutable_data());
+                 return out;
+             })
         .def("train_guess", &DeviceSaddlePoint::train_guess, py::arg("guess"), py::arg("lr"), py::arg("iters") = 1,
              py::arg("replay") = 1);
 

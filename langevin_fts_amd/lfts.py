@@ -216,7 +216,7 @@ def guess_initial_params(seed, n_in=1):
     return np.concatenate(parts).astype(np.float32)
 
 
-GUESS_MODES = {"absolute": 0, "delta": 1, "state": 2}
+GUESS_MODES = {"absolute": 0, "delta": 1, "residual": 2}
 
 
 class NoiseStream:
@@ -261,9 +261,11 @@ class LFTS:
     An optional "learned_guess" entry {"train_steps", "lr", "lr_final", "train_iters",
     "replay", "inputs", "seed"} trains a periodic 3D CNN on the saddle points of the
     first train_steps accepted Langevin steps and then starts every saddle point from
-    its prediction. "inputs": "state" (default: the network sees w-, its change in the
-    Langevin update and the previous w+, and predicts the change of w+), "delta" (a
-    network f(w-) ~ w+, used as w+ += f(w-_new) - f(w-_old)) or "absolute" (w+ = f(w-))."""
+    its prediction. "inputs": "delta" (default: a network f(w-) ~ w+, used as
+    w+ += f(w-_new) - f(w-_old)), "absolute" (w+ = f(w-)) or "residual" (the first
+    saddle iteration evaluates the previous w+ as plain mixing would; a network g of
+    w-, that iteration's residual and w+ predicts the correction of w+, which replaces
+    the first mixing step)."""
 
     def __init__(self, params, random_seed=None, n_replicas=1):
         """n_replicas > 1: that many independent runs of the same system advance together,
@@ -434,13 +436,11 @@ class LFTS:
             self.learned.setdefault("train_iters", 16)
             self.learned.setdefault("replay", 64)
             if "inputs" not in self.learned:
-                # the earlier boolean form: "delta": True / False
-                self.learned["inputs"] = "delta" if self.learned.get("delta", True) is True else \
-                    ("absolute" if "delta" in self.learned else "state")
-            assert self.learned["inputs"] in GUESS_MODES, "inputs must be state, delta or absolute"
+                # the boolean form: "delta": True / False
+                self.learned["inputs"] = "delta" if self.learned.get("delta", True) else "absolute"
+            assert self.learned["inputs"] in GUESS_MODES, "inputs must be delta, absolute or residual"
             self.learned.setdefault("seed", 0)
         self.guess = None
-        self._guess_prev_fields = None
         self.saddle = params["saddle"].copy()
         self.recording = params["recording"].copy()
         self.cb = cb
@@ -484,18 +484,15 @@ class LFTS:
     def init_guess(self):
         """The network and its scales, with He-normal weights from the configured seed.
         One input (delta, absolute): w- -> w+, scaled by the current fields' standard
-        deviations. State: [w-, w- - w-_prev, w+_prev] -> w+ - w+_prev, scaled by the
-        standard deviations of the first training sample (the current converged step
-        and the previous step's fields, kept by the driver for this)."""
-        w_aux = self.saddle_device.get_fields()
-        ri, ii = self.mpt.aux_fields_real_idx[0], self.mpt.aux_fields_imag_idx[0]
-        if self.learned["inputs"] == "state":
-            prev = self._guess_prev_fields
-            assert prev is not None, "the state guess trains on a step whose previous fields were kept"
-            in_scales = [float(np.std(w_aux[ri])) or 1.0, float(np.std(w_aux[ri] - prev[ri])) or 1.0,
-                         float(np.std(prev[ii])) or 1.0]
-            out_scale = float(np.std(w_aux[ii] - prev[ii])) or 1.0
+        deviations. Residual: [w-, first residual, starting w+] -> converged w+ minus the
+        starting w+, scaled by the standard deviations of the first training sample."""
+        if self.learned["inputs"] == "residual":
+            sample = self.saddle_device.guess_sample()
+            in_scales = [float(np.std(sample[c])) or 1.0 for c in range(3)]
+            out_scale = float(np.std(sample[3])) or 1.0
         else:
+            w_aux = self.saddle_device.get_fields()
+            ri, ii = self.mpt.aux_fields_real_idx[0], self.mpt.aux_fields_imag_idx[0]
             in_scales = [float(np.std(w_aux[ri])) or 1.0]
             out_scale = float(np.std(w_aux[ii])) or 1.0
         self.guess = lf.LearnedGuess(self.cb.get_nx(), in_scales, out_scale)
@@ -503,18 +500,17 @@ class LFTS:
         self.guess_trained_steps = 0
         return in_scales, out_scale
 
-    def _keep_fields_for_guess(self):
-        """Before the first training step's Langevin update: the fields the state
-        guess's scales are measured against."""
-        if self.learned is not None and self.guess is None and self.learned["inputs"] == "state":
-            self._guess_prev_fields = self.saddle_device.get_fields()
-
     def _guess_before_update(self, ready):
-        if ready and self.learned["inputs"] == "delta":
+        if self.learned is None:
+            return
+        if self.learned["inputs"] == "delta" and ready:
             self.saddle_device.prepare_guess(self.guess)
+        if self.learned["inputs"] == "residual":
+            # record the first residual while training; apply the network once it is ready
+            self.saddle_device.set_residual_guess(self.guess if ready else None, not ready)
 
     def _guess_after_update(self, ready):
-        if ready:
+        if ready and self.learned["inputs"] != "residual":
             self.saddle_device.predict_guess(self.guess, GUESS_MODES[self.learned["inputs"]])
 
     def train_guess(self):
@@ -572,7 +568,6 @@ class LFTS:
         dev = self.saddle_device
         dev.backup()
         ready = self.guess_ready()
-        self._keep_fields_for_guess()
         self._guess_before_update(ready)
         dev.langevin_update(noise, self.langevin["dt"])
         self._guess_after_update(ready)
@@ -715,7 +710,6 @@ class LFTS:
             dev.backup()
             noise = self.next_noise()
             ready = self.guess_ready()
-            self._keep_fields_for_guess()
             self._guess_before_update(ready)
             dev.langevin_update(noise, self.langevin["dt"])
             self._guess_after_update(ready)

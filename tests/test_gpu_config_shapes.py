@@ -13,7 +13,9 @@ launch shape):
 FP64 bar (CpuComputationContinuous.cpp:148-726; tolerances as the reference's
 TestPseudoLinearContinuous3D.cpp:247-285 but tightened to the FP64 restatement):
 Q, phi and stress within 1e-10 relative, and Q(n) constant along every block
-within 1e-7 (check_total_partition, CpuComputationContinuous.cpp:755-817).
+(check_total_partition, CpuComputationContinuous.cpp:755-817) to 1e-10 relative of
+Q: the reference's absolute 1e-7 cannot hold for the 8-cosine fields of C2, whose
+Q is about 1.7e13 (FP64 rounding alone spreads it by about 1).
 The precision form ("hip-fp32": propagators and FFTs in FP32, Q/phi/stress in
 FP64) is measured against the same oracle result at the C3 shapes.
 """
@@ -96,7 +98,22 @@ def oracle_result(name):
     return _oracle[name]
 
 
-def hip_result(name, platform):
+def partition_spread(capfd, solver):
+    """largest (max - min) / max of Q(n) over the polymers, from check_total_partition's
+    printed rows "p: max, min, difference" """
+    capfd.readouterr()
+    solver.check_total_partition()
+    out = capfd.readouterr().out
+    spread = 0.0
+    for line in out.splitlines():
+        parts = line.strip().split(":")
+        if len(parts) == 2 and parts[0].strip().isdigit():
+            mx, mn, diff = (float(v) for v in parts[1].split(","))
+            spread = max(spread, diff / abs(mx))
+    return spread
+
+
+def hip_result(name, platform, capfd=None):
     """The same evaluation through the reference's Python surface on a "hip" platform."""
     from langevin_fts_amd import langevinfts as lf
 
@@ -112,7 +129,7 @@ def hip_result(name, platform):
     solver.compute_statistics(fields(name))
     out = {"Q": [solver.get_total_partition(p) for p in range(len(inp["polymers"]))],
            "phi": {m: solver.get_total_concentration(m) for m in inp["bonds"]}}
-    out["partition_constant"] = solver.check_total_partition() if platform == "hip" else None
+    out["partition_spread"] = partition_spread(capfd, solver) if capfd is not None else None
     solver.compute_stress()
     out["stress"] = np.array(solver.get_stress())
     return out
@@ -127,15 +144,18 @@ def errors(h, r):
 
 
 @pytest.mark.parametrize("name", list(CONFIGS))
-def test_fp64_matches_oracle_at_config_shape(name):
+def test_fp64_matches_oracle_at_config_shape(name, capfd):
     key, n, L, _ = CONFIGS[name]
-    say(f"config {name}: {key} {n}^3 L={L}")
+    with capfd.disabled():
+        say(f"config {name}: {key} {n}^3 L={L}")
     r = oracle_result(name)
-    h = hip_result(name, "hip")
+    h = hip_result(name, "hip", capfd)
     q, phi_rel, _, stress = errors(h, r)
-    say(f"  Q rel {q:.1e}  phi rel {phi_rel:.1e}  stress rel {stress:.1e}  Q(n) constant {h['partition_constant']}")
+    with capfd.disabled():
+        say(f"  Q rel {q:.1e}  phi rel {phi_rel:.1e}  stress rel {stress:.1e}  "
+            f"Q(n) spread {h['partition_spread']:.1e} (Q = {h['Q'][0]:.4e})")
     assert q < 1e-10 and phi_rel < 1e-10 and stress < 1e-10
-    assert h["partition_constant"]
+    assert h["partition_spread"] < 1e-10
 
 
 def free_energy(res, w, chin):

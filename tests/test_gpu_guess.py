@@ -11,7 +11,7 @@ def say(msg):
     print(msg, flush=True)
 
 
-@pytest.mark.parametrize("inputs", ["state", "delta"])
+@pytest.mark.parametrize("inputs", ["residual", "delta"])
 def test_learned_guess_reduces_saddle_iterations(tmp_path, inputs):
     from langevin_fts_amd import lfts
 
