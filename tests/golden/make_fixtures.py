@@ -266,6 +266,8 @@ def molecule_set():
                                    "polymers": [[1.0, [["A", 0.4, 0, 1], ["B", 0.6, 1, 2]], {}]]}
     mols["diblock_f0.4_ds1_90"] = {"ds": 1.0 / 90, "bonds": {"A": 1.0, "B": 1.0},
                                    "polymers": [[1.0, [["A", 0.4, 0, 1], ["B", 0.6, 1, 2]], {}]]}
+    mols["diblock_f0.5_ds1_16"] = {"ds": 1.0 / 16, "bonds": {"A": 1.0, "B": 1.0},
+                                   "polymers": [[1.0, [["A", 0.5, 0, 1], ["B", 0.5, 1, 2]], {}]]}
     mols["diblock_f0.5_ds1_64"] = {"ds": 1.0 / 64, "bonds": {"A": 1.0, "B": 1.0},
                                    "polymers": [[1.0, [["A", 0.5, 0, 1], ["B", 0.5, 1, 2]], {}]]}
     mols["star_abc_homo_a"] = {"ds": 0.01, "bonds": {"A": 1.0, "B": 1.0, "C": 1.2},
