@@ -783,9 +783,9 @@ def main():
     ap.add_argument("--guess-lr", type=float, default=1e-3, help="initial Adam learning rate of the guess")
     ap.add_argument("--guess-lr-final", type=float, default=1e-4, help="learning rate at the end of training")
     ap.add_argument("--guess-replay", type=int, default=64, help="stored training samples replayed in training")
-    ap.add_argument("--guess-inputs", default="state", choices=["state", "delta", "absolute"],
-                    help="state: the CNN sees w-, its Langevin change and the previous w+ and predicts the change "
-                         "of w+; delta: w+ += f(w-_new) - f(w-_old)")
+    ap.add_argument("--guess-inputs", default="residual", choices=["residual", "delta", "absolute"],
+                    help="residual: the CNN sees w-, the first residual of the search and the starting w+ and "
+                         "predicts the correction of w+; delta: w+ += f(w-_new) - f(w-_old)")
     ap.add_argument("--sk-steps", type=int, default=20,
                     help="steps of the untimed structure-function agreement pass (0: skip)")
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"],

@@ -97,11 +97,12 @@ def test_c4b_replicas_at_config_shape(tmp_path):
     rep.saddle_device.set_noise_prev(np.zeros((n_rep, 1, M)))
     its0, errs0, _, _ = rep.saddle_device.find_saddle_point(tol, 200, 0)
     assert all(e < tol for e in errs0)
-    rep_its, rep_fields = [its0], [rep.saddle_device.get_fields()]
+    rep_its, rep_errs, rep_fields = [its0], [errs0], [rep.saddle_device.get_fields()]
     for _ in range(n_steps):
         its, errs, ok = rep.replica_step()
         assert all(ok)
         rep_its.append(its)
+        rep_errs.append(errs)
         rep_fields.append(rep.saddle_device.get_fields())
 
     for r in range(n_rep):
@@ -127,14 +128,18 @@ def test_c4b_replicas_at_config_shape(tmp_path):
     inp = plan["input"]
     oracle = ABLangevinOracle(O.Solver(O.Grid(nx, lx), plan, inp["ds"], inp["bonds"]),
                               O.AndersonMixing(M, 20, 5e-1, 0.01, 0.01), 17.0, [(1.0, 1.0)], tol, 200)
+    # the saddle points stop at a 1e-4 error: rounding differences of the evaluations
+    # (1e-14) grow through the Anderson steps and the Langevin step's phi, so the
+    # fields agree to about 1e-7 and the iteration counts and error levels exactly
     w_ref = np.ascontiguousarray(rep.mpt.to_aux_fields(init[0]))
-    phi_ref, _, n_ref, _ = oracle.find_saddle_point(w_ref)
-    assert n_ref == rep_its[0][0]
+    phi_ref, _, n_ref, e_ref = oracle.find_saddle_point(w_ref)
+    assert n_ref == rep_its[0][0] and abs(e_ref / errs0[0] - 1) < 1e-6
     np.testing.assert_allclose(w_ref, rep_fields[0][0], rtol=0, atol=1e-8)
     g = np.random.Generator(np.random.PCG64(seed))
     noise = g.normal(0.0, rep.langevin["sigma"], [1, M])[0]
     oracle.langevin_step(w_ref, phi_ref, np.zeros(M), noise, 0.8)
-    _, _, n_ref, _ = oracle.find_saddle_point(w_ref)
-    say(f"  replica 0 vs oracle: iterations {rep_its[1][0]} / {n_ref}")
-    assert n_ref == rep_its[1][0]
-    np.testing.assert_allclose(w_ref, rep_fields[1][0], rtol=0, atol=1e-8)
+    _, _, n_ref, e_ref = oracle.find_saddle_point(w_ref)
+    say(f"  replica 0 vs oracle: iterations {rep_its[1][0]} / {n_ref}, "
+        f"max |dw| {np.max(np.abs(w_ref - rep_fields[1][0])):.1e}")
+    assert n_ref == rep_its[1][0] and abs(e_ref / rep_errs[1][0] - 1) < 1e-5
+    np.testing.assert_allclose(w_ref, rep_fields[1][0], rtol=0, atol=1e-6)

@@ -119,12 +119,14 @@ def c2_fields(n, L, chi_n, seed):
 @pytest.mark.parametrize("model,plan_key", [("continuous", "diblock_f0.36_ds0.01|continuous|1"),
                                             ("discrete", "diblock_f0.36_ds0.01|discrete|1")])
 def test_c2_scft_at_config_shape(capsys, model, plan_key):
-    """C2: AB diblock f=0.36, chiN=20, ds=0.01 (discrete: N=100), 64^3, L=3.3, the
-    8-cosine start, AM(history 20, start 1e-2, mix_min 0.01, mix_init 0.1) on [w, L]
-    with box relaxation (scft.py:617-757). The first three iterations (fields and box)
-    against the oracle with the reference's residual, stress, mixing and 10 % box
-    clip restated in numpy; then the run to 1e-8, whose final fields and box meet the
-    tolerance when the oracle evaluates them again."""
+    """C2: AB diblock f=0.36, chiN=20, ds=0.01 (discrete: N=100), 64^3, L=3.3,
+    AM(history 20, start 1e-2, mix_min 0.01, mix_init 0.1) on [w, L] with box
+    relaxation (scft.py:617-757). From the 8-cosine start, the first three iterations
+    (fields and box) against the oracle with the reference's residual, stress, mixing
+    and 10 % box clip restated in numpy. The 8-cosine start is no ordered phase and
+    does not reach 1e-8 in thousands of iterations (its box drifts); the run to 1e-8
+    starts from the double-gyroid fields of examples/scft/Gyroid.py at 64^3 instead,
+    and its final fields and box meet the tolerance when the oracle evaluates them."""
     from langevin_fts_amd import scft
 
     n, L, chi_n = 64, 3.3, 20.0
@@ -175,13 +177,15 @@ def test_c2_scft_at_config_shape(capsys, model, plan_key):
         grid = O.Grid(nx, lx)
         for i in range(S):
             w[i] -= grid.integral(w[i]) / grid.volume
-    say(f"  3 iterations: |dw| {np.max(np.abs(w_dev - w)):.1e}, box {lx_dev} / oracle {lx}")
-    np.testing.assert_allclose(w_dev, w, rtol=0, atol=1e-9)
+    say(f"  3 iterations: |dw| {np.max(np.abs(w_dev - w)):.1e} (|w| up to {np.max(np.abs(w)):.0f}), "
+        f"box {lx_dev} / oracle {lx}")
+    np.testing.assert_allclose(w_dev, w, rtol=0, atol=1e-11 * np.max(np.abs(w)))
     np.testing.assert_allclose(lx_dev, lx, rtol=0, atol=1e-12)
 
     p["max_iter"] = 3000
     calc = scft.SCFT(p)
-    calc.run(initial_fields=fields)
+    w_a, w_b = gyroid_fields(nx)
+    calc.run(initial_fields={"A": w_a, "B": w_b})
     out = capsys.readouterr().out
     n_iter = sum(1 for line in out.split("\n") if re.match(r"\s+\d+\s+[-+]?\d", line))
     w_fin, lx_fin = np.array(calc.w), np.array(calc.cb.get_lx())
