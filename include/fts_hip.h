@@ -83,6 +83,16 @@ int fts_grid_inner_product(fts_grid* g, const double* a, const double* b, int on
  * allowed. batch fields are contiguous. */
 int fts_grid_fft_forward(fts_grid* g, int precision, const void* d_in, void* d_out, int batch);
 int fts_grid_fft_backward(fts_grid* g, int precision, const void* d_in, void* d_out, int batch);
+/* No counterpart in the reference: fts_grid_fft_backward for callers that cannot
+ * allocate device memory (code under HIP-graph capture, serving loops, harnesses
+ * that forbid hipMalloc). Same arguments and results (bit for bit); it allocates
+ * nothing and does not synchronise: the work is enqueued on the context's stream.
+ * Its workspace is a fixed 32 MiB array per GPU, kept until the process ends: a grid
+ * whose half spectrum (nx ny (nz/2+1) rounded up to rows of 8, times 16 bytes in
+ * FP64, 8 in FP32) exceeds it is refused with an error (128^3 FP64 needs 18.9 MB).
+ * Every call on one GPU shares that array, so calls whose work overlaps on one GPU
+ * (other streams or contexts on the device) race; calls in one stream do not. */
+int fts_grid_fft_backward_noalloc(fts_grid* g, int precision, const void* d_in, void* d_out, int batch);
 /* FFT schedule of N^3 grids (N = 32, 48, 64, 96, 128); no counterpart in the
  * reference, whose FFTs are library calls (MklFFT3D.cpp:11-79, CudaFFT). Per
  * precision, the grid picks the slab-pass workgroup size (slab_threads) and the kz

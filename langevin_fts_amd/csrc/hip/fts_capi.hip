@@ -44,6 +44,7 @@ int fts_context_create(int device, fts_context** out) {
         FTS_HIP_CHECK(hipGetDeviceCount(&n));
         FTS_REQUIRE(device >= 0 && device < n, "device index out of range");
         FTS_HIP_CHECK(hipSetDevice(device));
+        fts::noalloc_scratch_init(device);
         auto* c = new fts_context;
         c->device = device;
         if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
@@ -218,6 +219,21 @@ int fts_grid_fft_backward(fts_grid* g, int precision, const void* d_in, void* d_
             g->g->fft_backward<double>(static_cast<const double2*>(d_in), static_cast<double*>(d_out), batch);
         else if (precision == FTS_FP32)
             g->g->fft_backward<float>(static_cast<const float2*>(d_in), static_cast<float*>(d_out), batch);
+        else
+            throw fts::Error("precision must be FTS_FP64 or FTS_FP32");
+    });
+}
+
+int fts_grid_fft_backward_noalloc(fts_grid* g, int precision, const void* d_in, void* d_out, int batch) {
+    return guarded([&] {
+        need(g, "grid");
+        need(d_in, "in");
+        need(d_out, "out");
+        FTS_REQUIRE(batch >= 1, "batch must be positive");
+        if (precision == FTS_FP64)
+            g->g->fft_backward_noalloc<double>(static_cast<const double2*>(d_in), static_cast<double*>(d_out), batch);
+        else if (precision == FTS_FP32)
+            g->g->fft_backward_noalloc<float>(static_cast<const float2*>(d_in), static_cast<float*>(d_out), batch);
         else
             throw fts::Error("precision must be FTS_FP64 or FTS_FP32");
     });

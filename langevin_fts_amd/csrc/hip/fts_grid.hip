@@ -777,31 +777,87 @@ void Grid::fft_forward(const T* in, typename vec2<T>::type* out, int batch) {
 }
 
 template <typename T>
-void Grid::fft_backward(const typename vec2<T>::type* in, T* out, int batch) {
+void Grid::fft_backward_ws(const typename vec2<T>::type* in, T* out, int batch, typename vec2<T>::type* ws,
+                           int ws_fields) {
     require_fft();
-    // the pencil pass transforms in place, so the input is first copied to out's
-    // companion workspace: use a temporary spectrum buffer
+    // the pencil pass transforms in place, so the input is first copied into the
+    // workspace (ws_fields spectra of Ms elements), in chunks of at most ws_fields
     using T2 = typename vec2<T>::type;
-    DeviceBuffer tmp(sizeof(T2) * Ms * std::min(batch, MAX_BATCH));
-    for (int b0 = 0; b0 < batch; b0 += MAX_BATCH) {
-        const int nb = std::min(MAX_BATCH, batch - b0);
+    FTS_REQUIRE(ws_fields >= 1, "the FFT workspace holds no spectrum");
+    const int chunk = std::min(ws_fields, MAX_BATCH);
+    for (int b0 = 0; b0 < batch; b0 += chunk) {
+        const int nb = std::min(chunk, batch - b0);
         // the caller's rows of nzc values into the workspace's rows of spitch
-        FTS_HIP_CHECK(hipMemcpy2DAsync(tmp.get(), sizeof(T2) * spitch, in + (size_t)b0 * Mc, sizeof(T2) * nzc,
+        FTS_HIP_CHECK(hipMemcpy2DAsync(ws, sizeof(T2) * spitch, in + (size_t)b0 * Mc, sizeof(T2) * nzc,
                                        sizeof(T2) * nzc, (size_t)tnx[0] * tnx[1] * nb, hipMemcpyDeviceToDevice,
                                        stream()));
         SlabBatch<T> sb{};
         XBatch<T> xb{};
         sb.n = xb.n = nb;
         for (int b = 0; b < nb; b++) {
-            xb.it[b] = XItem<T>{tmp.as<T2>() + (size_t)b * Ms, -2};
+            xb.it[b] = XItem<T>{ws + (size_t)b * Ms, -2};
             SlabItem<T>& it = sb.it[b];
-            it.c0 = tmp.as<T2>() + (size_t)b * Ms;
+            it.c0 = ws + (size_t)b * Ms;
             it.qout = out + (size_t)(b0 + b) * M;
         }
         launch_xpass<T>(xb, nullptr, nullptr, nullptr);
         launch_slab<T>(SLAB_INV1, sb);
     }
+}
+
+template <typename T>
+void Grid::fft_backward(const typename vec2<T>::type* in, T* out, int batch) {
+    require_fft();
+    using T2 = typename vec2<T>::type;
+    const int nw = std::min(batch, MAX_BATCH);
+    DeviceBuffer tmp(sizeof(T2) * Ms * nw);
+    fft_backward_ws<T>(in, out, batch, tmp.as<T2>(), nw);
     FTS_HIP_CHECK(hipStreamSynchronize(stream()));
+}
+
+// ------------------------------------------------------------------ no-allocation workspace
+// fts_grid_fft_backward_noalloc's workspace: a fixed array in device memory, one copy
+// per GPU, held until the process ends. Host code never passes `scratch` itself (its
+// name is a host-side stand-in there); it uses the device address that
+// hipGetSymbolAddress returns for the current device, kept per device below.
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)32 << 20;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int MAX_DEVICES = 64;
+std::mutex scratch_mu;
+void* scratch_ptr[MAX_DEVICES] = {};
+}  // namespace
+
+void noalloc_scratch_init(int device) {
+    FTS_REQUIRE(device >= 0 && device < MAX_DEVICES, "device index beyond the no-allocation workspace table");
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    if (scratch_ptr[device]) return;
+    int cur = -1;
+    FTS_HIP_CHECK(hipGetDevice(&cur));
+    FTS_REQUIRE(cur == device, "no-allocation workspace set-up on a device that is not current");
+    void* p = nullptr;
+    FTS_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    FTS_REQUIRE(p != nullptr, "hipGetSymbolAddress returned no address for the no-allocation workspace");
+    scratch_ptr[device] = p;
+}
+
+template <typename T>
+void Grid::fft_backward_noalloc(const typename vec2<T>::type* in, T* out, int batch) {
+    require_fft();
+    using T2 = typename vec2<T>::type;
+    void* ws = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(scratch_mu);
+        ws = ctx->device >= 0 && ctx->device < MAX_DEVICES ? scratch_ptr[ctx->device] : nullptr;
+    }
+    FTS_REQUIRE(ws != nullptr, "the no-allocation workspace of this device is not set up");
+    const size_t per_field = sizeof(T2) * Ms;
+    if (per_field > NOALLOC_SCRATCH_BYTES)
+        throw Error("fts_grid_fft_backward_noalloc: one spectrum of this grid takes " + std::to_string(per_field) +
+                    " bytes of workspace; the no-allocation workspace holds " +
+                    std::to_string(NOALLOC_SCRATCH_BYTES) + " bytes");
+    fft_backward_ws<T>(in, out, batch, static_cast<T2*>(ws), (int)(NOALLOC_SCRATCH_BYTES / per_field));
 }
 
 template void Grid::launch_slab<double>(int, const SlabBatch<double>&) const;
@@ -814,6 +870,8 @@ template void Grid::fft_forward<double>(const double*, double2*, int);
 template void Grid::fft_forward<float>(const float*, float2*, int);
 template void Grid::fft_backward<double>(const double2*, double*, int);
 template void Grid::fft_backward<float>(const float2*, float*, int);
+template void Grid::fft_backward_noalloc<double>(const double2*, double*, int);
+template void Grid::fft_backward_noalloc<float>(const float2*, float*, int);
 
 // ------------------------------------------------------------------ bond tables
 void BondTables::build(const Grid& g, const std::vector<double>& bsq, double ds) {

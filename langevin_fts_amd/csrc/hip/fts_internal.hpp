@@ -150,7 +150,19 @@ public:
     void fft_forward(const T* in, typename vec2<T>::type* out, int batch);
     template <typename T>
     void fft_backward(const typename vec2<T>::type* in, T* out, int batch);
+    // fft_backward through a caller's workspace of ws_fields spectra (Ms elements
+    // each): enqueued on the stream, no allocation, no synchronisation
+    template <typename T>
+    void fft_backward_ws(const typename vec2<T>::type* in, T* out, int batch, typename vec2<T>::type* ws,
+                         int ws_fields);
+    // fft_backward_ws on the per-device no-allocation workspace
+    template <typename T>
+    void fft_backward_noalloc(const typename vec2<T>::type* in, T* out, int batch);
 };
+
+// Looks up the device address of the no-allocation workspace for `device` (which must
+// be current) once per device; called when a context is created.
+void noalloc_scratch_init(int device);
 
 // Boltzmann factors of one (monomer, bond kind), separable along the three axes.
 // Row t of bx/by/bz holds exp(-b^2 ds/6 (2 pi k_d / L_d)^2) along axis d
