@@ -688,6 +688,8 @@ def eval_bench(args, w, lf):
                                             "compute_stress", "steps_per_evaluation": steps_per_eval,
                    "evaluations_per_s": rates},
     }
+    if len(set(w["nx"])) == 1 and w["nx"][0] in (32, 48, 64, 96, 128):
+        out["config"]["fft_schedule"] = fft_schedule_info(w["nx"][0])
     if kern and kern["launches"]:
         achieved = kern["bytes"] / (kern["milliseconds"] * 1e-3) / 1e9
         out["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBPS, "unit": "GB/s",

@@ -99,9 +99,12 @@ int fts_grid_fft_backward_noalloc(fts_grid* g, int precision, const void* d_in, 
  * columns per wavefront of the pencil pass (pencil_columns) by timing the candidates
  * when it is created (environment FTS_FFT_TUNE=0: the defaults); results of the
  * candidates agree to rounding (bit-identical at 32^3 and 64^3), and one process
- * keeps one choice per (N, precision). get writes the choice and a note with the measured
- * candidates (NUL-terminated, truncated to notelen); set pins a candidate. Other
- * grids report 0 and refuse set. */
+ * keeps one choice per (N, precision). slab_threads is the half step's (discrete
+ * chains); the forward transform with two spectra out and the Richardson step of
+ * continuous chains are timed alone and may take another candidate, which the note
+ * names. get writes the choice and a note with the measured candidates
+ * (NUL-terminated, truncated to notelen); set pins one candidate for every slab mode.
+ * Other grids report 0 and refuse set. */
 int fts_grid_get_fft_schedule(fts_grid* g, int precision, int* slab_threads, int* pencil_columns, char* note,
                               int notelen);
 int fts_grid_set_fft_schedule(fts_grid* g, int precision, int slab_threads, int pencil_columns);

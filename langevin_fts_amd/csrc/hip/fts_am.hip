@@ -14,20 +14,22 @@ namespace fts {
 
 namespace {
 
-constexpr int AM_MAX_HIST = 64;
-
 struct AmDotBatch {
     int n;
     const double* a;
     const double* b[MAX_BATCH];
 };
 
+// a . b over this workgroup's share of a grid of gridDim.x workgroups (first stage)
+__device__ __forceinline__ double dot_share(const double* a, const double* b, int n_var, double* ws) {
+    double acc = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n_var; i += gridDim.x * blockDim.x) acc += a[i] * b[i];
+    return block_sum(acc, ws);
+}
+
 __global__ void __launch_bounds__(RED_THREADS) am_dot_stage1(AmDotBatch bt, int n_var, double* partials) {
     __shared__ double ws[16];
-    const double* b = bt.b[blockIdx.y];
-    double acc = 0.0;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n_var; i += gridDim.x * blockDim.x) acc += bt.a[i] * b[i];
-    acc = block_sum(acc, ws);
+    const double acc = dot_share(bt.a, bt.b[blockIdx.y], n_var, ws);
     if (threadIdx.x == 0) partials[blockIdx.y * gridDim.x + blockIdx.x] = acc;
 }
 
@@ -38,8 +40,13 @@ __global__ void __launch_bounds__(RED_BLOCKS) am_dot_stage2(const double* partia
     if (threadIdx.x == 0) out[blockIdx.x] = v;
 }
 
+__device__ __forceinline__ double simple_mix_value(const double* w, const double* d, double mix, int i) {
+    return w[i] + mix * d[i];
+}
+
 __global__ void am_simple_mix(const double* w, const double* d, double mix, int n, double* out) {
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) out[i] = w[i] + mix * d[i];
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        out[i] = simple_mix_value(w, d, mix, i);
 }
 
 struct AmMixArgs {
@@ -62,11 +69,11 @@ struct AmSolveArgs {
 // CpuAndersonMixing.cpp:128-147), with the host code's operation order and no contraction into
 // fused multiply-adds, so the coefficients are those the host would compute. Row j of the
 // elimination belongs to thread j; one workgroup of AM_MAX_HIST threads.
-__global__ void __launch_bounds__(AM_MAX_HIST) am_solve(const double* ring, AmSolveArgs a, double* coef) {
+__device__ __forceinline__ void am_solve_body(const double* ring, int na, int L, const int* rows, double* coef,
+                                              double* lds) {
 #pragma clang fp contract(off)
-    // dynamic LDS: D (na+1 rows of L), u (na x na), v, x
-    extern __shared__ double lds[];
-    const int na = a.na, L = a.L, t = threadIdx.x;
+    // LDS: D (na+1 rows of L), u (na x na), v, x
+    const int t = threadIdx.x;
     double* D = lds;
     double (*u)[AM_MAX_HIST] = reinterpret_cast<double (*)[AM_MAX_HIST]>(D + (size_t)(na + 1) * L);
     double* v = reinterpret_cast<double*>(u + na);
@@ -74,7 +81,7 @@ __global__ void __launch_bounds__(AM_MAX_HIST) am_solve(const double* ring, AmSo
     // the rows D(n, .), n = 0 (newest) .. na, in one coalesced pass
     for (int e = t; e < (na + 1) * L; e += blockDim.x) {
         const int n = e / L;
-        D[e] = ring[(size_t)a.rows[n] * L + (e - n * L)];
+        D[e] = ring[(size_t)rows[n] * L + (e - n * L)];
     }
     __syncthreads();
     const double* d0 = D;
@@ -106,6 +113,11 @@ __global__ void __launch_bounds__(AM_MAX_HIST) am_solve(const double* ring, AmSo
     }
 }
 
+__global__ void __launch_bounds__(AM_MAX_HIST) am_solve(const double* ring, AmSolveArgs a, double* coef) {
+    extern __shared__ double lds[];
+    am_solve_body(ring, a.na, a.L, a.rows, coef, lds);
+}
+
 // w_new = (w_0 + d_0) + sum_i a_i ((w_{i+1} + d_{i+1}) - (w_0 + d_0)), same operation order
 __global__ void am_mix(AmMixArgs args, int n, double* out) {
     for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
@@ -116,7 +128,63 @@ __global__ void am_mix(AmMixArgs args, int n, double* out) {
     }
 }
 
+// ---- batched steps: blockIdx.z (dot products) or the last grid index is the mixing
+__global__ void __launch_bounds__(RED_THREADS) am_dot_batch_stage1(const AmBatchItem* items, int n_var,
+                                                                   double* partials) {
+    __shared__ double ws[16];
+    const AmBatchItem& it = items[blockIdx.z];
+    const int n = blockIdx.y;
+    if (n > it.na) return;
+    const double acc = dot_share(it.d, it.dh[n], n_var, ws);
+    if (threadIdx.x == 0) partials[((size_t)blockIdx.z * AM_MAX_HIST + n) * RED_BLOCKS + blockIdx.x] = acc;
+}
+
+__global__ void __launch_bounds__(RED_BLOCKS) am_dot_batch_stage2(const AmBatchItem* items, const double* partials) {
+    __shared__ double ws[16];
+    const AmBatchItem& it = items[blockIdx.y];
+    const int n = blockIdx.x;
+    if (n > it.na) return;
+    double v = partials[((size_t)blockIdx.y * AM_MAX_HIST + n) * RED_BLOCKS + threadIdx.x];
+    v = block_sum(v, ws);
+    if (threadIdx.x == 0) it.dots_out[n] = v;
+}
+
+__global__ void __launch_bounds__(AM_MAX_HIST) am_solve_batch(const AmBatchItem* items) {
+    extern __shared__ double lds[];
+    const AmBatchItem& it = items[blockIdx.x];
+    if (it.mode != 2) return;
+    am_solve_body(it.dots, it.na, it.L, it.drows, it.coef, lds);
+}
+
+__global__ void am_mix_batch(const AmBatchItem* items, int n) {
+    const AmBatchItem& it = items[blockIdx.y];
+    if (it.mode == 1) {
+        for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+            it.out[i] = simple_mix_value(it.w, it.d, it.mix, i);
+    } else if (it.mode == 2) {
+        // the expression of am_mix, history rows n = 0 (newest) .. na
+        for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+            const double w1 = it.wh[0][j], e1 = it.dh[0][j];
+            double v = w1 + e1;
+            for (int i = 0; i < it.na; i++) v += it.coef[i] * (it.wh[i + 1][j] + it.dh[i + 1][j] - w1 - e1);
+            it.out[j] = v;
+        }
+    }
+}
+
+size_t solve_lds_bytes(int na, int L) {
+    return sizeof(double) * ((size_t)(na + 1) * L + (size_t)na * AM_MAX_HIST + 2 * AM_MAX_HIST);
+}
+
 }  // namespace
+
+void AmBatchWorkspace::reserve(int n) {
+    if (n <= capacity) return;
+    host_table.allocate_host_pinned(sizeof(AmBatchItem) * n);
+    table.allocate(sizeof(AmBatchItem) * n);
+    partials.allocate(sizeof(double) * n * AM_MAX_HIST * RED_BLOCKS);
+    capacity = n;
+}
 
 AndersonMixingDevice::AndersonMixingDevice(fts_context* ctx, int n_var, int max_hist, double start_error,
                                            double mix_min, double mix_init, bool reduce_memory)
@@ -148,9 +216,12 @@ AndersonMixingDevice::AndersonMixingDevice(fts_context* ctx, int n_var, int max_
     FTS_HIP_CHECK(hipMemset(dots_ring_.get(), 0, dots_ring_.bytes()));
     // the solve's LDS grows with the history (up to about 66 KB at 63)
     const size_t solve_lds = sizeof(double) * (rows * rows + (rows - 1) * AM_MAX_HIST + 2 * AM_MAX_HIST);
-    if (solve_lds > 64 * 1024)
+    if (solve_lds > 64 * 1024) {
         FTS_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&am_solve),
                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)solve_lds));
+        FTS_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&am_solve_batch),
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)solve_lds));
+    }
     reset();
 }
 
@@ -169,12 +240,54 @@ double* AndersonMixingDevice::staged_d() const {
     return d_hist_.as<double>() + (size_t)((ring_start_ + ring_n_) % (max_hist_ + 1)) * n_var_;
 }
 
+AmBatchItem AndersonMixingDevice::plan(const double* w, const double* d, double* out, double old_err, double err) {
+    const int L = max_hist_ + 1;
+    AmBatchItem it{};
+    it.mode = 1;
+    it.na = -1;
+    it.L = L;
+    it.w = w;
+    it.d = d;
+    it.out = out;
+    it.dots = dots_ring_.as<const double>();
+    it.coef = coef_.as<double>();
+    auto ring_row = [&](int n) { return (ring_start_ + ring_n_ - n - 1 + L) % L; };
+    auto dots_row = [&](int n) { return (dots_start_ + dots_n_ - n - 1 + L) % L; };
+
+    if (err < start_error_ || n_anderson_ >= 0) n_anderson_ = n_anderson_ + 1;
+    if (n_anderson_ >= 0) {
+        n_anderson_ = std::min(max_hist_, n_anderson_);
+        // the input enters the ring at slot (ring_start + ring_n) % L
+        if (ring_n_ == L) ring_start_ = (ring_start_ + 1) % L;
+        ring_n_ = std::min(ring_n_ + 1, L);
+        it.na = n_anderson_;
+        for (int n = 0; n <= it.na; n++) {
+            it.wh[n] = w_hist_.as<const double>() + (size_t)ring_row(n) * n_var_;
+            it.dh[n] = d_hist_.as<const double>() + (size_t)ring_row(n) * n_var_;
+        }
+        // dots[n] = d . d_hist[n], n = 0..n_anderson (no volume weights), into the next row
+        // of the ring of dot-product rows
+        it.dots_out = dots_ring_.as<double>() + (size_t)((dots_start_ + dots_n_) % L) * L;
+        if (dots_n_ == L) dots_start_ = (dots_start_ + 1) % L;
+        dots_n_ = std::min(dots_n_ + 1, L);
+    }
+    if (n_anderson_ <= 0) {
+        if (old_err < err) mix_ = std::max(mix_ * 0.7, mix_min_);
+        else mix_ = mix_ * 1.01;
+        it.mode = 1;
+        it.mix = mix_;
+    } else {
+        it.mode = 2;
+        for (int n = 0; n <= n_anderson_; n++) it.drows[n] = dots_row(n);
+    }
+    return it;
+}
+
 void AndersonMixingDevice::calculate_new_fields(double* w_new, const double* w, const double* d, double old_err,
                                                 double err, bool on_device, bool staged) {
     FTS_REQUIRE(!staged || (on_device && will_insert(err) && w == staged_w() && d == staged_d()),
                 "staged fields must sit in the next history slot");
     hipStream_t st = ctx_->stream;
-    const int L = max_hist_ + 1;
     const size_t nb = sizeof(double) * n_var_;
     const double* dw = w;
     const double* dd = d;
@@ -184,62 +297,44 @@ void AndersonMixingDevice::calculate_new_fields(double* w_new, const double* w, 
         dw = in_w_.as<double>();
         dd = in_d_.as<double>();
     }
-    auto ring_row = [&](int n) { return (ring_start_ + ring_n_ - n - 1 + L) % L; };
-    auto dots_row = [&](int n) { return (dots_start_ + dots_n_ - n - 1 + L) % L; };
-
-    if (err < start_error_ || n_anderson_ >= 0) n_anderson_ = n_anderson_ + 1;
-    if (n_anderson_ >= 0) {
-        n_anderson_ = std::min(max_hist_, n_anderson_);
-        const int slot = (ring_start_ + ring_n_) % L;
-        // the rings may be pinned host memory: the runtime picks the direction
-        if (!staged) {
-            FTS_HIP_CHECK(hipMemcpyAsync(w_hist_.as<double>() + (size_t)slot * n_var_, dw, nb, hipMemcpyDefault, st));
-            FTS_HIP_CHECK(hipMemcpyAsync(d_hist_.as<double>() + (size_t)slot * n_var_, dd, nb, hipMemcpyDefault, st));
-        }
-        if (ring_n_ == L) ring_start_ = (ring_start_ + 1) % L;
-        ring_n_ = std::min(ring_n_ + 1, L);
-        // dots[i] = d . d_hist[i], i = 0..n_anderson (no volume weights), straight into the next row
-        // of the device ring of dot-product rows
-        const int ds = (dots_start_ + dots_n_) % L;
-        double* drow = dots_ring_.as<double>() + (size_t)ds * L;
-        for (int i0 = 0; i0 <= n_anderson_; i0 += MAX_BATCH) {
-            AmDotBatch b{};
-            b.n = std::min(MAX_BATCH, n_anderson_ + 1 - i0);
-            b.a = dd;
-            for (int k = 0; k < b.n; k++) b.b[k] = d_hist_.as<double>() + (size_t)ring_row(i0 + k) * n_var_;
-            hipLaunchKernelGGL(am_dot_stage1, dim3(RED_BLOCKS, b.n), dim3(RED_THREADS), 0, st, b, n_var_,
-                               partials_.as<double>());
-            hipLaunchKernelGGL(am_dot_stage2, dim3(b.n), dim3(RED_BLOCKS), 0, st, partials_.as<double>(),
-                               drow + i0);
-            FTS_HIP_CHECK(hipGetLastError());
-        }
-        if (dots_n_ == L) dots_start_ = (dots_start_ + 1) % L;
-        dots_n_ = std::min(dots_n_ + 1, L);
+    // the rings may be pinned host memory: the runtime picks the direction
+    if (!staged && will_insert(err)) {
+        FTS_HIP_CHECK(hipMemcpyAsync(staged_w(), dw, nb, hipMemcpyDefault, st));
+        FTS_HIP_CHECK(hipMemcpyAsync(staged_d(), dd, nb, hipMemcpyDefault, st));
     }
     double* out = on_device ? w_new : out_.as<double>();
+    const AmBatchItem it = plan(dw, dd, out, old_err, err);
+    for (int i0 = 0; i0 <= it.na; i0 += MAX_BATCH) {
+        AmDotBatch b{};
+        b.n = std::min(MAX_BATCH, it.na + 1 - i0);
+        b.a = dd;
+        for (int k = 0; k < b.n; k++) b.b[k] = it.dh[i0 + k];
+        hipLaunchKernelGGL(am_dot_stage1, dim3(RED_BLOCKS, b.n), dim3(RED_THREADS), 0, st, b, n_var_,
+                           partials_.as<double>());
+        hipLaunchKernelGGL(am_dot_stage2, dim3(b.n), dim3(RED_BLOCKS), 0, st, partials_.as<double>(),
+                           it.dots_out + i0);
+        FTS_HIP_CHECK(hipGetLastError());
+    }
     const int gb = std::min(1024, ceil_div(n_var_, 256));
-    if (n_anderson_ <= 0) {
-        if (old_err < err) mix_ = std::max(mix_ * 0.7, mix_min_);
-        else mix_ = mix_ * 1.01;
-        hipLaunchKernelGGL(am_simple_mix, dim3(gb), dim3(256), 0, st, dw, dd, mix_, n_var_, out);
+    if (it.mode == 1) {
+        hipLaunchKernelGGL(am_simple_mix, dim3(gb), dim3(256), 0, st, dw, dd, it.mix, n_var_, out);
         FTS_HIP_CHECK(hipGetLastError());
     } else {
-        const int na = n_anderson_;
+        const int na = it.na;
         AmSolveArgs sa{};
         sa.na = na;
-        sa.L = L;
-        for (int n = 0; n <= na; n++) sa.rows[n] = dots_row(n);
-        const size_t lds = sizeof(double) * ((size_t)(na + 1) * L + (size_t)na * AM_MAX_HIST + 2 * AM_MAX_HIST);
-        hipLaunchKernelGGL(am_solve, dim3(1), dim3(AM_MAX_HIST), lds, st, dots_ring_.as<const double>(), sa,
-                           coef_.as<double>());
+        sa.L = it.L;
+        for (int n = 0; n <= na; n++) sa.rows[n] = it.drows[n];
+        hipLaunchKernelGGL(am_solve, dim3(1), dim3(AM_MAX_HIST), solve_lds_bytes(na, it.L), st,
+                           dots_ring_.as<const double>(), sa, coef_.as<double>());
         FTS_HIP_CHECK(hipGetLastError());
         AmMixArgs args{};
         args.n_hist = na;
-        args.w1 = w_hist_.as<double>() + (size_t)ring_row(0) * n_var_;
-        args.e1 = d_hist_.as<double>() + (size_t)ring_row(0) * n_var_;
+        args.w1 = it.wh[0];
+        args.e1 = it.dh[0];
         for (int i = 0; i < na; i++) {
-            args.w2[i] = w_hist_.as<double>() + (size_t)ring_row(i + 1) * n_var_;
-            args.e2[i] = d_hist_.as<double>() + (size_t)ring_row(i + 1) * n_var_;
+            args.w2[i] = it.wh[i + 1];
+            args.e2[i] = it.dh[i + 1];
         }
         args.a = coef_.as<const double>();
         hipLaunchKernelGGL(am_mix, dim3(gb), dim3(256), 0, st, args, n_var_, out);
@@ -249,6 +344,62 @@ void AndersonMixingDevice::calculate_new_fields(double* w_new, const double* w, 
         FTS_HIP_CHECK(hipMemcpyAsync(w_new, out, nb, hipMemcpyDeviceToHost, st));
         FTS_HIP_CHECK(hipStreamSynchronize(st));
     }
+}
+
+void AndersonMixingDevice::calculate_batch(fts_context* ctx, const std::vector<AndersonMixingDevice*>& ams,
+                                           const std::vector<char>& active, const std::vector<double*>& out,
+                                           const std::vector<const double*>& w, const std::vector<const double*>& d,
+                                           const std::vector<double>& old_err, const std::vector<double>& err,
+                                           const std::vector<char>& staged, AmBatchWorkspace& ws) {
+    const int n = (int)ams.size();
+    FTS_REQUIRE(n >= 1 && (int)active.size() == n && (int)out.size() == n && (int)w.size() == n &&
+                    (int)d.size() == n && (int)old_err.size() == n && (int)err.size() == n &&
+                    (int)staged.size() == n,
+                "one entry per mixing");
+    hipStream_t st = ctx->stream;
+    // the pinned table is rewritten only after its previous copy has left it
+    if (ws.copied) FTS_HIP_CHECK(hipEventSynchronize(ws.copied));
+    else FTS_HIP_CHECK(hipEventCreateWithFlags(&ws.copied, hipEventDisableTiming));
+    ws.reserve(n);
+    AmBatchItem* items = static_cast<AmBatchItem*>(ws.host_table.host());
+    const int n_var = ams[0]->n_var_;
+    int max_na = -1, max_solve = -1, L = ams[0]->max_hist_ + 1;
+    bool any = false;
+    for (int r = 0; r < n; r++) {
+        AndersonMixingDevice* a = ams[r];
+        FTS_REQUIRE(a->n_var_ == n_var && a->max_hist_ + 1 == L, "batched mixings share n_var and the history");
+        if (!active[r]) {
+            items[r] = AmBatchItem{};
+            items[r].mode = 0;
+            items[r].na = -1;
+            continue;
+        }
+        FTS_REQUIRE(!a->reduce_memory() && a->will_insert(err[r]) == (staged[r] != 0) &&
+                        (!staged[r] || (w[r] == a->staged_w() && d[r] == a->staged_d())),
+                    "a batched input that the mixing keeps must sit in its next history slot");
+        items[r] = a->plan(w[r], d[r], out[r], old_err[r], err[r]);
+        max_na = std::max(max_na, items[r].na);
+        if (items[r].mode == 2) max_solve = std::max(max_solve, items[r].na);
+        any = true;
+    }
+    if (!any) return;
+    FTS_HIP_CHECK(hipMemcpyAsync(ws.table.get(), items, sizeof(AmBatchItem) * n, hipMemcpyHostToDevice, st));
+    FTS_HIP_CHECK(hipEventRecord(ws.copied, st));
+    const AmBatchItem* dev = ws.table.as<const AmBatchItem>();
+    if (max_na >= 0) {
+        hipLaunchKernelGGL(am_dot_batch_stage1, dim3(RED_BLOCKS, max_na + 1, n), dim3(RED_THREADS), 0, st, dev,
+                           n_var, ws.partials.as<double>());
+        hipLaunchKernelGGL(am_dot_batch_stage2, dim3(max_na + 1, n), dim3(RED_BLOCKS), 0, st, dev,
+                           ws.partials.as<const double>());
+        FTS_HIP_CHECK(hipGetLastError());
+    }
+    if (max_solve > 0) {
+        hipLaunchKernelGGL(am_solve_batch, dim3(n), dim3(AM_MAX_HIST), solve_lds_bytes(max_solve, L), st, dev);
+        FTS_HIP_CHECK(hipGetLastError());
+    }
+    const int gb = std::min(1024, ceil_div(n_var, 256));
+    hipLaunchKernelGGL(am_mix_batch, dim3(gb, n), dim3(256), 0, st, dev, n_var);
+    FTS_HIP_CHECK(hipGetLastError());
 }
 
 }  // namespace fts

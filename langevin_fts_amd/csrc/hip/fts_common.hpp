@@ -98,6 +98,8 @@ public:
         return static_cast<T*>(ptr_);
     }
     void* get() const { return ptr_; }
+    // host address of a pinned allocation (nullptr for device memory)
+    void* host() const { return host_; }
     size_t bytes() const { return bytes_; }
 
 private:

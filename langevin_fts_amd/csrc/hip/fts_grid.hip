@@ -253,7 +253,8 @@ void launch_slab_ct(const Grid& g, int mode, const SlabBatch<T>& b) {
                     throw Error("slab item without the real slice its second pass reads (N=" + std::to_string(N) + ")");
             }
         }
-        const int bd = g.slab_bd[prec_index<T>()];
+        const int bd_mode = g.slab_bd_mode[prec_index<T>()][mode];
+        const int bd = bd_mode ? bd_mode : g.slab_bd[prec_index<T>()];
         launch_slab_wl_bd<T, N>(g, bd ? bd : wslab_default_bd(N), mode, b);
         return;
     }
@@ -439,6 +440,7 @@ namespace {
 struct TuneChoice {
     int bd, nl;
     std::string note;
+    int bd_mode[3];   // per slab mode, 0: bd
 };
 std::map<std::pair<int, int>, TuneChoice>& tune_cache() {
     static std::map<std::pair<int, int>, TuneChoice> c;
@@ -483,12 +485,15 @@ TuneChoice tune_wave_local(const Grid& g) {
     constexpr int NZC = N / 2 + 1, ITEMS = 2;
     const size_t M = (size_t)N * N * N, Mc = (size_t)N * N * wspec_pitch(N);
     DeviceBuffer spec, q, w, tab;
-    spec.allocate(sizeof(T2) * Mc * ITEMS);
+    // two spectra per item (c0, c1: the Richardson step and the two-spectrum forward
+    // transform of continuous chains) and one real slice (qin and qout)
+    spec.allocate(sizeof(T2) * Mc * 2 * ITEMS);
     q.allocate(sizeof(T) * M * ITEMS);
     w.allocate(sizeof(T) * M);
     tab.allocate(sizeof(T) * (2 * N + NZC));
     hipStream_t st = g.stream();
-    FTS_HIP_CHECK(hipMemsetAsync(spec.get(), 0, sizeof(T2) * Mc * ITEMS, st));
+    FTS_HIP_CHECK(hipMemsetAsync(spec.get(), 0, sizeof(T2) * Mc * 2 * ITEMS, st));
+    FTS_HIP_CHECK(hipMemsetAsync(q.get(), 0, sizeof(T) * M * ITEMS, st));
     FTS_HIP_CHECK(hipMemsetAsync(w.get(), 0, sizeof(T) * M, st));
     FTS_HIP_CHECK(hipMemsetAsync(tab.get(), 0, sizeof(T) * (2 * N + NZC), st));
     SlabBatch<T> sb{};
@@ -496,18 +501,21 @@ TuneChoice tune_wave_local(const Grid& g) {
     sb.n = xb.n = ITEMS;
     for (int i = 0; i < ITEMS; i++) {
         SlabItem<T>& it = sb.it[i];
+        it.qin = q.as<T>() + i * M;
         it.qout = q.as<T>() + i * M;
-        it.c0 = spec.as<T2>() + i * Mc;
+        it.c0 = spec.as<T2>() + 2 * i * Mc;
+        it.c1 = spec.as<T2>() + (2 * i + 1) * Mc;
         it.w = w.as<T>();
         // FP64 discrete steps read exp(-w ds) from a table (coef < 0), FP32 steps evaluate it
         it.post0 = it.pre0 = std::is_same<T, double>::value ? (T)-1 : (T)0.01;
+        it.post1 = it.pre1 = (T)0.01;
         it.n_out = 1;
         it.forward = 1;
-        xb.it[i].c = spec.as<T2>() + i * Mc;
+        xb.it[i].c = spec.as<T2>() + 2 * i * Mc;
         xb.it[i].table = 0;
     }
     const T* bx = tab.as<T>();
-    TuneChoice c{wslab_default_bd(N), WPENCIL_DEFAULT_NL, ""};
+    TuneChoice c{wslab_default_bd(N), WPENCIL_DEFAULT_NL, "", {0, 0, 0}};
     // whole steps (slab pass then pencil pass, as the propagators alternate them)
     std::string note = "us per step of 2 propagators, slab threads/pencil columns";
     float t_def = 0.f, t_best = 1e30f;
@@ -532,6 +540,32 @@ TuneChoice tune_wave_local(const Grid& g) {
         c.bd = best.bd;
         c.nl = best.nl;
     }
+    // the forward (two spectra out) and Richardson modes of continuous chains hold
+    // more values per thread than the half step: each picks its own workgroup size
+    // among the candidates, timed alone (the pencil pass does not depend on it)
+    SlabBatch<T> sc = sb;
+    for (int i = 0; i < ITEMS; i++) {
+        sc.it[i].post0 = sc.it[i].pre0 = (T)0.01;
+        sc.it[i].n_out = 2;
+    }
+    for (int mode : {SLAB_FWD, SLAB_INV2}) {
+        note += mode == SLAB_FWD ? "; forward, two spectra:" : "; Richardson:";
+        float t_c = 0.f, t_m = 1e30f;
+        int bd_m = c.bd;
+        for (int bd : SLAB_BD_ALL) {
+            if (!slab_bd_ok(N, bd)) continue;
+            const float t = time_launches(st, [&] { launch_slab_wl_bd<T, N>(g, bd, mode, sc); });
+            char buf[32];
+            std::snprintf(buf, sizeof(buf), " %d:%.2f", bd, 1e3 * t);
+            note += buf;
+            if (bd == c.bd) t_c = t;
+            if (t < t_m) {
+                t_m = t;
+                bd_m = bd;
+            }
+        }
+        if (bd_m != c.bd && t_m < 0.98f * t_c) c.bd_mode[mode] = bd_m;
+    }
     FTS_HIP_CHECK(hipStreamSynchronize(st));
     c.note = "tuned (" + note + ")";
     return c;
@@ -542,6 +576,7 @@ template <typename T>
 void Grid::tune_fft_schedule() {
     const int p = prec_index<T>();
     slab_bd[p] = wslab_default_bd(ct_n);
+    for (int m = 0; m < 3; m++) slab_bd_mode[p][m] = 0;
     pencil_nl[p] = WPENCIL_DEFAULT_NL;
     const char* env = std::getenv("FTS_FFT_TUNE");
     if (!wave_local || (env && env[0] == '0')) {
@@ -553,7 +588,7 @@ void Grid::tune_fft_schedule() {
     const auto key = std::make_pair(ct_n, p);
     auto hit = cache.find(key);
     if (hit == cache.end()) {
-        TuneChoice c{slab_bd[p], pencil_nl[p], ""};
+        TuneChoice c{slab_bd[p], pencil_nl[p], "", {0, 0, 0}};
         try {
             switch (ct_n) {
                 case 32: c = tune_wave_local<T, 32>(*this); break;
@@ -564,12 +599,13 @@ void Grid::tune_fft_schedule() {
             }
         } catch (const Error& e) {
             (void)hipGetLastError();
-            c = TuneChoice{wslab_default_bd(ct_n), WPENCIL_DEFAULT_NL, std::string("defaults (tuning failed: ") +
-                                                                       e.what() + ")"};
+            c = TuneChoice{wslab_default_bd(ct_n), WPENCIL_DEFAULT_NL,
+                           std::string("defaults (tuning failed: ") + e.what() + ")", {0, 0, 0}};
         }
         hit = cache.emplace(key, c).first;
     }
     slab_bd[p] = hit->second.bd;
+    for (int m = 0; m < 3; m++) slab_bd_mode[p][m] = hit->second.bd_mode[m];
     pencil_nl[p] = hit->second.nl;
     fft_schedule[p] = hit->second.note;
 }
@@ -581,6 +617,7 @@ void Grid::set_fft_schedule(int precision, int threads, int columns) {
                                                std::to_string(ct_n));
     FTS_REQUIRE(columns == 4 || columns == 8 || columns == 16, "pencil columns must be 4, 8 or 16");
     slab_bd[precision] = threads;
+    for (int m = 0; m < 3; m++) slab_bd_mode[precision][m] = 0;
     pencil_nl[precision] = columns;
     fft_schedule[precision] = "set by the caller";
 }

@@ -109,6 +109,9 @@ public:
     // launch shapes of the wave-local kernels per precision ([0] FP64, [1] FP32):
     // slab workgroup size and pencil kz columns per wavefront, with how they were chosen
     int slab_bd[2] = {0, 0};
+    // per slab mode (forward, half step, Richardson step): a workgroup size that
+    // replaces slab_bd for that mode, 0: slab_bd
+    int slab_bd_mode[2][3] = {{0, 0, 0}, {0, 0, 0}};
     int pencil_nl[2] = {0, 0};
     std::string fft_schedule[2];
     template <typename T>
@@ -191,6 +194,12 @@ public:
     virtual void dq_dl(int p, double* out) = 0;
     virtual void sum_concentrations(int n, const int* kind, const int* idx, const double* scale, double* out,
                                     bool on_device) = 0;
+    // the same sums for n_rep replicas of a replicated plan at once: replica r reads
+    // block idx + r block_step, solvent idx + r solvent_step, writes out + r out_step
+    // (device output)
+    virtual void sum_concentrations_replicas(int n, const int* kind, const int* idx, const double* scale,
+                                             double* out, int n_rep, int block_step, int solvent_step,
+                                             size_t out_step) = 0;
     virtual void propagator(int prop, int n, int half, double* out, bool on_device) = 0;
     // bytes of device memory and of pinned host memory the solver holds
     virtual void memory_usage(size_t* device, size_t* host) const = 0;
@@ -204,6 +213,41 @@ public:
 std::unique_ptr<SolverBase> make_solver(Grid* g, const fts_plan* plan, int precision, const int* bc = nullptr,
                                         bool reduce_memory = false);
 
+constexpr int AM_MAX_HIST = 64;
+
+// One mixing's step in a batched launch (an entry of a device table): what
+// AndersonMixingDevice::calculate_new_fields decided on the host, with the device
+// rows it reads and writes.
+struct AmBatchItem {
+    int mode;      // 0: no step, 1: simple mixing, 2: Anderson mixing
+    int na;        // -1: input not in the history; else dot products d . dh[n] for n = 0..na
+    int L;         // ring length (row stride of the dot-product ring)
+    double mix;
+    const double* w;   // input fields
+    const double* d;   // input derivatives
+    double* out;       // new fields
+    double* dots_out;  // dot-product row receiving d . dh[n]
+    const double* dots;   // this mixing's ring of dot-product rows (L x L)
+    double* coef;         // na coefficients of the Gram solve
+    int drows[AM_MAX_HIST];            // dot-product ring rows of history n = 0 (newest) .. na
+    const double* wh[AM_MAX_HIST];     // field history rows n = 0 .. na
+    const double* dh[AM_MAX_HIST];     // derivative history rows n = 0 .. na
+};
+
+// Host and device tables and reduction scratch of batched mixing steps
+struct AmBatchWorkspace {
+    DeviceBuffer host_table, table, partials;
+    hipEvent_t copied = nullptr;   // recorded after the last copy of the host table
+    int capacity = 0;
+    void reserve(int n);
+    AmBatchWorkspace() = default;
+    AmBatchWorkspace(const AmBatchWorkspace&) = delete;
+    AmBatchWorkspace& operator=(const AmBatchWorkspace&) = delete;
+    ~AmBatchWorkspace() {
+        if (copied) (void)hipEventDestroy(copied);
+    }
+};
+
 class AndersonMixingDevice {
 public:
     AndersonMixingDevice(fts_context* ctx, int n_var, int max_hist, double start_error, double mix_min,
@@ -215,11 +259,25 @@ public:
                               bool on_device, bool staged = false);
     // whether a call with this error level inserts into the history (CpuAndersonMixing.cpp:86-93)
     bool will_insert(double err) const { return err < start_error_ || n_anderson_ >= 0; }
+    bool reduce_memory() const { return w_hist_.pinned_host(); }
     // the ring slots the next insertion fills (device, n_var doubles each)
     double* staged_w() const;
     double* staged_d() const;
 
+    // Steps of several mixings of one n_var in one set of launches. For each r with
+    // active[r], the host decisions and device results of
+    // ams[r]->calculate_new_fields(out[r], w[r], d[r], old_err[r], err[r], true, staged[r]),
+    // where an input the mixing keeps must be staged (already in its ring slots).
+    static void calculate_batch(fts_context* ctx, const std::vector<AndersonMixingDevice*>& ams,
+                                const std::vector<char>& active, const std::vector<double*>& out,
+                                const std::vector<const double*>& w, const std::vector<const double*>& d,
+                                const std::vector<double>& old_err, const std::vector<double>& err,
+                                const std::vector<char>& staged, AmBatchWorkspace& ws);
+
 private:
+    // the host part of a step: mixing decisions, ring insertion and the rows the
+    // device work reads (w, d: the inputs as they sit on the device)
+    AmBatchItem plan(const double* w, const double* d, double* out, double old_err, double err);
     fts_context* ctx_;
     int n_var_, max_hist_, n_anderson_;
     double start_error_, mix_min_, mix_, mix_init_;
@@ -273,9 +331,14 @@ private:
     double* ph(int r) const { return phi_.as<double>() + (size_t)r * S_ * M_; }
     void concentrations(SolverBase* solver);
     void run_lincomb(int rep, const std::vector<double>& cw, const std::vector<double>& cp,
-                     const std::vector<double>& c0, int n_out, double* out);
+                     const std::vector<double>& c0, int n_out, double* out, int n_rep = 1, size_t out_step = 0);
     void row_sums(const std::vector<const double*>& rows, const double* centre, bool square, double* out_dev);
+    void row_sums_strided(const double* base, int n, size_t stride, const double* centre, bool square,
+                          double* out_dev);
     double hamiltonian(int rep, SolverBase* solver);
+    void hamiltonians(const std::vector<int>& reps, SolverBase* solver, std::vector<double>& ham);
+    void mix_replicas(const std::vector<char>& mix, const std::vector<double>& old_err,
+                      const std::vector<double>& err, bool contiguous);
 
     fts_context* ctx_;
     SolverBase* solver_;
@@ -293,6 +356,10 @@ private:
     double h_const_;
     DeviceBuffer w_aux_, phi_, w_solver_, h_, lam_, am_w_, am_d_, noise_prev_, noise_cur_, w_backup_, phi_backup_,
         partials_, stats_;
+    // replicas: per-replica gather destinations (pinned host table and its device copy)
+    // and the batched mixing's tables
+    DeviceBuffer gather_host_, gather_dev_;
+    AmBatchWorkspace am_ws_;
     // learned guess: replay ring of (w_real, w_imag) pairs and the prepared prediction
     // the three inputs of the residual guess (and its target): the w+ the first residual
     // was evaluated at is the backup's (from_backup) or the current one

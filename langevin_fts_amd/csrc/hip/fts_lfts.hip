@@ -25,21 +25,24 @@ constexpr int LC_MAX_IN = 2 * FTS_MAX_TYPES;
 constexpr int STAT_BLOCKS = 128;
 constexpr int STAT_THREADS = 256;
 
-// out[r][i] = c0[r] + sum_j c[r][j] * in[j][i]
+// out[r][i] = c0[r] + sum_j c[r][j] * in[j][i]; blockIdx.y = replica q reads in[j] + q in_step
+// and writes out[r] + q out_step
 struct LinComb {
     int n_out, n_in;
     const double* in[LC_MAX_IN];
     double* out[LC_MAX_OUT];
     double c[LC_MAX_OUT][LC_MAX_IN];
     double c0[LC_MAX_OUT];
+    size_t in_step, out_step;
 };
 
 __global__ void __launch_bounds__(256) lincomb_kernel(LinComb a, int M) {
+    const size_t qi = blockIdx.y * a.in_step, qo = blockIdx.y * a.out_step;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < M; i += gridDim.x * blockDim.x) {
         double x[LC_MAX_IN];
 #pragma unroll
         for (int j = 0; j < LC_MAX_IN; j++)
-            if (j < a.n_in) x[j] = a.in[j][i];
+            if (j < a.n_in) x[j] = a.in[j][qi + i];
 #pragma unroll
         for (int r = 0; r < LC_MAX_OUT; r++) {
             if (r >= a.n_out) break;
@@ -47,7 +50,7 @@ __global__ void __launch_bounds__(256) lincomb_kernel(LinComb a, int M) {
 #pragma unroll
             for (int j = 0; j < LC_MAX_IN; j++)
                 if (j < a.n_in) v += a.c[r][j] * x[j];
-            a.out[r][i] = v;
+            a.out[r][qo + i] = v;
         }
     }
 }
@@ -58,17 +61,31 @@ struct Rows {
     const double* row[2 * FTS_MAX_TYPES];
 };
 
-__global__ void __launch_bounds__(STAT_THREADS) row_sum_stage1(Rows r, int M, const double* centre, int square,
-                                                              double* partials) {
-    __shared__ double ws[16];
-    const double* x = r.row[blockIdx.y];
-    const double c = centre ? centre[blockIdx.y] / M : 0.0;
+// this workgroup's share of sum_i (x_i - c) (or its square) over a grid of gridDim.x workgroups
+__device__ __forceinline__ double row_share(const double* x, int M, double c, int square, double* ws) {
     double acc = 0.0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < M; i += gridDim.x * blockDim.x) {
         const double v = x[i] - c;
         acc += square ? v * v : v;
     }
-    acc = block_sum(acc, ws);
+    return block_sum(acc, ws);
+}
+
+__global__ void __launch_bounds__(STAT_THREADS) row_sum_stage1(Rows r, int M, const double* centre, int square,
+                                                              double* partials) {
+    __shared__ double ws[16];
+    const double c = centre ? centre[blockIdx.y] / M : 0.0;
+    const double acc = row_share(r.row[blockIdx.y], M, c, square, ws);
+    if (threadIdx.x == 0) partials[blockIdx.y * gridDim.x + blockIdx.x] = acc;
+}
+
+// rows base + k stride, k = blockIdx.y (the same per-row sums as row_sum_stage1)
+__global__ void __launch_bounds__(STAT_THREADS) row_sum_strided_stage1(const double* base, size_t stride, int M,
+                                                                      const double* centre, int square,
+                                                                      double* partials) {
+    __shared__ double ws[16];
+    const double c = centre ? centre[blockIdx.y] / M : 0.0;
+    const double acc = row_share(base + blockIdx.y * stride, M, c, square, ws);
     if (threadIdx.x == 0) partials[blockIdx.y * gridDim.x + blockIdx.x] = acc;
 }
 
@@ -79,9 +96,11 @@ __global__ void __launch_bounds__(STAT_BLOCKS) row_sum_stage2(const double* part
     if (threadIdx.x == 0) out[blockIdx.x] = v;
 }
 
-// w_aux[S-1] -= mean(w_aux[S-1]) with the row sum already on the device
-__global__ void subtract_mean_kernel(double* x, const double* sum, int M) {
-    const double m = sum[0] / M;
+// w_aux[S-1] -= mean(w_aux[S-1]) with the row sum already on the device; blockIdx.y =
+// replica q: row x + q step, sum[q]
+__global__ void subtract_mean_kernel(double* x, size_t step, const double* sum, int M) {
+    const double m = sum[blockIdx.y] / M;
+    x += blockIdx.y * step;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < M; i += gridDim.x * blockDim.x) x[i] -= m;
 }
 
@@ -103,6 +122,41 @@ __global__ void gather_kernel(Gather g, int M) {
         }
 }
 
+// the gather of several replicas at once: per replica its destinations (a table in
+// device memory), the sources by stride from replica 0's
+struct GatherRep {
+    double* w_out[FTS_MAX_TYPES];   // null: the fields are read where they are
+    double* d_out[FTS_MAX_TYPES];   // null: the replica is not mixed this iteration
+    int scatter;                    // copy its mixed fields (am_w) back into w_aux
+};
+
+struct GatherShared {
+    int n;
+    double* wim[FTS_MAX_TYPES];         // imaginary w_aux rows of replica 0; replica q at + q wim_step
+    const double* h[FTS_MAX_TYPES];     // derivative rows of replica 0; + q h_step
+    double scale[FTS_MAX_TYPES];
+    size_t wim_step, h_step;
+    const double* am_w;                 // mixed fields of replica 0 (n rows); + q am_step
+    size_t am_step;
+};
+
+__global__ void gather_batch_kernel(GatherShared g, const GatherRep* reps, int M) {
+    const GatherRep& gr = reps[blockIdx.y];
+    const size_t q = blockIdx.y;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < M; i += gridDim.x * blockDim.x)
+        for (int c = 0; c < g.n; c++) {
+            if (gr.w_out[c]) gr.w_out[c][i] = g.wim[c][q * g.wim_step + i];
+            if (gr.d_out[c]) gr.d_out[c][i] = -g.scale[c] * g.h[c][q * g.h_step + i];
+        }
+}
+
+__global__ void scatter_batch_kernel(GatherShared g, const GatherRep* reps, int M) {
+    if (!reps[blockIdx.y].scatter) return;
+    const size_t q = blockIdx.y;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < M; i += gridDim.x * blockDim.x)
+        for (int c = 0; c < g.n; c++) g.wim[c][q * g.wim_step + i] = g.am_w[q * g.am_step + (size_t)c * M + i];
+}
+
 struct Scatter {
     int n;
     const double* src[FTS_MAX_TYPES];
@@ -116,6 +170,7 @@ __global__ void scatter_kernel(Scatter s, int M) {
 
 // Leimkuhler-Matthews step of the real fields (lfts.py:705-709):
 // w[c] += -lambda[c] * dt * s_c + 0.5 * (prev[c] + cur[c]) * sqrt(s_c)
+// blockIdx.y = replica q: w at + q w_step, lambda, prev and cur at + q r_step
 struct LmArgs {
     int n;
     double* w[FTS_MAX_TYPES];
@@ -124,12 +179,15 @@ struct LmArgs {
     const double* cur[FTS_MAX_TYPES];
     double dt_s[FTS_MAX_TYPES];
     double sqrt_s[FTS_MAX_TYPES];
+    size_t w_step, r_step;
 };
 
 __global__ void lm_kernel(LmArgs a, int M) {
+    const size_t qw = blockIdx.y * a.w_step, qr = blockIdx.y * a.r_step;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < M; i += gridDim.x * blockDim.x)
         for (int c = 0; c < a.n; c++)
-            a.w[c][i] += -a.lambda[c][i] * a.dt_s[c] + 0.5 * (a.prev[c][i] + a.cur[c][i]) * a.sqrt_s[c];
+            a.w[c][qw + i] += -a.lambda[c][qr + i] * a.dt_s[c] +
+                              0.5 * (a.prev[c][qr + i] + a.cur[c][qr + i]) * a.sqrt_s[c];
 }
 
 int grid_blocks(int M) { return std::min(2048, ceil_div(M, 256)); }
@@ -215,8 +273,12 @@ SaddlePoint::SaddlePoint(fts_context* ctx, SolverBase* solver, std::vector<Ander
     noise_cur_.allocate(f * std::max(R_, 1));
     w_backup_.allocate(f * S_);
     phi_backup_.allocate(f * S_);
-    partials_.allocate(sizeof(double) * 2 * FTS_MAX_TYPES * STAT_BLOCKS);
+    partials_.allocate(sizeof(double) * std::max(2 * FTS_MAX_TYPES, NR_ * S_) * STAT_BLOCKS);
     stats_.allocate(sizeof(double) * 4 * FTS_MAX_TYPES * NR_);
+    if (NR_ > 1) {
+        gather_host_.allocate_host_pinned(sizeof(GatherRep) * NR_);
+        gather_dev_.allocate(sizeof(GatherRep) * NR_);
+    }
     FTS_HIP_CHECK(hipMemsetAsync(w_aux_.get(), 0, f * S_, ctx_->stream));
     FTS_HIP_CHECK(hipMemsetAsync(phi_.get(), 0, f * S_, ctx_->stream));
     FTS_HIP_CHECK(hipMemsetAsync(noise_prev_.get(), 0, f * std::max(R_, 1), ctx_->stream));
@@ -280,9 +342,12 @@ void SaddlePoint::restore(const int* replicas) {
 }
 
 void SaddlePoint::run_lincomb(int rep, const std::vector<double>& cw, const std::vector<double>& cp,
-                              const std::vector<double>& c0, int n_out, double* out) {
-    // inputs: w_aux rows 0..S-1 of the replica, then its phi rows (cp empty: w_aux only)
+                              const std::vector<double>& c0, int n_out, double* out, int n_rep, size_t out_step) {
+    // inputs: w_aux rows 0..S-1 of the replica, then its phi rows (cp empty: w_aux only);
+    // n_rep replicas from rep on, the outputs of replica rep + q at out + q out_step
     LinComb a{};
+    a.in_step = (size_t)S_ * M_;
+    a.out_step = out_step;
     a.n_out = n_out;
     a.n_in = cp.empty() ? S_ : 2 * S_;
     for (int j = 0; j < S_; j++) {
@@ -297,8 +362,23 @@ void SaddlePoint::run_lincomb(int rep, const std::vector<double>& cw, const std:
             if (!cp.empty()) a.c[r][S_ + j] = cp[(size_t)r * S_ + j];
         }
     }
-    hipLaunchKernelGGL(lincomb_kernel, dim3(grid_blocks(M_)), dim3(256), 0, ctx_->stream, a, M_);
+    hipLaunchKernelGGL(lincomb_kernel, dim3(grid_blocks(M_), n_rep), dim3(256), 0, ctx_->stream, a, M_);
     FTS_HIP_CHECK(hipGetLastError());
+}
+
+void SaddlePoint::row_sums_strided(const double* base, int n, size_t stride, const double* centre, bool square,
+                                   double* out_dev) {
+    // fixed-order sums of rows base + k stride, k < n (the sums of row_sums)
+    const int chunk = std::max(2 * FTS_MAX_TYPES, NR_ * S_);
+    for (int r0 = 0; r0 < n; r0 += chunk) {
+        const int k = std::min(chunk, n - r0);
+        hipLaunchKernelGGL(row_sum_strided_stage1, dim3(STAT_BLOCKS, k), dim3(STAT_THREADS), 0, ctx_->stream,
+                           base + (size_t)r0 * stride, stride, M_, centre ? centre + r0 : nullptr, square ? 1 : 0,
+                           partials_.as<double>());
+        hipLaunchKernelGGL(row_sum_stage2, dim3(k), dim3(STAT_BLOCKS), 0, ctx_->stream, partials_.as<double>(),
+                           out_dev + r0);
+        FTS_HIP_CHECK(hipGetLastError());
+    }
 }
 
 void SaddlePoint::row_sums(const std::vector<const double*>& rows, const double* centre, bool square,
@@ -318,22 +398,18 @@ void SaddlePoint::row_sums(const std::vector<const double*>& rows, const double*
 }
 
 void SaddlePoint::concentrations(SolverBase* solver) {
-    // monomer fields of the solver (lfts.py:534-545), replica r's at r n_solver_w
-    for (int r = 0; r < NR_; r++)
-        run_lincomb(r, w_map_, {}, {}, n_solver_w_, w_solver_.as<double>() + (size_t)r * n_solver_w_ * M_);
+    // monomer fields of the solver (lfts.py:534-545), replica r's at r n_solver_w, all
+    // replicas in one launch
+    run_lincomb(0, w_map_, {}, {}, n_solver_w_, w_solver_.as<double>(), NR_, (size_t)n_solver_w_ * M_);
     std::vector<const double*> w(n_solver_w_ * NR_);
     for (size_t m = 0; m < w.size(); m++) w[m] = w_solver_.as<double>() + m * M_;
     solver->compute_propagators(w.data(), nullptr, true);
     solver->compute_concentrations();
-    std::vector<int> idx(term_index_.size());
-    for (int r = 0; r < NR_; r++) {
-        for (size_t t = 0; t < idx.size(); t++)
-            idx[t] = term_index_[t] + r * (term_kind_[t] == FTS_SRC_BLOCK ? n_blocks_ : n_solvents_);
-        for (int i = 0; i < S_; i++) {
-            const int o = term_off_[i], n = term_off_[i + 1] - o;
-            solver->sum_concentrations(n, term_kind_.data() + o, idx.data() + o, term_scale_.data() + o,
-                                        ph(r) + (size_t)i * M_, true);
-        }
+    // per monomer type, the sums of every replica in one set of launches
+    for (int i = 0; i < S_; i++) {
+        const int o = term_off_[i], n = term_off_[i + 1] - o;
+        solver->sum_concentrations_replicas(n, term_kind_.data() + o, term_index_.data() + o, term_scale_.data() + o,
+                                            ph(0) + (size_t)i * M_, NR_, n_blocks_, n_solvents_, (size_t)S_ * M_);
     }
 }
 
@@ -372,6 +448,81 @@ double SaddlePoint::hamiltonian(int rep, SolverBase* solver) {
     return hp + h + h_const_;
 }
 
+// hamiltonian(rep, solver) for several replicas, with one set of row sums over every
+// replica's w_aux rows (the same per-row sums) and one transfer
+void SaddlePoint::hamiltonians(const std::vector<int>& reps, SolverBase* solver, std::vector<double>& ham) {
+    const int n = NR_ * S_;
+    double* st = stats_.as<double>();
+    row_sums_strided(w_aux_.as<double>(), n, M_, nullptr, false, st);
+    row_sums_strided(w_aux_.as<double>(), n, M_, nullptr, true, st + n);
+    std::vector<double> m(2 * n);
+    FTS_HIP_CHECK(hipMemcpyAsync(m.data(), st, sizeof(double) * 2 * n, hipMemcpyDeviceToHost, ctx_->stream));
+    FTS_HIP_CHECK(hipStreamSynchronize(ctx_->stream));
+    for (int rep : reps) {
+        const double* sum = m.data() + (size_t)rep * S_;
+        const double* sq = m.data() + n + (size_t)rep * S_;
+        double h = 0.0;
+        for (int j = 0; j < S_; j++) h += h_lin_[j] * (sum[j] / M_);
+        for (int j = 0; j < S_; j++) h += h_sq_[j] * (sq[j] / M_);
+        double hp = 0.0;
+        for (int p = 0; p < P_; p++) hp -= pol_weight_[p] * std::log(solver->partition(p + rep * P_));
+        ham[rep] = hp + h + h_const_;
+    }
+}
+
+// One Anderson step (lfts.py:854-860) of every replica with mix[r], in batched launches:
+// the gather of the inputs (into the history slots the mixings keep), the mixings
+// (AndersonMixingDevice::calculate_batch) and, for non-contiguous imaginary rows, the
+// copy back into w_aux. Each replica's results equal its own step of the unbatched loop.
+void SaddlePoint::mix_replicas(const std::vector<char>& mix, const std::vector<double>& old_err,
+                               const std::vector<double>& err, bool contiguous) {
+    hipStream_t st = ctx_->stream;
+    // the previous table copy finished before this iteration's errors were read back
+    GatherRep* tab = static_cast<GatherRep*>(gather_host_.host());
+    std::vector<double*> out(NR_, nullptr);
+    std::vector<const double*> w(NR_, nullptr), d(NR_, nullptr);
+    std::vector<char> staged(NR_, 0);
+    for (int r = 0; r < NR_; r++) {
+        tab[r] = GatherRep{};
+        if (!mix[r]) continue;
+        double* am_w = am_w_.as<double>() + (size_t)r * I_ * M_;
+        double* am_d = am_d_.as<double>() + (size_t)r * I_ * M_;
+        staged[r] = ams_[r]->will_insert(err[r]) ? 1 : 0;
+        double* in_w = staged[r] ? ams_[r]->staged_w() : am_w;
+        double* in_d = staged[r] ? ams_[r]->staged_d() : am_d;
+        double* w_im = contiguous ? wa(r) + (size_t)imag_idx_[0] * M_ : am_w;
+        for (int c = 0; c < I_; c++) {
+            tab[r].w_out[c] = staged[r] || !contiguous ? in_w + (size_t)c * M_ : nullptr;
+            tab[r].d_out[c] = in_d + (size_t)c * M_;
+        }
+        tab[r].scatter = contiguous ? 0 : 1;
+        w[r] = staged[r] ? in_w : w_im;
+        d[r] = in_d;
+        out[r] = w_im;
+    }
+    FTS_HIP_CHECK(hipMemcpyAsync(gather_dev_.get(), tab, sizeof(GatherRep) * NR_, hipMemcpyHostToDevice, st));
+    GatherShared g{};
+    g.n = I_;
+    for (int c = 0; c < I_; c++) {
+        g.wim[c] = wa(0) + (size_t)imag_idx_[c] * M_;
+        g.h[c] = h_.as<const double>() + (size_t)c * M_;
+        g.scale[c] = dt_scaling_[imag_idx_[c]];
+    }
+    g.wim_step = (size_t)S_ * M_;
+    g.h_step = (size_t)I_ * M_;
+    g.am_w = am_w_.as<const double>();
+    g.am_step = (size_t)I_ * M_;
+    hipLaunchKernelGGL(gather_batch_kernel, dim3(grid_blocks(M_), NR_), dim3(256), 0, st, g,
+                       gather_dev_.as<const GatherRep>(), M_);
+    FTS_HIP_CHECK(hipGetLastError());
+    AndersonMixingDevice::calculate_batch(ctx_, ams_, mix, out, w, d, old_err, err, staged, am_ws_);
+    if (!contiguous) {
+        hipLaunchKernelGGL(scatter_batch_kernel, dim3(grid_blocks(M_), NR_), dim3(256), 0, st, g,
+                           gather_dev_.as<const GatherRep>(), M_);
+        FTS_HIP_CHECK(hipGetLastError());
+    }
+}
+
 void SaddlePoint::find(double tolerance, int max_iter, int verbose, double* log, int* n_iter, double* error,
                        double* hamiltonian_out) {
     FTS_REQUIRE(max_iter >= 1, "max_iter must be positive");
@@ -384,6 +535,9 @@ void SaddlePoint::find(double tolerance, int max_iter, int verbose, double* log,
     std::vector<double> stats(2 * nrows);
     bool contiguous = true;
     for (int c = 1; c < I_; c++) contiguous = contiguous && imag_idx_[c] == imag_idx_[0] + c;
+    // replicas mix in batched launches (mixings with history rings in device memory)
+    bool batched = NR_ > 1;
+    for (auto* a : ams_) batched = batched && !a->reduce_memory();
     // low-precision phase while the errors are large (none when no low-precision solver is set)
     bool low = lo_ != nullptr;
     for (int it = 1; it <= max_iter; it++) {
@@ -392,19 +546,18 @@ void SaddlePoint::find(double tolerance, int max_iter, int verbose, double* log,
         if (it == max_iter) low = false;
         SolverBase* solver = low ? lo_ : solver_;
         concentrations(solver);
-        // dH/dw of the imaginary fields, sign already flipped (lfts.py:219-245)
-        std::vector<const double*> rows;
-        for (int r = 0; r < NR_; r++) {
-            double* hr = h_.as<double>() + (size_t)r * I_ * M_;
-            run_lincomb(r, h_cw_, h_cp_, h_c0_, I_, hr);
-            for (int c = 0; c < I_; c++) rows.push_back(hr + (size_t)c * M_);
-        }
+        // dH/dw of the imaginary fields, sign already flipped (lfts.py:219-245), rows
+        // h + (r I + c) M of every replica, with their means and deviations
+        run_lincomb(0, h_cw_, h_cp_, h_c0_, I_, h_.as<double>(), NR_, (size_t)I_ * M_);
         double* sd = stats_.as<double>();
-        row_sums(rows, nullptr, false, sd);
-        row_sums(rows, sd, true, sd + nrows);
+        row_sums_strided(h_.as<double>(), nrows, M_, nullptr, false, sd);
+        row_sums_strided(h_.as<double>(), nrows, M_, sd, true, sd + nrows);
         FTS_HIP_CHECK(hipMemcpyAsync(stats.data(), sd, sizeof(double) * 2 * nrows, hipMemcpyDeviceToHost, st));
         FTS_HIP_CHECK(hipStreamSynchronize(st));
         int active = 0;
+        const std::vector<double> prev_err = err;
+        std::vector<char> to_mix(NR_, 0);
+        std::vector<int> need_h;
         for (int r = 0; r < NR_; r++) {
             if (done[r]) continue;
             const double old_err = err[r];
@@ -419,9 +572,10 @@ void SaddlePoint::find(double tolerance, int max_iter, int verbose, double* log,
             // (verbose 2) or at the final one
             const bool converged = !low && e < tolerance;
             const bool last = converged || it == max_iter;
-            if ((NR_ == 1 && verbose == 2) || last) {
+            if (NR_ > 1 && last) need_h.push_back(r);
+            if (NR_ == 1 && (verbose == 2 || last)) {
                 ham[r] = hamiltonian(r, solver);
-                if (NR_ == 1 && log && verbose >= 1) {
+                if (log && verbose >= 1) {
                     double* lr = log + (size_t)(verbose == 2 ? it - 1 : 0) * row;
                     lr[0] = it;
                     lr[1] = stats[I_ - 1] / M_;   // mass error: mean of the pressure-field derivative
@@ -446,6 +600,10 @@ void SaddlePoint::find(double tolerance, int max_iter, int verbose, double* log,
                     apply_residual_guess(*res_cnn_);
                     continue;
                 }
+            }
+            if (batched) {
+                to_mix[r] = 1;
+                continue;
             }
             // Anderson mixing of the imaginary fields with d = -dt_scaling * dH/dw (lfts.py:854-860)
             double* am_w = am_w_.as<double>() + (size_t)r * I_ * M_;
@@ -479,6 +637,9 @@ void SaddlePoint::find(double tolerance, int max_iter, int verbose, double* log,
                 FTS_HIP_CHECK(hipGetLastError());
             }
         }
+        // replicas: the Hamiltonians of those that stop here, then one mixing step of the others
+        if (!need_h.empty()) hamiltonians(need_h, solver, ham);
+        if (batched && active) mix_replicas(to_mix, prev_err, err, contiguous);
         if (active == 0) break;
         if (low) {
             bool below = true;
@@ -491,12 +652,9 @@ void SaddlePoint::find(double tolerance, int max_iter, int verbose, double* log,
     res_cnn_ = nullptr;
     res_record_ = false;
     // zero mean of the pressure field (lfts.py:865)
-    std::vector<const double*> last_rows;
-    for (int r = 0; r < NR_; r++) last_rows.push_back(wa(r) + (size_t)(S_ - 1) * M_);
-    row_sums(last_rows, nullptr, false, stats_.as<double>());
-    for (int r = 0; r < NR_; r++)
-        hipLaunchKernelGGL(subtract_mean_kernel, dim3(grid_blocks(M_)), dim3(256), 0, st,
-                           wa(r) + (size_t)(S_ - 1) * M_, stats_.as<const double>() + r, M_);
+    row_sums_strided(wa(0) + (size_t)(S_ - 1) * M_, NR_, (size_t)S_ * M_, nullptr, false, stats_.as<double>());
+    hipLaunchKernelGGL(subtract_mean_kernel, dim3(grid_blocks(M_), NR_), dim3(256), 0, st,
+                       wa(0) + (size_t)(S_ - 1) * M_, (size_t)S_ * M_, stats_.as<const double>(), M_);
     FTS_HIP_CHECK(hipGetLastError());
     FTS_HIP_CHECK(hipStreamSynchronize(st));
     for (int r = 0; r < NR_; r++) {
@@ -512,24 +670,24 @@ void SaddlePoint::langevin_update(const double* noise, bool on_device, double dt
     hipStream_t st = ctx_->stream;
     FTS_HIP_CHECK(hipMemcpyAsync(noise_cur_.get(), noise, sizeof(double) * M_ * R_ * NR_,
                                  on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st));
-    for (int rep = 0; rep < NR_; rep++) {
-        // dH/dw of the real fields (no sign flip) from the current w_aux and phi
-        double* lam = lam_.as<double>() + (size_t)rep * R_ * M_;
-        run_lincomb(rep, lam_cw_, lam_cp_, lam_c0_, R_, lam);
-        LmArgs a{};
-        a.n = R_;
-        for (int c = 0; c < R_; c++) {
-            const double s = dt_scaling_[real_idx_[c]];
-            a.w[c] = wa(rep) + (size_t)real_idx_[c] * M_;
-            a.lambda[c] = lam + (size_t)c * M_;
-            a.prev[c] = noise_prev_.as<double>() + ((size_t)rep * R_ + c) * M_;
-            a.cur[c] = noise_cur_.as<double>() + ((size_t)rep * R_ + c) * M_;
-            a.dt_s[c] = dt * s;
-            a.sqrt_s[c] = std::sqrt(s);
-        }
-        hipLaunchKernelGGL(lm_kernel, dim3(grid_blocks(M_)), dim3(256), 0, st, a, M_);
-        FTS_HIP_CHECK(hipGetLastError());
+    // dH/dw of the real fields (no sign flip) from the current w_aux and phi, then the
+    // update, each for every replica in one launch
+    run_lincomb(0, lam_cw_, lam_cp_, lam_c0_, R_, lam_.as<double>(), NR_, (size_t)R_ * M_);
+    LmArgs a{};
+    a.n = R_;
+    for (int c = 0; c < R_; c++) {
+        const double s = dt_scaling_[real_idx_[c]];
+        a.w[c] = wa(0) + (size_t)real_idx_[c] * M_;
+        a.lambda[c] = lam_.as<const double>() + (size_t)c * M_;
+        a.prev[c] = noise_prev_.as<const double>() + (size_t)c * M_;
+        a.cur[c] = noise_cur_.as<const double>() + (size_t)c * M_;
+        a.dt_s[c] = dt * s;
+        a.sqrt_s[c] = std::sqrt(s);
     }
+    a.w_step = (size_t)S_ * M_;
+    a.r_step = (size_t)R_ * M_;
+    hipLaunchKernelGGL(lm_kernel, dim3(grid_blocks(M_), NR_), dim3(256), 0, st, a, M_);
+    FTS_HIP_CHECK(hipGetLastError());
     // the current noise becomes the previous one (lfts.py:712)
     std::swap(noise_prev_, noise_cur_);
     FTS_HIP_CHECK(hipStreamSynchronize(st));

@@ -126,18 +126,23 @@ __global__ void solvent_phi_kernel(const T* w, double ds, double vf, const doubl
         out[i] = exp(-(double)w[i] * ds) * s;
 }
 
-// out = (accumulate ? out : 0) + sum_t scale_t src_t
+// out = (accumulate ? out : 0) + sum_t scale_t src_t; blockIdx.y = r sums the sources
+// src_t + r step_t into out + r out_step (replicated plans)
 struct SumBatch {
     int n;
     int accumulate;
     const double* src[MAX_BATCH];
     double scale[MAX_BATCH];
+    size_t step[MAX_BATCH];
+    size_t out_step;
 };
 
 static __global__ void sum_kernel(SumBatch b, int M, double* out) {
+    const size_t r = blockIdx.y;
+    out += r * b.out_step;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < M; i += gridDim.x * blockDim.x) {
         double acc = b.accumulate ? out[i] : 0.0;
-        for (int t = 0; t < b.n; t++) acc += b.scale[t] * b.src[t][i];
+        for (int t = 0; t < b.n; t++) acc += b.scale[t] * b.src[t][r * b.step[t] + i];
         out[i] = acc;
     }
 }

@@ -38,6 +38,8 @@ public:
     void dq_dl(int p, double* out) override;
     void sum_concentrations(int n, const int* kind, const int* idx, const double* scale, double* out,
                             bool on_device) override;
+    void sum_concentrations_replicas(int n, const int* kind, const int* idx, const double* scale, double* out,
+                                     int n_rep, int block_step, int solvent_step, size_t out_step) override;
     void propagator(int prop, int n, int half, double* out, bool on_device) override;
     void set_graphs(bool on) override {
         graphs_ = on;
@@ -107,9 +109,10 @@ private:
     struct GraphKey {
         std::vector<const void*> ptrs;
         double volume = 0.0;
-        int bd = 0, nl = 0;
+        int bd = 0, nl = 0, bd_fwd = 0, bd_inv2 = 0;
         bool operator==(const GraphKey& o) const {
-            return ptrs == o.ptrs && volume == o.volume && bd == o.bd && nl == o.nl;
+            return ptrs == o.ptrs && volume == o.volume && bd == o.bd && nl == o.nl && bd_fwd == o.bd_fwd &&
+                   bd_inv2 == o.bd_inv2;
         }
     };
     GraphKey graph_key(const double* const* w, const double* const* q_init) const;
@@ -816,6 +819,8 @@ typename Solver<T>::GraphKey Solver<T>::graph_key(const double* const* w, const 
     k.volume = g_->volume;
     const int p = std::is_same<T, float>::value ? 1 : 0;
     k.bd = g_->slab_bd[p];
+    k.bd_fwd = g_->slab_bd_mode[p][SLAB_FWD];
+    k.bd_inv2 = g_->slab_bd_mode[p][SLAB_INV2];
     k.nl = g_->pencil_nl[p];
     return k;
 }
@@ -1095,34 +1100,48 @@ void Solver<T>::dq_dl(int p, double* out) {
 }
 
 template <typename T>
-void Solver<T>::sum_concentrations(int n, const int* kind, const int* idx, const double* scale, double* out,
-                                   bool on_device) {
-    double* dst = on_device ? out : g_->scratch64.as<double>();
+void Solver<T>::sum_concentrations_replicas(int n, const int* kind, const int* idx, const double* scale, double* out,
+                                            int n_rep, int block_step, int solvent_step, size_t out_step) {
+    FTS_REQUIRE(n_rep >= 1 && n_rep <= 65535, "replica count out of range");
     const int gb = std::min(1024, ceil_div(M_, 256));
     SumBatch sb{};
+    sb.out_step = out_step;
     int first = 1;
     auto flush = [&]() {
         sb.accumulate = first ? 0 : 1;
-        hipLaunchKernelGGL(sum_kernel, dim3(gb), dim3(256), 0, g_->stream(), sb, M_, dst);
+        hipLaunchKernelGGL(sum_kernel, dim3(gb, n_rep), dim3(256), 0, g_->stream(), sb, M_, out);
         FTS_HIP_CHECK(hipGetLastError());
         first = 0;
         sb.n = 0;
     };
     for (int t = 0; t < n; t++) {
         const double* src;
+        size_t step;
         if (kind[t] == FTS_SRC_BLOCK) {
-            FTS_REQUIRE(idx[t] >= 0 && idx[t] < (int)blocks_.size(), "block index out of range");
+            FTS_REQUIRE(idx[t] >= 0 && idx[t] + (size_t)(n_rep - 1) * block_step < blocks_.size(),
+                        "block index out of range");
             src = phib_.as<double>() + (size_t)idx[t] * M_;
+            step = (size_t)block_step * M_;
         } else {
-            FTS_REQUIRE(idx[t] >= 0 && idx[t] < (int)svf_.size(), "solvent index out of range");
+            FTS_REQUIRE(idx[t] >= 0 && idx[t] + (size_t)(n_rep - 1) * solvent_step < svf_.size(),
+                        "solvent index out of range");
             src = phis_.as<double>() + (size_t)idx[t] * M_;
+            step = (size_t)solvent_step * M_;
         }
         sb.src[sb.n] = src;
         sb.scale[sb.n] = scale ? scale[t] : 1.0;
+        sb.step[sb.n] = step;
         sb.n++;
         if (sb.n == MAX_BATCH) flush();
     }
     if (sb.n || first) flush();
+}
+
+template <typename T>
+void Solver<T>::sum_concentrations(int n, const int* kind, const int* idx, const double* scale, double* out,
+                                   bool on_device) {
+    double* dst = on_device ? out : g_->scratch64.as<double>();
+    sum_concentrations_replicas(n, kind, idx, scale, dst, 1, 0, 0, 0);
     if (!on_device) {
         FTS_HIP_CHECK(hipMemcpyAsync(out, dst, sizeof(double) * M_, hipMemcpyDeviceToHost, g_->stream()));
         FTS_HIP_CHECK(hipStreamSynchronize(g_->stream()));
