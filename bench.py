@@ -782,8 +782,8 @@ def main():
                     help="learned: after the plain arm, train the CNN guess and time the steps that use it")
     ap.add_argument("--guess-train-steps", type=int, default=1000, help="Langevin steps the CNN trains on")
     ap.add_argument("--guess-train-iters", type=int, default=16, help="Adam iterations per training step")
-    ap.add_argument("--guess-lr", type=float, default=1e-3, help="initial Adam learning rate of the guess")
-    ap.add_argument("--guess-lr-final", type=float, default=1e-4, help="learning rate at the end of training")
+    ap.add_argument("--guess-lr", type=float, default=5e-4, help="initial Adam learning rate of the guess")
+    ap.add_argument("--guess-lr-final", type=float, default=5e-5, help="learning rate at the end of training")
     ap.add_argument("--guess-replay", type=int, default=64, help="stored training samples replayed in training")
     ap.add_argument("--guess-inputs", default="residual", choices=["residual", "delta", "absolute"],
                     help="residual: the CNN sees w-, the first residual of the search and the starting w+ and "
@@ -955,6 +955,9 @@ def main():
             "learned_vs_plain": (args.steps / learned64["s"]) / (args.steps / plain_s),
             "guess_train_steps": n_train, "train_step_ms": 1e3 * train_step_s,
             "train_saddle_iterations_per_step": train["iterations"] / max(n_train, 1),
+            # the newest sample's loss (scaled units) over the first and the last 50 training steps
+            "train_loss_first50": float(np.mean(sim.guess_losses[:50])) if sim.guess_losses else None,
+            "train_loss_last50": float(np.mean(sim.guess_losses[-50:])) if sim.guess_losses else None,
             # a 10,000-step run that trains on its first n_train steps, at the measured rates
             "run10k_steps_per_s": n_run / (n_train * train_step_s + (n_run - n_train) * learned64["s"] / args.steps)})
     if kern:

@@ -15,7 +15,9 @@
 //   backward         data gradients: the same MFMA kernel with flipped, transposed
 //                    bf16 weights and the ReLU mask of the layer input; hidden weight
 //                    gradients: MFMA GEMMs over voxels, split over voxel chunks and
-//                    reduced in a fixed order
+//                    reduced in a fixed order, with both operands read from
+//                    channel-major copies (written by the producing kernels while
+//                    training) when nz % 8 == 0
 //   optimizer        Adam on FP32 master weights, then bf16 copies for the kernels
 //
 // Tap t = 9 (dx + 1) + 3 (dy + 1) + (dz + 1); the flipped tap is 26 - t.
@@ -34,6 +36,7 @@ constexpr int TAPS = 27;
 constexpr int NHID = 4;                                 // hidden C->C layers
 constexpr int HSZ = TAPS * C * C + C;
 constexpr int WG_CHUNK = 1024;                          // voxels per block in the weight-gradient kernels
+constexpr int HID_CHUNK = 4096;                         // voxels per block of the channel-major hidden kernel
 
 // Offsets in the FP32 master vector for n_in input channels:
 // W0[n_in][27][C], b0[C], NHID x (W[27][C co][C ci], b[C]), WL[27][C], bL[1]
@@ -50,6 +53,7 @@ struct Layout {
 };
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 struct Geo {
@@ -91,7 +95,7 @@ __device__ __forceinline__ int neighbour(const Geo& g, Vox c, int t) {
 // ---------------------------------------------------------------- forward
 // x: [n_in][M], W: [n_in][27][C]
 __global__ void __launch_bounds__(256) conv_first(Geo g, int n_in, const float* x, const float* W, const float* b,
-                                                  __bf16* out) {
+                                                  __bf16* out, __bf16* out_cm) {
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= g.M) return;
     float acc[C];
@@ -114,13 +118,17 @@ __global__ void __launch_bounds__(256) conv_first(Geo g, int n_in, const float* 
 #pragma unroll
         for (int e = 0; e < 8; e++) h[e] = (__bf16)fmaxf(acc[q * 8 + e], 0.f);
         o[q] = h;
+        if (out_cm) {
+#pragma unroll
+            for (int e = 0; e < 8; e++) out_cm[(size_t)(q * 8 + e) * g.M + v] = h[e];
+        }
     }
 }
 
 // Y[v][co] = act(bias[co] + sum_t sum_ci W[t][co][ci] X[nbr(v,t)][ci]) (x [X_in[v][co] > 0] when
 // mask is given: the ReLU derivative of a backward pass). M % 32 == 0.
 __global__ void __launch_bounds__(256) conv_hidden(Geo g, const __bf16* X, const __bf16* W, const float* bias,
-                                                   const __bf16* mask, int relu, __bf16* Y) {
+                                                   const __bf16* mask, int relu, __bf16* Y, __bf16* Ycm) {
     const int lane = threadIdx.x & 63;
     const int v0 = (blockIdx.x * 4 + (threadIdx.x >> 6)) * 32;
     if (v0 >= g.M) return;
@@ -140,13 +148,24 @@ __global__ void __launch_bounds__(256) conv_hidden(Geo g, const __bf16* X, const
     }
     // D[i][j]: j = lane % 32 (output channel), i = 8 (r / 4) + 4 (lane / 32) + r % 4 (voxel)
     const float bj = bias ? bias[col] : 0.f;
+    __bf16 h[16];
 #pragma unroll
     for (int r = 0; r < 16; r++) {
         const int v = v0 + 8 * (r >> 2) + 4 * half + (r & 3);
         float val = acc[r] + bj;
         if (relu) val = fmaxf(val, 0.f);
         if (mask && !((float)mask[(size_t)v * C + col] > 0.f)) val = 0.f;
-        Y[(size_t)v * C + col] = (__bf16)val;
+        h[r] = (__bf16)val;
+        Y[(size_t)v * C + col] = h[r];
+    }
+    // channel-major copy: the 4 consecutive voxels r % 4 of a group in one 8-byte store
+    if (Ycm) {
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+            const bf16x4 pk = {h[4 * q], h[4 * q + 1], h[4 * q + 2], h[4 * q + 3]};
+            *reinterpret_cast<bf16x4*>(Ycm + (size_t)col * g.M + v0 + 8 * q + 4 * half) = pk;
+        }
     }
 }
 
@@ -192,7 +211,8 @@ __global__ void __launch_bounds__(256) sum_partials(const double* partials, int 
 
 // ---------------------------------------------------------------- backward
 // dX[u][c] = [X[u][c] > 0] sum_t W[t][c] dy[nbr(u, 26 - t)]  (last layer, 1 output channel)
-__global__ void __launch_bounds__(256) back_last(Geo g, const float* dy, const float* W, const __bf16* X, __bf16* dX) {
+__global__ void __launch_bounds__(256) back_last(Geo g, const float* dy, const float* W, const __bf16* X, __bf16* dX,
+                                                 __bf16* dXcm) {
     const int u = blockIdx.x * blockDim.x + threadIdx.x;
     if (u >= g.M) return;
     float acc[C] = {};
@@ -211,6 +231,10 @@ __global__ void __launch_bounds__(256) back_last(Geo g, const float* dy, const f
 #pragma unroll
         for (int e = 0; e < 8; e++) r[e] = (__bf16)((float)h[e] > 0.f ? acc[q * 8 + e] : 0.f);
         o[q] = r;
+        if (dXcm) {
+#pragma unroll
+            for (int e = 0; e < 8; e++) dXcm[(size_t)(q * 8 + e) * g.M + u] = r[e];
+        }
     }
 }
 
@@ -246,6 +270,65 @@ __global__ void __launch_bounds__(256) wgrad_hidden(Geo g, const __bf16* dZ, con
     __syncthreads();
     float* out = partial + ((size_t)chunk * TAPS + t) * C * C;
     for (int i = threadIdx.x; i < C * C; i += blockDim.x) out[i] = ((red[0][i] + red[1][i]) + red[2][i]) + red[3][i];
+}
+
+// The same gradient from channel-major copies dZc[C][M], Xc[C][M] (nz % 8 == 0). Block
+// (p, chunk) owns the taps t = 3 p + dz + 1 (dz = -1, 0, 1) of the line offset
+// p = 3 (dx + 1) + (dy + 1); its 4 wavefronts split the chunk's voxels. A wavefront step
+// takes 16 voxels: lane (c, h) holds dZc[c][v .. v + 7], v = v0 + 8 h (the A operand, one
+// z line), and builds the three B operands Xc[c][nbr(v .. v + 7)] from the aligned
+// 8-voxel vectors at z - 8, z and z + 8 of the neighbour line (periodic in z), shifted by
+// one element with v_alignbit for dz = -1 and +1: 4 vector loads for 3 MFMAs.
+// partial[chunk][t][co][ci].
+__global__ void __launch_bounds__(256) wgrad_hidden_cm(Geo g, const __bf16* dZc, const __bf16* Xc, float* partial) {
+    __shared__ float red[4][3][C * C];
+    const int p = blockIdx.x, chunk = blockIdx.y;
+    const int dx = p / 3 - 1, dy = p % 3 - 1;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int half = lane >> 5, col = lane & 31;
+    const int vbeg = chunk * HID_CHUNK + wave * (HID_CHUNK / 4);
+    const int vend = min(vbeg + HID_CHUNK / 4, g.M);
+    const __bf16* dz_c = dZc + (size_t)col * g.M;
+    const __bf16* x_c = Xc + (size_t)col * g.M;
+    f32x16 acc0 = {}, acc1 = {}, acc2 = {};
+    for (int v0 = vbeg; v0 < vend; v0 += 16) {
+        const int v = v0 + 8 * half;
+        u32x4 a = {0, 0, 0, 0}, lo = {0, 0, 0, 0}, cu = {0, 0, 0, 0}, hi = {0, 0, 0, 0};
+        if (v < vend) {
+            a = *reinterpret_cast<const u32x4*>(dz_c + v);
+            const int z = v % g.nz, r = v / g.nz;
+            int xx = r / g.ny + dx, yy = r % g.ny + dy;
+            xx = xx < 0 ? xx + g.nx : (xx >= g.nx ? xx - g.nx : xx);
+            yy = yy < 0 ? yy + g.ny : (yy >= g.ny ? yy - g.ny : yy);
+            const __bf16* line = x_c + (size_t)(xx * g.ny + yy) * g.nz;
+            lo = *reinterpret_cast<const u32x4*>(line + (z == 0 ? g.nz - 8 : z - 8));
+            cu = *reinterpret_cast<const u32x4*>(line + z);
+            hi = *reinterpret_cast<const u32x4*>(line + (z + 8 == g.nz ? 0 : z + 8));
+        }
+        // element e of a vector sits in bits 16 (e % 2) of dword e / 2
+        const u32x4 bm = {__builtin_amdgcn_alignbit(cu.x, lo.w, 16), __builtin_amdgcn_alignbit(cu.y, cu.x, 16),
+                          __builtin_amdgcn_alignbit(cu.z, cu.y, 16), __builtin_amdgcn_alignbit(cu.w, cu.z, 16)};
+        const u32x4 bp = {__builtin_amdgcn_alignbit(cu.y, cu.x, 16), __builtin_amdgcn_alignbit(cu.z, cu.y, 16),
+                          __builtin_amdgcn_alignbit(cu.w, cu.z, 16), __builtin_amdgcn_alignbit(hi.x, cu.w, 16)};
+        const bf16x8 av = __builtin_bit_cast(bf16x8, a);
+        acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, __builtin_bit_cast(bf16x8, bm), acc0, 0, 0, 0);
+        acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, __builtin_bit_cast(bf16x8, cu), acc1, 0, 0, 0);
+        acc2 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, __builtin_bit_cast(bf16x8, bp), acc2, 0, 0, 0);
+    }
+    // D[i = co][j = ci]: j = lane % 32, i = 8 (r / 4) + 4 (lane / 32) + r % 4
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        const int i = (8 * (r >> 2) + 4 * half + (r & 3)) * C + col;
+        red[wave][0][i] = acc0[r];
+        red[wave][1][i] = acc1[r];
+        red[wave][2][i] = acc2[r];
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < 3 * C * C; i += blockDim.x) {
+        const int k = i / (C * C), e = i % (C * C);
+        partial[((size_t)chunk * TAPS + 3 * p + k) * C * C + e] =
+            ((red[0][k][e] + red[1][k][e]) + red[2][k][e]) + red[3][k][e];
+    }
 }
 
 // First layer: G[t][c] = sum_v dZ[v][c] x[nbr(v,t)]; last layer: G[t][c] = sum_v dy[v] X[nbr(v,t)][c].
@@ -380,6 +463,13 @@ Cnn::Cnn(fts_context* ctx, int nx, int ny, int nz, int n_in, const double* in_sc
     acts_.allocate(sizeof(__bf16) * (size_t)(NHID + 1) * M_ * C);
     grad_a_.allocate(sizeof(__bf16) * (size_t)M_ * C);
     grad_b_.allocate(sizeof(__bf16) * (size_t)M_ * C);
+    cm_ = nz % 8 == 0;
+    n_hchunks_ = (M_ + HID_CHUNK - 1) / HID_CHUNK;
+    if (cm_) {
+        acts_cm_.allocate(sizeof(__bf16) * (size_t)NHID * M_ * C);
+        gcm_a_.allocate(sizeof(__bf16) * (size_t)M_ * C);
+        gcm_b_.allocate(sizeof(__bf16) * (size_t)M_ * C);
+    }
     partial_.allocate(sizeof(float) * (size_t)std::max(TAPS * n_chunks_ * C * C, n_chunks_ * TAPS * C));
     loss_partials_.allocate(sizeof(double) * 256);
     loss_.allocate(sizeof(double));
@@ -430,19 +520,22 @@ void Cnn::load_input(const double* w_minus, bool on_device) {
     FTS_HIP_CHECK(hipGetLastError());
 }
 
-void Cnn::forward() {
+void Cnn::forward(bool train) {
     hipStream_t st = ctx_->stream;
     const Geo g{nx_, ny_, nz_, M_};
     const float* p = params_.as<float>();
     const Layout L(n_in_);
     __bf16* a = acts_.as<__bf16>();
     const size_t AS = (size_t)M_ * C;
+    // channel-major copies of the hidden layers' inputs a[0..NHID-1] for the weight gradients
+    __bf16* acm = train && cm_ ? acts_cm_.as<__bf16>() : nullptr;
     hipLaunchKernelGGL(conv_first, dim3(blocks(M_)), dim3(256), 0, st, g, n_in_, x_.as<float>(), p + L.w0, p + L.b0,
-                       a);
+                       a, acm);
     for (int l = 0; l < NHID; l++)
         hipLaunchKernelGGL(conv_hidden, dim3(blocks(M_ / 32, 4)), dim3(256), 0, st, g, a + l * AS,
                            wf_.as<__bf16>() + (size_t)l * TAPS * C * C, p + L.wh + l * HSZ + TAPS * C * C,
-                           (const __bf16*)nullptr, 1, a + (l + 1) * AS);
+                           (const __bf16*)nullptr, 1, a + (l + 1) * AS,
+                           acm && l + 1 < NHID ? acm + (l + 1) * AS : (__bf16*)nullptr);
     hipLaunchKernelGGL(conv_last, dim3(blocks(M_)), dim3(256), 0, st, g, a + NHID * AS, p + L.wl, p + L.bl,
                        y_.as<float>());
     FTS_HIP_CHECK(hipGetLastError());
@@ -474,7 +567,7 @@ double Cnn::gradients(const double* w_minus, const double* w_plus_target, bool o
         tsrc = stage_out_.as<double>();
     }
     hipLaunchKernelGGL(to_float, dim3(blocks(M_)), dim3(256), 0, st, tsrc, 1.0 / out_scale_, M_, target_.as<float>());
-    forward();
+    forward(true);
     const int lb = std::min(256, blocks(M_));
     hipLaunchKernelGGL(loss_grad, dim3(lb), dim3(256), 0, st, g, y_.as<float>(), target_.as<float>(), dy_.as<float>(),
                        loss_partials_.as<double>());
@@ -492,21 +585,33 @@ double Cnn::gradients(const double* w_minus, const double* w_plus_target, bool o
                        gr + L.wl);
     hipLaunchKernelGGL(bias_grad, dim3(n_chunks_), dim3(256), 0, st, g, (const __bf16*)nullptr, dy_.as<float>(), part);
     hipLaunchKernelGGL(reduce_chunks, dim3(1), dim3(256), 0, st, part, n_chunks_, C, 1, gr + L.bl);
+    __bf16* dz_cm = cm_ ? gcm_a_.as<__bf16>() : nullptr;
+    __bf16* other_cm = cm_ ? gcm_b_.as<__bf16>() : nullptr;
     hipLaunchKernelGGL(back_last, dim3(blocks(M_)), dim3(256), 0, st, g, dy_.as<float>(), p + L.wl, a + NHID * AS,
-                       grad_a_.as<__bf16>());
+                       grad_a_.as<__bf16>(), dz_cm);
     __bf16* dz = grad_a_.as<__bf16>();
     __bf16* other = grad_b_.as<__bf16>();
     for (int l = NHID - 1; l >= 0; l--) {
         // layer l maps a[l] -> a[l + 1]; dz = dL/d(pre-activation of a[l + 1])
-        hipLaunchKernelGGL(wgrad_hidden, dim3(n_chunks_, TAPS), dim3(256), 0, st, g, dz, a + l * AS, part);
-        hipLaunchKernelGGL(reduce_chunks, dim3(blocks(TAPS * C * C)), dim3(256), 0, st, part, n_chunks_,
-                           TAPS * C * C, TAPS * C * C, gr + L.wh + l * HSZ);
+        if (cm_) {
+            hipLaunchKernelGGL(wgrad_hidden_cm, dim3(TAPS / 3, n_hchunks_), dim3(256), 0, st, g, dz_cm,
+                               acts_cm_.as<const __bf16>() + l * AS, part);
+            hipLaunchKernelGGL(reduce_chunks, dim3(blocks(TAPS * C * C)), dim3(256), 0, st, part, n_hchunks_,
+                               TAPS * C * C, TAPS * C * C, gr + L.wh + l * HSZ);
+        } else {
+            hipLaunchKernelGGL(wgrad_hidden, dim3(n_chunks_, TAPS), dim3(256), 0, st, g, dz, a + l * AS, part);
+            hipLaunchKernelGGL(reduce_chunks, dim3(blocks(TAPS * C * C)), dim3(256), 0, st, part, n_chunks_,
+                               TAPS * C * C, TAPS * C * C, gr + L.wh + l * HSZ);
+        }
         hipLaunchKernelGGL(bias_grad, dim3(n_chunks_), dim3(256), 0, st, g, dz, (const float*)nullptr, part);
         hipLaunchKernelGGL(reduce_chunks, dim3(1), dim3(256), 0, st, part, n_chunks_, C, C,
                            gr + L.wh + l * HSZ + TAPS * C * C);
+        // the next lower layer's dz, with its channel-major copy unless it is the first layer's
         hipLaunchKernelGGL(conv_hidden, dim3(blocks(M_ / 32, 4)), dim3(256), 0, st, g, dz,
-                           wb_.as<__bf16>() + (size_t)l * TAPS * C * C, (const float*)nullptr, a + l * AS, 0, other);
+                           wb_.as<__bf16>() + (size_t)l * TAPS * C * C, (const float*)nullptr, a + l * AS, 0, other,
+                           l > 0 ? other_cm : (__bf16*)nullptr);
         std::swap(dz, other);
+        std::swap(dz_cm, other_cm);
     }
     // first layer
     for (int c = 0; c < n_in_; c++) {

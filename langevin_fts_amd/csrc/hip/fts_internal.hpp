@@ -409,10 +409,16 @@ public:
 
 private:
     void load_input(const double* w_minus, bool on_device);
-    void forward();
+    // train: also write the channel-major copies of a[0..NHID-1] the weight gradients read
+    void forward(bool train = false);
     void refresh_weights();
     fts_context* ctx_;
     int nx_, ny_, nz_, n_in_, n_params_, M_, n_chunks_;
+    // hidden weight gradients from channel-major operands ([C][M] copies of the layer
+    // inputs and of dZ) when nz % 8 == 0, over chunks of n_hchunks_
+    bool cm_ = false;
+    int n_hchunks_ = 0;
+    DeviceBuffer acts_cm_, gcm_a_, gcm_b_;
     std::vector<double> in_scales_;
     double out_scale_;
     long long step_ = 0;

@@ -431,7 +431,7 @@ class LFTS:
                 "the learned guess maps one real field to one imaginary field"
             assert cb.get_dim() == 3, "the learned guess needs a 3D grid"
             self.learned.setdefault("train_steps", 1000)
-            self.learned.setdefault("lr", 1e-3)
+            self.learned.setdefault("lr", 5e-4)
             self.learned.setdefault("lr_final", self.learned["lr"] / 10)
             self.learned.setdefault("train_iters", 16)
             self.learned.setdefault("replay", 64)
@@ -498,6 +498,7 @@ class LFTS:
         self.guess = lf.LearnedGuess(self.cb.get_nx(), in_scales, out_scale)
         self.guess.set_params(guess_initial_params(self.learned["seed"], len(in_scales)))
         self.guess_trained_steps = 0
+        self.guess_losses = []   # the newest sample's loss at each training step
         return in_scales, out_scale
 
     def _guess_before_update(self, ready):
@@ -525,6 +526,7 @@ class LFTS:
         lr = lr1 + 0.5 * (lr0 - lr1) * (1.0 + np.cos(np.pi * frac))
         loss = self.saddle_device.train_guess(self.guess, lr, self.learned["train_iters"],
                                               self.learned["replay"])
+        self.guess_losses.append(loss)
         self.guess_trained_steps += 1
         return loss
 

@@ -72,7 +72,10 @@ def test_three_input_forward_and_scales(lf):
     assert rel_l2(net.predict(x.ravel()), 0.25 * y_ref.ravel()) < 3e-3
 
 
-@pytest.mark.parametrize("nx,n_in", [([16, 16, 16], 1), ([8, 12, 16], 1), ([8, 12, 16], 3)])
+# nz % 8 == 0: hidden weight gradients from channel-major copies (32^3: 8 voxel chunks);
+# nz = 12: the channels-last kernel
+@pytest.mark.parametrize("nx,n_in", [([16, 16, 16], 1), ([8, 12, 16], 1), ([8, 12, 16], 3), ([32, 32, 32], 1),
+                                     ([8, 8, 12], 1)])
 def test_gradients_match_oracle(lf, nx, n_in):
     say(f"cnn gradients {nx} inputs {n_in}")
     p = params(5, n_in)

@@ -117,7 +117,7 @@ def test_guess_learning_rate_falls_on_a_cosine():
             return 0.0
 
     T = 100
-    sim = types.SimpleNamespace(guess=object(), guess_trained_steps=0, saddle_device=Device(),
+    sim = types.SimpleNamespace(guess=object(), guess_trained_steps=0, guess_losses=[], saddle_device=Device(),
                                 learned={"lr": 1e-3, "lr_final": 1e-4, "train_steps": T, "train_iters": 16,
                                          "replay": 64})
     for _ in range(T + 3):
