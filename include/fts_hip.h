@@ -97,9 +97,12 @@ int fts_grid_fft_backward_noalloc(fts_grid* g, int precision, const void* d_in, 
  * reference, whose FFTs are library calls (MklFFT3D.cpp:11-79, CudaFFT). Per
  * precision, the grid picks the slab-pass workgroup size (slab_threads) and the kz
  * columns per wavefront of the pencil pass (pencil_columns) by timing the candidates
- * when it is created (environment FTS_FFT_TUNE=0: the defaults); results of the
- * candidates agree to rounding (bit-identical at 32^3 and 64^3), and one process
- * keeps one choice per (N, precision). slab_threads is the half step's (discrete
+ * when it is created. Results of the candidates are bit-identical at 32^3 and 64^3,
+ * where tuning is on by default; at 48^3, 96^3 and 128^3 they agree to rounding only,
+ * so the defaults stay there unless the environment sets FTS_FFT_TUNE=1 (a timed
+ * choice could differ between processes and with it a chaotic run's trajectory).
+ * FTS_FFT_TUNE=0 keeps the defaults everywhere. One process keeps one choice per
+ * (N, precision). slab_threads is the half step's (discrete
  * chains); the forward transform with two spectra out and the Richardson step of
  * continuous chains are timed alone and may take another candidate, which the note
  * names. get writes the choice and a note with the measured candidates

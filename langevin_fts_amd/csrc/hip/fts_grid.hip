@@ -578,9 +578,18 @@ void Grid::tune_fft_schedule() {
     slab_bd[p] = wslab_default_bd(ct_n);
     for (int m = 0; m < 3; m++) slab_bd_mode[p][m] = 0;
     pencil_nl[p] = WPENCIL_DEFAULT_NL;
+    // Tuning picks launch shapes by wall-clock timing, so the choice can differ between
+    // processes. At 32^3 and 64^3 every candidate gives bit-identical results; at 48^3,
+    // 96^3 and 128^3 they differ by rounding, and a run could then follow another
+    // trajectory in another process: there the defaults stay unless FTS_FFT_TUNE=1.
     const char* env = std::getenv("FTS_FFT_TUNE");
+    const bool identical = ct_n == 32 || ct_n == 64;
     if (!wave_local || (env && env[0] == '0')) {
         fft_schedule[p] = "defaults";
+        return;
+    }
+    if (!identical && !(env && env[0] == '1')) {
+        fft_schedule[p] = "defaults (candidates at this size differ by rounding; FTS_FFT_TUNE=1 tunes)";
         return;
     }
     std::lock_guard<std::mutex> lock(tune_mutex());

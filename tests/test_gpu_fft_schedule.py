@@ -1,8 +1,10 @@
 """FFT schedule autotuner of the N^3 wave-local kernels (fts_grid_get_fft_schedule /
 fts_grid_set_fft_schedule): every launch-shape candidate gives the same propagators,
 Q and concentrations to rounding (relative 1e-12 in FP64, 1e-5 in FP32), bit for bit
-at 32^3 and 64^3; the tuned choice is one of the candidates, and results through a
-non-default schedule still meet the oracle's 1e-10 bar."""
+at 32^3 and 64^3 (where tuning is on by default; at the other sizes the defaults stay
+unless FTS_FFT_TUNE=1), for discrete and continuous chains; the chosen shape is one of
+the candidates, and results through a non-default schedule still meet the oracle's
+1e-10 bar."""
 import numpy as np
 import pytest
 
@@ -41,17 +43,19 @@ def evaluate(capi, g, precision, name, w):
             s.propagator(key, n))
 
 
+# the discrete step (slab half-step mode) and the continuous chain's forward transform with
+# two spectra and Richardson step, the modes the autotuner times on their own
+@pytest.mark.parametrize("name", ["diblock_f0.5_ds1_90|discrete|0", "diblock_f0.5_ds1_16|continuous|0"])
 @pytest.mark.parametrize("n", [32, 48, 64, 96, 128])
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
-def test_every_schedule_gives_identical_results(capi, ctx, n, prec):
+def test_every_schedule_gives_identical_results(capi, ctx, n, prec, name):
     precision = capi.FTS_FP64 if prec == "fp64" else capi.FTS_FP32
-    name = "diblock_f0.5_ds1_90|discrete|0"
     M = n ** 3
     rng = np.random.default_rng(n)
     w = {"A": rng.standard_normal(M), "B": rng.standard_normal(M)}
     g = capi.Grid(ctx, [n] * 3, [6.0] * 3)
     bd, nl, note = g.fft_schedule(precision)
-    say(f"{n}^3 {prec}: tuned slab {bd} threads, pencil {nl} columns; {note}")
+    say(f"{n}^3 {prec} {name}: slab {bd} threads, pencil {nl} columns; {note}")
     assert bd in SLAB[n] and nl in PENCIL
     ref = evaluate(capi, g, precision, name, w)
     tol = 1e-12 if prec == "fp64" else 1e-5

@@ -193,4 +193,5 @@ def test_c2_scft_at_config_shape(capsys, model, plan_key):
     say(f"  converged in {n_iter} iterations, box {lx_fin}, oracle error of the result {e_fin:.2e}, "
         f"stress {stress_fin}")
     assert n_iter < 3000
-    assert e_fin < 1e-8
+    # the driver stopped below 1e-8; the oracle's error of the same fields differs by rounding
+    assert e_fin < 1e-8 * (1 + 1e-6)
