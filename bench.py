@@ -389,6 +389,7 @@ def replica_bench(args, w, lf, lfts):
     params["saddle"]["mixed_precision"] = mixed
     with stdout_to_stderr():
         sim = lfts.LFTS(params, random_seed=args.seed, n_replicas=R)
+        sim.saddle_device.set_graphs(not args.no_graphs)
         rng = np.random.Generator(np.random.PCG64(args.seed))
         w_aux = np.stack([sim.mpt.to_aux_fields(rng.standard_normal((2, M))) for _ in range(R)])
         dev = sim.saddle_device
@@ -397,8 +398,8 @@ def replica_bench(args, w, lf, lfts):
         dev.find_saddle_point(sim.saddle["tolerance"], sim.saddle["max_iter"], 0)
         for _ in range(args.warmup):
             sim.replica_step()
-        if not args.no_kernel_timing:
-            lf.set_kernel_timing(SAMPLE)
+        # the timed steps run without sampled launch events, so the saddle iterations and
+        # Langevin updates replay their captured graphs; a roofline pass of its own follows
         its, fails = [], 0
         dev.field_moments()
         t0 = time.perf_counter()
@@ -410,6 +411,9 @@ def replica_bench(args, w, lf, lfts):
         elapsed = time.perf_counter() - t0
         kern = {}
         if not args.no_kernel_timing:
+            lf.set_kernel_timing(SAMPLE)
+            for _ in range(max(2, args.steps // 4)):
+                sim.replica_step()
             kern = {f: lf.get_kernel_timing(f) for f in ("slab", "pencil")}
             lf.set_kernel_timing(0)
     out = {
@@ -421,7 +425,10 @@ def replica_bench(args, w, lf, lfts):
         "data": "synthetic: w_A, w_B ~ N(0,1) from PCG64(seed); replica r's noise from PCG64(seed + r)",
         "config": {"workload": describe(args.workload, w), "replicas": R,
                    "saddle_iterations_per_step": float(np.mean(its)), "saddle_failures": fails,
-                   "max_saddle_iterations": int(np.max(its)), "fft_schedule": fft_schedule_info(w["nx"][0])},
+                   "max_saddle_iterations": int(np.max(its)), "fft_schedule": fft_schedule_info(w["nx"][0]),
+                   "iteration_graphs": ("off: launches issued one by one" if args.no_graphs else
+                                        "saddle iterations and Langevin updates replayed from captured HIP graphs "
+                                        "(the kernel-timed roofline pass runs its launches one by one)")},
     }
     if kern:
         out["roofline"] = kernel_roofline(
@@ -788,6 +795,8 @@ def main():
     ap.add_argument("--guess-inputs", default="residual", choices=["residual", "delta", "absolute"],
                     help="residual: the CNN sees w-, the first residual of the search and the starting w+ and "
                          "predicts the correction of w+; delta: w+ += f(w-_new) - f(w-_old)")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="replicas48: issue the saddle iterations' launches one by one instead of replaying graphs")
     ap.add_argument("--sk-steps", type=int, default=20,
                     help="steps of the untimed structure-function agreement pass (0: skip)")
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"],

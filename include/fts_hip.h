@@ -341,6 +341,17 @@ int fts_cnn_train_step(fts_cnn* c, const double* w_minus, const double* w_plus_t
  * returned fields, concentrations and Hamiltonian meet the tolerance in the main
  * solver's precision. lo = NULL switches it off. lo must outlive its use here. */
 int fts_saddle_set_low_precision(fts_saddle* s, fts_solver* lo, double switch_error);
+/* Replica saddles (no counterpart in the reference, whose replicas are separate runs):
+ * with on != 0 (the default) every saddle iteration of fts_saddle_find and every
+ * Langevin update replays captured HIP graphs. An iteration is two graphs: the
+ * evaluation with the residual statistics and a planning kernel that makes every
+ * replica's convergence and mixing decisions on the device, then the batched
+ * Anderson step. The host reads the errors back once per iteration. Results equal
+ * those of on = 0, which issues the same launches one by one. Graphs are recaptured
+ * when the solver, its box, FFT launch shapes or Boltzmann tables, the tolerance or
+ * the time step change; with kernel timing on, launches run one by one. No effect on
+ * single-replica saddles. */
+int fts_saddle_set_graphs(fts_saddle* s, int on);
 
 /* Field theories with one real and one imaginary field (AB melts), on the device.
  * With a one-input network f(w-):

@@ -66,6 +66,7 @@ def lib():
             "fts_grid_fft_forward": (i, [vp, i, vp, vp, i]),
             "fts_grid_fft_backward": (i, [vp, i, vp, vp, i]),
             "fts_grid_fft_backward_noalloc": (i, [vp, i, vp, vp, i]),
+            "fts_saddle_set_graphs": (i, [vp, i]),
             "fts_grid_get_fft_schedule": (i, [vp, i, _IP, _IP, ctypes.c_char_p, i]),
             "fts_grid_set_fft_schedule": (i, [vp, i, i, i]),
             "fts_solver_create": (i, [vp, ctypes.POINTER(FtsPlan), i, VPP]),

@@ -385,12 +385,17 @@ void AndersonMixingDevice::calculate_batch(fts_context* ctx, const std::vector<A
     if (!any) return;
     FTS_HIP_CHECK(hipMemcpyAsync(ws.table.get(), items, sizeof(AmBatchItem) * n, hipMemcpyHostToDevice, st));
     FTS_HIP_CHECK(hipEventRecord(ws.copied, st));
-    const AmBatchItem* dev = ws.table.as<const AmBatchItem>();
+    launch_batch(st, ws.table.as<const AmBatchItem>(), n, n_var, max_na, max_solve, L, ws.partials.as<double>());
+}
+
+void AndersonMixingDevice::launch_batch(hipStream_t st, const AmBatchItem* dev, int n, int n_var, int max_na,
+                                        int max_solve, int L, double* partials) {
+    FTS_REQUIRE(max_na < AM_MAX_HIST && max_solve < AM_MAX_HIST, "history beyond the batched kernels' table");
     if (max_na >= 0) {
         hipLaunchKernelGGL(am_dot_batch_stage1, dim3(RED_BLOCKS, max_na + 1, n), dim3(RED_THREADS), 0, st, dev,
-                           n_var, ws.partials.as<double>());
+                           n_var, partials);
         hipLaunchKernelGGL(am_dot_batch_stage2, dim3(max_na + 1, n), dim3(RED_BLOCKS), 0, st, dev,
-                           ws.partials.as<const double>());
+                           (const double*)partials);
         FTS_HIP_CHECK(hipGetLastError());
     }
     if (max_solve > 0) {

@@ -810,6 +810,13 @@ int fts_cnn_train_step(fts_cnn* c, const double* w_minus, const double* w_plus_t
     });
 }
 
+int fts_saddle_set_graphs(fts_saddle* s, int on) {
+    return guarded([&] {
+        need(s, "saddle");
+        s->s->set_graphs(on != 0);
+    });
+}
+
 int fts_saddle_set_low_precision(fts_saddle* s, fts_solver* lo, double switch_error) {
     return guarded([&] {
         need(s, "saddle");

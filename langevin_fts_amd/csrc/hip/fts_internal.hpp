@@ -206,6 +206,13 @@ public:
     // replay compute_propagators + compute_concentrations from a captured HIP graph when
     // the inputs are device fields at the same addresses as the captured call
     virtual void set_graphs(bool on) = 0;
+    virtual bool graphs_on() const = 0;
+    // a caller replayed this solver's launches from a graph of its own: host-side caches
+    // of the results (the partition functions) are stale
+    virtual void results_changed() = 0;
+    // what the captured launches depend on besides buffer addresses: box volume, FFT
+    // launch shapes, version of the Boltzmann tables
+    virtual std::vector<double> launch_signature() const = 0;
 };
 
 // bc: NULL for pseudo-spectral steps, else 2 dim FTS_BC_* codes (real-space steps, FP64 continuous)
@@ -274,6 +281,22 @@ public:
                                 const std::vector<double>& old_err, const std::vector<double>& err,
                                 const std::vector<char>& staged, AmBatchWorkspace& ws);
 
+    // the device work of a batched step from a device table of n items: dot products for
+    // n = 0..max_na (items with a smaller na skip the rest), Gram solves of up to
+    // max_solve coefficients, mixes; partials: n x AM_MAX_HIST x RED_BLOCKS doubles
+    static void launch_batch(hipStream_t st, const AmBatchItem* dev, int n, int n_var, int max_na, int max_solve,
+                             int L, double* partials);
+    // what a device-resident planner of the same steps needs
+    int n_var() const { return n_var_; }
+    int max_hist() const { return max_hist_; }
+    double start_error() const { return start_error_; }
+    double mix_min() const { return mix_min_; }
+    double mix_init() const { return mix_init_; }
+    double* w_hist() const { return w_hist_.as<double>(); }
+    double* d_hist() const { return d_hist_.as<double>(); }
+    double* dots_ring() const { return dots_ring_.as<double>(); }
+    double* coef() const { return coef_.as<double>(); }
+
 private:
     // the host part of a step: mixing decisions, ring insertion and the rows the
     // device work reads (w, d: the inputs as they sit on the device)
@@ -295,6 +318,13 @@ public:
     // one Anderson mixing per replica; n_blocks / n_solvents: per replica (replicated plans)
     SaddlePoint(fts_context* ctx, SolverBase* solver, std::vector<AndersonMixingDevice*> ams, int M,
                 const fts_field_theory& ft, int n_blocks = 0, int n_solvents = 0);
+    ~SaddlePoint();
+    SaddlePoint(const SaddlePoint&) = delete;
+    SaddlePoint& operator=(const SaddlePoint&) = delete;
+    // replicas: run every saddle iteration and the Langevin update from captured HIP graphs
+    // (the mixing decisions made on the device); off: the same launches issued one by one
+    void set_graphs(bool on) { rep_graphs_ = on; }
+    bool graphs() const { return rep_graphs_; }
     void set_fields(const double* w, bool on_device);
     void get_fields(double* w, bool on_device);
     void get_concentration(int type, double* out, bool on_device);
@@ -339,6 +369,36 @@ private:
     void hamiltonians(const std::vector<int>& reps, SolverBase* solver, std::vector<double>& ham);
     void mix_replicas(const std::vector<char>& mix, const std::vector<double>& old_err,
                       const std::vector<double>& err, bool contiguous);
+    // graph form of the replica iteration: A = evaluation, statistics and the device-side
+    // mixing plan (one graph per precision), B = gather and batched Anderson step
+    bool use_replica_graphs() const;
+    void setup_replica_graphs(bool contiguous);
+    void find_replicas_graph(double tolerance, int max_iter, bool contiguous, std::vector<int>& done,
+                             std::vector<double>& err, std::vector<double>& ham);
+    void launch_graph_a(SolverBase* solver, bool low, double tolerance, bool contiguous);
+    void launch_graph_b(bool contiguous);
+    void issue_graph_a(SolverBase* solver, bool low, double tolerance, bool contiguous);
+    void issue_graph_b(bool contiguous);
+    void issue_langevin(double dt);
+    template <typename F>
+    void capture(hipGraphExec_t& exec, F&& issue);
+    bool rep_graphs_ = true;
+    struct RepGraph {
+        hipGraphExec_t exec = nullptr;
+        const SolverBase* solver = nullptr;
+        std::vector<double> sig;
+        double tol = 0.0;
+    };
+    RepGraph rep_a_[2];                 // [0] main solver, [1] low-precision solver
+    hipGraphExec_t rep_b_ = nullptr;
+    struct LmGraph {                    // Langevin update, one per order of the two noise buffers
+        hipGraphExec_t exec = nullptr;
+        const void* prev = nullptr;
+        double dt = 0.0;
+    };
+    LmGraph rep_lm_[2];
+    // device-side mixing state, constants, step table and status (errors, done iterations)
+    DeviceBuffer rep_state_, rep_const_, rep_items_, rep_status_, rep_status_host_;
 
     fts_context* ctx_;
     SolverBase* solver_;

@@ -157,6 +157,126 @@ __global__ void scatter_batch_kernel(GatherShared g, const GatherRep* reps, int 
         for (int c = 0; c < g.n; c++) g.wim[c][q * g.wim_step + i] = g.am_w[q * g.am_step + (size_t)c * M + i];
 }
 
+// ---- device-side mixing plan of the replica graphs: the host decisions of the
+// unbatched loop (error level, convergence, AndersonMixingDevice::plan) per replica
+struct RepAmState {
+    int n_anderson, ring_start, ring_n, dots_start, dots_n;
+    int done;   // iteration at which the replica converged, 0: not yet
+    int it;     // iterations of this search
+    int pad;
+    double mix, err;
+};
+
+struct RepAmConst {   // the replica's mixing buffers
+    double* w_hist;
+    double* d_hist;
+    double* dots;
+    double* coef;
+};
+
+struct RepPlanArgs {
+    int NR, I, M, L, max_hist, contiguous, low, nrows;
+    double tol, start_error, mix_min, mix_init;
+    double* am_w;   // non-staged inputs of replica 0; replica r at + r am_step
+    double* am_d;
+    size_t am_step;
+    double* w_im;   // contiguous imaginary rows of replica 0; + r w_step
+    size_t w_step;
+};
+
+__global__ void replica_reset_kernel(RepAmState* st, int NR, double mix_init) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= NR) return;
+    RepAmState s{};
+    s.n_anderson = -1;
+    s.mix = mix_init;
+    s.err = 1e20;
+    st[r] = s;
+}
+
+// one thread per replica; stats: sums then centred sums of squares of the derivative rows
+__global__ void replica_plan_kernel(RepPlanArgs a, const double* stats, RepAmState* st, const RepAmConst* cs,
+                                    AmBatchItem* items, GatherRep* gather, double* status) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= a.NR) return;
+    // the error level of lfts.py:829-831: the largest deviation of the imaginary derivatives
+    double e = 0.0;
+    for (int c = 0; c < a.I; c++) {
+        const double ec = sqrt(stats[a.nrows + r * a.I + c] / a.M);
+        if (c == 0 || ec > e || isnan(ec)) e = ec;
+    }
+    RepAmState s = st[r];
+    AmBatchItem& it = items[r];
+    it.mode = 0;
+    it.na = -1;
+    it.L = a.L;
+    GatherRep g{};
+    if (!s.done) {
+        s.it += 1;
+        const double old_err = s.err;
+        s.err = e;
+        if (!a.low && e < a.tol) {
+            s.done = s.it;
+        } else {
+            const RepAmConst k = cs[r];
+            const int L = a.L;
+            const size_t nv = (size_t)a.I * a.M;
+            // the gather writes a kept input straight into the next history slot
+            const bool staged = e < a.start_error || s.n_anderson >= 0;
+            const int slot = (s.ring_start + s.ring_n) % L;
+            double* am_w = a.am_w + r * a.am_step;
+            double* am_d = a.am_d + r * a.am_step;
+            double* in_w = staged ? k.w_hist + slot * nv : am_w;
+            double* in_d = staged ? k.d_hist + slot * nv : am_d;
+            double* w_im = a.contiguous ? a.w_im + r * a.w_step : am_w;
+            for (int c = 0; c < a.I; c++) {
+                g.w_out[c] = staged || !a.contiguous ? in_w + (size_t)c * a.M : nullptr;
+                g.d_out[c] = in_d + (size_t)c * a.M;
+            }
+            g.scatter = a.contiguous ? 0 : 1;
+            it.w = staged ? in_w : w_im;
+            it.d = in_d;
+            it.out = w_im;
+            it.dots = k.dots;
+            it.coef = k.coef;
+            it.dots_out = nullptr;
+            // AndersonMixingDevice::plan (CpuAndersonMixing.cpp:86-160)
+            if (e < a.start_error || s.n_anderson >= 0) s.n_anderson = s.n_anderson + 1;
+            if (s.n_anderson >= 0) {
+                s.n_anderson = min(a.max_hist, s.n_anderson);
+                if (s.ring_n == L) s.ring_start = (s.ring_start + 1) % L;
+                s.ring_n = min(s.ring_n + 1, L);
+                it.na = s.n_anderson;
+                for (int n = 0; n <= it.na; n++) {
+                    const int row = (s.ring_start + s.ring_n - n - 1 + L) % L;
+                    it.wh[n] = k.w_hist + row * nv;
+                    it.dh[n] = k.d_hist + row * nv;
+                }
+                it.dots_out = k.dots + (size_t)((s.dots_start + s.dots_n) % L) * L;
+                if (s.dots_n == L) s.dots_start = (s.dots_start + 1) % L;
+                s.dots_n = min(s.dots_n + 1, L);
+            }
+            if (s.n_anderson <= 0) {
+                if (old_err < e) {
+                    const double m7 = s.mix * 0.7;
+                    s.mix = m7 < a.mix_min ? a.mix_min : m7;
+                } else {
+                    s.mix = s.mix * 1.01;
+                }
+                it.mode = 1;
+                it.mix = s.mix;
+            } else {
+                it.mode = 2;
+                for (int n = 0; n <= s.n_anderson; n++) it.drows[n] = (s.dots_start + s.dots_n - n - 1 + L) % L;
+            }
+        }
+    }
+    gather[r] = g;
+    st[r] = s;
+    status[r] = s.err;
+    status[a.NR + r] = (double)s.done;
+}
+
 struct Scatter {
     int n;
     const double* src[FTS_MAX_TYPES];
@@ -523,6 +643,210 @@ void SaddlePoint::mix_replicas(const std::vector<char>& mix, const std::vector<d
     }
 }
 
+SaddlePoint::~SaddlePoint() {
+    for (auto& g : rep_a_)
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    if (rep_b_) (void)hipGraphExecDestroy(rep_b_);
+    for (auto& g : rep_lm_)
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+}
+
+template <typename F>
+void SaddlePoint::capture(hipGraphExec_t& exec, F&& issue) {
+    hipStream_t st = ctx_->stream;
+    if (exec) FTS_HIP_CHECK(hipGraphExecDestroy(exec));
+    exec = nullptr;
+    FTS_HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    hipGraph_t graph = nullptr;
+    try {
+        issue();
+    } catch (...) {
+        (void)hipStreamEndCapture(st, &graph);
+        if (graph) (void)hipGraphDestroy(graph);
+        throw;
+    }
+    FTS_HIP_CHECK(hipStreamEndCapture(st, &graph));
+    const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    FTS_HIP_CHECK(e);
+}
+
+// graphs hold no sampled launch events (kernel timing on: the launches run one by one)
+bool SaddlePoint::use_replica_graphs() const {
+    if (NR_ < 2 || !rep_graphs_ || ctx_->timer.active()) return false;
+    for (auto* a : ams_)
+        if (a->reduce_memory() || a->max_hist() >= AM_MAX_HIST) return false;
+    return true;
+}
+
+void SaddlePoint::setup_replica_graphs(bool contiguous) {
+    (void)contiguous;
+    if (rep_state_.bytes()) return;
+    const AndersonMixingDevice& a0 = *ams_[0];
+    for (auto* a : ams_)
+        FTS_REQUIRE(a->n_var() == a0.n_var() && a->max_hist() == a0.max_hist() &&
+                        a->start_error() == a0.start_error() && a->mix_min() == a0.mix_min() &&
+                        a->mix_init() == a0.mix_init(),
+                    "the replicas' mixings differ in their parameters");
+    rep_state_.allocate(sizeof(RepAmState) * NR_);
+    rep_const_.allocate(sizeof(RepAmConst) * NR_);
+    rep_items_.allocate(sizeof(AmBatchItem) * NR_);
+    rep_status_.allocate(sizeof(double) * 2 * NR_);
+    rep_status_host_.allocate_host_pinned(sizeof(double) * 2 * NR_);
+    std::vector<RepAmConst> k(NR_);
+    for (int r = 0; r < NR_; r++) k[r] = RepAmConst{ams_[r]->w_hist(), ams_[r]->d_hist(), ams_[r]->dots_ring(),
+                                                    ams_[r]->coef()};
+    FTS_HIP_CHECK(hipMemcpyAsync(rep_const_.get(), k.data(), sizeof(RepAmConst) * NR_, hipMemcpyHostToDevice,
+                                 ctx_->stream));
+    FTS_HIP_CHECK(hipStreamSynchronize(ctx_->stream));
+    am_ws_.reserve(NR_);
+}
+
+// A: the evaluation, dH/dw of the imaginary fields and their statistics, then the
+// device-side plan: errors, convergence and every replica's mixing decisions
+void SaddlePoint::issue_graph_a(SolverBase* solver, bool low, double tolerance, bool contiguous) {
+    concentrations(solver);
+    run_lincomb(0, h_cw_, h_cp_, h_c0_, I_, h_.as<double>(), NR_, (size_t)I_ * M_);
+    const int nrows = NR_ * I_;
+    double* sd = stats_.as<double>();
+    row_sums_strided(h_.as<double>(), nrows, M_, nullptr, false, sd);
+    row_sums_strided(h_.as<double>(), nrows, M_, sd, true, sd + nrows);
+    const AndersonMixingDevice& a0 = *ams_[0];
+    RepPlanArgs a{};
+    a.NR = NR_;
+    a.I = I_;
+    a.M = M_;
+    a.L = a0.max_hist() + 1;
+    a.max_hist = a0.max_hist();
+    a.contiguous = contiguous ? 1 : 0;
+    a.low = low ? 1 : 0;
+    a.nrows = nrows;
+    a.tol = tolerance;
+    a.start_error = a0.start_error();
+    a.mix_min = a0.mix_min();
+    a.mix_init = a0.mix_init();
+    a.am_w = am_w_.as<double>();
+    a.am_d = am_d_.as<double>();
+    a.am_step = (size_t)I_ * M_;
+    a.w_im = wa(0) + (size_t)imag_idx_[0] * M_;
+    a.w_step = (size_t)S_ * M_;
+    hipLaunchKernelGGL(replica_plan_kernel, dim3(ceil_div(NR_, 64)), dim3(64), 0, ctx_->stream, a,
+                       stats_.as<const double>(), rep_state_.as<RepAmState>(), rep_const_.as<const RepAmConst>(),
+                       rep_items_.as<AmBatchItem>(), gather_dev_.as<GatherRep>(), rep_status_.as<double>());
+    FTS_HIP_CHECK(hipGetLastError());
+}
+
+// B: the gather into the planned history slots and one batched Anderson step
+void SaddlePoint::issue_graph_b(bool contiguous) {
+    hipStream_t st = ctx_->stream;
+    GatherShared g{};
+    g.n = I_;
+    for (int c = 0; c < I_; c++) {
+        g.wim[c] = wa(0) + (size_t)imag_idx_[c] * M_;
+        g.h[c] = h_.as<const double>() + (size_t)c * M_;
+        g.scale[c] = dt_scaling_[imag_idx_[c]];
+    }
+    g.wim_step = (size_t)S_ * M_;
+    g.h_step = (size_t)I_ * M_;
+    g.am_w = am_w_.as<const double>();
+    g.am_step = (size_t)I_ * M_;
+    hipLaunchKernelGGL(gather_batch_kernel, dim3(grid_blocks(M_), NR_), dim3(256), 0, st, g,
+                       gather_dev_.as<const GatherRep>(), M_);
+    FTS_HIP_CHECK(hipGetLastError());
+    const int mh = ams_[0]->max_hist();
+    AndersonMixingDevice::launch_batch(st, rep_items_.as<const AmBatchItem>(), NR_, ams_[0]->n_var(), mh, mh, mh + 1,
+                                       am_ws_.partials.as<double>());
+    if (!contiguous) {
+        hipLaunchKernelGGL(scatter_batch_kernel, dim3(grid_blocks(M_), NR_), dim3(256), 0, st, g,
+                           gather_dev_.as<const GatherRep>(), M_);
+        FTS_HIP_CHECK(hipGetLastError());
+    }
+}
+
+void SaddlePoint::launch_graph_a(SolverBase* solver, bool low, double tolerance, bool contiguous) {
+    RepGraph& g = rep_a_[low ? 1 : 0];
+    const std::vector<double> sig = solver->launch_signature();
+    if (!g.exec || g.solver != solver || g.sig != sig || g.tol != tolerance) {
+        // the solver issues its launches into this capture instead of replaying its own graph
+        const bool solver_graphs = solver->graphs_on();
+        if (solver_graphs) solver->set_graphs(false);
+        try {
+            capture(g.exec, [&] { issue_graph_a(solver, low, tolerance, contiguous); });
+        } catch (...) {
+            if (solver_graphs) solver->set_graphs(true);
+            throw;
+        }
+        if (solver_graphs) solver->set_graphs(true);
+        g.solver = solver;
+        g.sig = sig;
+        g.tol = tolerance;
+    }
+    FTS_HIP_CHECK(hipGraphLaunch(g.exec, ctx_->stream));
+    solver->results_changed();
+}
+
+void SaddlePoint::launch_graph_b(bool contiguous) {
+    if (!rep_b_) capture(rep_b_, [&] { issue_graph_b(contiguous); });
+    FTS_HIP_CHECK(hipGraphLaunch(rep_b_, ctx_->stream));
+}
+
+// The replica search with every iteration replayed from graphs A and B: the host only
+// reads the errors back (one transfer per iteration), switches the precision and stops
+// when every replica has converged. The results equal those of the launches issued one
+// by one: the plan kernel makes the host loop's decisions in the same arithmetic.
+void SaddlePoint::find_replicas_graph(double tolerance, int max_iter, bool contiguous, std::vector<int>& done,
+                                      std::vector<double>& err, std::vector<double>& ham) {
+    setup_replica_graphs(contiguous);
+    hipStream_t st = ctx_->stream;
+    hipLaunchKernelGGL(replica_reset_kernel, dim3(ceil_div(NR_, 64)), dim3(64), 0, st, rep_state_.as<RepAmState>(),
+                       NR_, ams_[0]->mix_init());
+    FTS_HIP_CHECK(hipGetLastError());
+    const double* status = static_cast<const double*>(rep_status_host_.host());
+    auto read_status = [&] {
+        FTS_HIP_CHECK(hipMemcpyAsync(rep_status_host_.host(), rep_status_.get(), sizeof(double) * 2 * NR_,
+                                     hipMemcpyDeviceToHost, st));
+        FTS_HIP_CHECK(hipStreamSynchronize(st));
+        for (int r = 0; r < NR_; r++) {
+            err[r] = status[r];
+            done[r] = (int)status[NR_ + r];
+        }
+    };
+    bool low = lo_ != nullptr;
+    for (int it = 1; it <= max_iter; it++) {
+        // the last permitted iteration always evaluates in FP64
+        if (it == max_iter) low = false;
+        SolverBase* solver = low ? lo_ : solver_;
+        launch_graph_a(solver, low, tolerance, contiguous);
+        if (it == max_iter) {
+            // the Hamiltonian of a replica that stops unconverged is that of the last
+            // evaluation, before its last mixing step moves the fields
+            read_status();
+            std::vector<int> open;
+            for (int r = 0; r < NR_; r++)
+                if (!done[r]) open.push_back(r);
+            if (!open.empty()) hamiltonians(open, solver, ham);
+        }
+        launch_graph_b(contiguous);
+        read_status();
+        int active = 0;
+        for (int r = 0; r < NR_; r++)
+            if (!done[r]) active++;
+        if (active == 0) break;
+        if (low) {
+            bool below = true;
+            for (int r = 0; r < NR_; r++)
+                if (!done[r] && !(err[r] < lo_switch_)) below = false;
+            low = !below;
+        }
+    }
+    // converged replicas kept their fields since their last evaluation, whose partition
+    // functions every later evaluation reproduced: their Hamiltonians now
+    std::vector<int> conv;
+    for (int r = 0; r < NR_; r++)
+        if (done[r]) conv.push_back(r);
+    if (!conv.empty()) hamiltonians(conv, solver_, ham);
+}
+
 void SaddlePoint::find(double tolerance, int max_iter, int verbose, double* log, int* n_iter, double* error,
                        double* hamiltonian_out) {
     FTS_REQUIRE(max_iter >= 1, "max_iter must be positive");
@@ -538,9 +862,11 @@ void SaddlePoint::find(double tolerance, int max_iter, int verbose, double* log,
     // replicas mix in batched launches (mixings with history rings in device memory)
     bool batched = NR_ > 1;
     for (auto* a : ams_) batched = batched && !a->reduce_memory();
+    const bool graphs = batched && use_replica_graphs();
+    if (graphs) find_replicas_graph(tolerance, max_iter, contiguous, done, err, ham);
     // low-precision phase while the errors are large (none when no low-precision solver is set)
     bool low = lo_ != nullptr;
-    for (int it = 1; it <= max_iter; it++) {
+    for (int it = 1; it <= max_iter && !graphs; it++) {
         // the last permitted iteration always evaluates in FP64: an error measured by
         // the low-precision solver never ends the loop as a converged saddle point
         if (it == max_iter) low = false;
@@ -670,6 +996,25 @@ void SaddlePoint::langevin_update(const double* noise, bool on_device, double dt
     hipStream_t st = ctx_->stream;
     FTS_HIP_CHECK(hipMemcpyAsync(noise_cur_.get(), noise, sizeof(double) * M_ * R_ * NR_,
                                  on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st));
+    if (use_replica_graphs()) {
+        // one graph per order of the two noise buffers, which swap every step
+        LmGraph& g = rep_lm_[noise_prev_.get() < noise_cur_.get() ? 0 : 1];
+        if (!g.exec || g.prev != noise_prev_.get() || g.dt != dt) {
+            capture(g.exec, [&] { issue_langevin(dt); });
+            g.prev = noise_prev_.get();
+            g.dt = dt;
+        }
+        FTS_HIP_CHECK(hipGraphLaunch(g.exec, st));
+    } else {
+        issue_langevin(dt);
+    }
+    // the current noise becomes the previous one (lfts.py:712)
+    std::swap(noise_prev_, noise_cur_);
+    FTS_HIP_CHECK(hipStreamSynchronize(st));
+}
+
+void SaddlePoint::issue_langevin(double dt) {
+    hipStream_t st = ctx_->stream;
     // dH/dw of the real fields (no sign flip) from the current w_aux and phi, then the
     // update, each for every replica in one launch
     run_lincomb(0, lam_cw_, lam_cp_, lam_c0_, R_, lam_.as<double>(), NR_, (size_t)R_ * M_);
@@ -688,9 +1033,6 @@ void SaddlePoint::langevin_update(const double* noise, bool on_device, double dt
     a.r_step = (size_t)R_ * M_;
     hipLaunchKernelGGL(lm_kernel, dim3(grid_blocks(M_), NR_), dim3(256), 0, st, a, M_);
     FTS_HIP_CHECK(hipGetLastError());
-    // the current noise becomes the previous one (lfts.py:712)
-    std::swap(noise_prev_, noise_cur_);
-    FTS_HIP_CHECK(hipStreamSynchronize(st));
 }
 
 void SaddlePoint::prepare_guess(Cnn& cnn) {

@@ -29,6 +29,13 @@ public:
     void update_laplacian() override {
         if (real_) build_real_tables();
         else bt_.build(*g_, bsq_, ds_);
+        tables_version_++;
+    }
+    bool graphs_on() const override { return graphs_; }
+    void results_changed() override { Q_valid_ = false; }
+    std::vector<double> launch_signature() const override {
+        const GraphKey k = graph_key(nullptr, nullptr);
+        return {k.volume, (double)k.bd, (double)k.nl, (double)k.bd_fwd, (double)k.bd_inv2, (double)tables_version_};
     }
     void compute_propagators(const double* const* w, const double* const* q_init, bool on_device) override;
     void compute_concentrations() override;
@@ -160,6 +167,7 @@ private:
     std::vector<double> dqdl_;
     std::vector<double> Qh_;
     bool Q_valid_ = false;
+    long long tables_version_ = 0;   // bumped by update_laplacian (tables rebuilt)
 };
 
 template <typename T>

@@ -725,6 +725,7 @@ void DeviceSaddlePoint::set_low_precision_solver(PropagatorComputation* lo, doub
     check_fts(fts_saddle_set_low_precision(saddle_, lo ? lo->handle() : nullptr, switch_error));
 }
 
+void DeviceSaddlePoint::set_graphs(bool on) { check_fts(fts_saddle_set_graphs(saddle_, on ? 1 : 0)); }
 void DeviceSaddlePoint::prepare_guess(LearnedGuess& g) { check_fts(fts_saddle_prepare_guess(saddle_, g.handle())); }
 void DeviceSaddlePoint::predict_guess(LearnedGuess& g, int mode) {
     check_fts(fts_saddle_predict_guess(saddle_, g.handle(), mode));

@@ -218,6 +218,8 @@ public:
     std::vector<double> guess_sample();
     // iterations on `lo` (an FP32 solver of the same molecules) while the error is at or above switch_error
     void set_low_precision_solver(PropagatorComputation* lo, double switch_error);
+    // replicas: saddle iterations and Langevin updates replayed from captured HIP graphs (default on)
+    void set_graphs(bool on);
     double train_guess(LearnedGuess& g, double lr, int iters, int replay);
 
 private:

@@ -373,6 +373,7 @@ PYBIND11_MODULE(langevinfts, m) {
              })
         .def("set_low_precision_solver", &DeviceSaddlePoint::set_low_precision_solver, py::arg("solver"),
              py::arg("switch_error"), py::keep_alive<1, 2>())
+        .def("set_graphs", &DeviceSaddlePoint::set_graphs, py::arg("on"))
         .def("prepare_guess", &DeviceSaddlePoint::prepare_guess)
         .def("predict_guess", &DeviceSaddlePoint::predict_guess, py::arg("guess"), py::arg("mode") = 0)
         .def(

@@ -143,3 +143,51 @@ def test_c4b_replicas_at_config_shape(tmp_path):
         f"max |dw| {np.max(np.abs(w_ref - rep_fields[1][0])):.1e}")
     assert n_ref == rep_its[1][0] and abs(e_ref / rep_errs[1][0] - 1) < 1e-5
     np.testing.assert_allclose(w_ref, rep_fields[1][0], rtol=0, atol=1e-6)
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_replica_graphs_equal_launches_one_by_one(tmp_path, mixed):
+    """The replica search replayed from captured HIP graphs (the mixing decisions made by
+    a planning kernel on the device) against the same launches issued one by one, bit
+    for bit: iteration counts, error levels, Hamiltonians and fields over a first saddle
+    point, three Langevin steps (each a graph of the update and two graphs per
+    iteration, replayed many times with the host's own transfers and eager work in
+    between) and a search stopped unconverged at max_iter. With mixed precision the FP32
+    solver's graph and the switch to FP64 take part."""
+    from langevin_fts_amd import lfts
+
+    nx, lx = [16, 12, 10], [3.2, 2.8, 2.4]
+    n_rep, seed, n_steps = 4, 60, 3
+    M = int(np.prod(nx))
+    init = np.random.default_rng(9).standard_normal((n_rep, 2, M))
+    runs = {}
+    for graphs in (True, False):
+        say(f"replica graphs {graphs} mixed {mixed}")
+        p = params(nx, lx, tmp_path)
+        p["saddle"]["mixed_precision"] = mixed
+        sim = lfts.LFTS(p, random_seed=seed, n_replicas=n_rep)
+        dev = sim.saddle_device
+        dev.set_graphs(graphs)
+        dev.set_fields(np.ascontiguousarray([sim.mpt.to_aux_fields(init[r]) for r in range(n_rep)]))
+        dev.set_noise_prev(np.zeros((n_rep, 1, M)))
+        its, errs, hams, _ = dev.find_saddle_point(1e-6, 200, 0)
+        rec = [(list(its), list(errs), list(hams), dev.get_fields())]
+        for _ in range(n_steps):
+            its, errs, ok = sim.replica_step()
+            assert all(ok)
+            dev.field_moments()   # eager work of the program's own between the replays
+            rec.append((list(its), list(errs), None, dev.get_fields()))
+        # a search that stops unconverged: the Hamiltonian of the last evaluation, then
+        # the last mixing step still moves the fields (lfts.py:806-867)
+        dev.backup()
+        dev.langevin_update(np.random.default_rng(10).normal(0.0, 1.0, (n_rep, 1, M)), 0.5)
+        its, errs, hams, _ = dev.find_saddle_point(1e-12, 5, 0)
+        assert list(its) == [5] * n_rep
+        rec.append((list(its), list(errs), list(hams), dev.get_fields()))
+        runs[graphs] = rec
+    for k, (a, b) in enumerate(zip(runs[True], runs[False])):
+        say(f"  search {k}: iterations {a[0]} / {b[0]}")
+        assert a[0] == b[0]
+        assert a[1] == b[1]
+        assert a[2] == b[2] or (a[2] is None and b[2] is None)
+        assert np.array_equal(a[3], b[3])
