@@ -217,6 +217,8 @@ def guess_initial_params(seed, n_in=1):
 
 
 GUESS_MODES = {"absolute": 0, "delta": 1, "residual": 2}
+# training steps whose samples' mean deviations become the network's scales
+GUESS_SCALE_STEPS = 8
 
 
 class NoiseStream:
@@ -441,6 +443,9 @@ class LFTS:
             assert self.learned["inputs"] in GUESS_MODES, "inputs must be delta, absolute or residual"
             self.learned.setdefault("seed", 0)
         self.guess = None
+        self.guess_trained_steps = 0
+        self.guess_losses = []          # the newest sample's loss at each training step
+        self.guess_scale_samples = []   # sample deviations collected before the network exists
         self.saddle = params["saddle"].copy()
         self.recording = params["recording"].copy()
         self.cb = cb
@@ -481,24 +486,25 @@ class LFTS:
         return self._phi_from_device(), hamiltonian, n_iter, error_level
 
     # ---------------------------------------------------------------- learned guess
-    def init_guess(self):
-        """The network and its scales, with He-normal weights from the configured seed.
-        One input (delta, absolute): w- -> w+, scaled by the current fields' standard
-        deviations. Residual: [w-, first residual, starting w+] -> converged w+ minus the
-        starting w+, scaled by the standard deviations of the first training sample."""
+    def guess_sample_stds(self):
+        """Standard deviations of the current training sample: the network's inputs, then
+        its target. One input (delta, absolute): w- and w+. Residual: w-, the first
+        residual, the starting w+ and the converged w+ minus the starting w+."""
         if self.learned["inputs"] == "residual":
-            sample = self.saddle_device.guess_sample()
-            in_scales = [float(np.std(sample[c])) or 1.0 for c in range(3)]
-            out_scale = float(np.std(sample[3])) or 1.0
-        else:
-            w_aux = self.saddle_device.get_fields()
-            ri, ii = self.mpt.aux_fields_real_idx[0], self.mpt.aux_fields_imag_idx[0]
-            in_scales = [float(np.std(w_aux[ri])) or 1.0]
-            out_scale = float(np.std(w_aux[ii])) or 1.0
+            return [float(np.std(row)) for row in self.saddle_device.guess_sample()]
+        w_aux = self.saddle_device.get_fields()
+        ri, ii = self.mpt.aux_fields_real_idx[0], self.mpt.aux_fields_imag_idx[0]
+        return [float(np.std(w_aux[ri])), float(np.std(w_aux[ii]))]
+
+    def init_guess(self, stds=None):
+        """The network, with He-normal weights from the configured seed, its inputs divided
+        and its output multiplied by the given standard deviations (inputs, then target;
+        default: those of the current sample)."""
+        stds = list(self.guess_sample_stds() if stds is None else stds)
+        in_scales = [float(v) or 1.0 for v in stds[:-1]]
+        out_scale = float(stds[-1]) or 1.0
         self.guess = lf.LearnedGuess(self.cb.get_nx(), in_scales, out_scale)
         self.guess.set_params(guess_initial_params(self.learned["seed"], len(in_scales)))
-        self.guess_trained_steps = 0
-        self.guess_losses = []   # the newest sample's loss at each training step
         return in_scales, out_scale
 
     def _guess_before_update(self, ready):
@@ -515,12 +521,25 @@ class LFTS:
             self.saddle_device.predict_guess(self.guess, GUESS_MODES[self.learned["inputs"]])
 
     def train_guess(self):
-        """Adam steps on the current (w-, w+*) pair and on pairs replayed from a device
-        ring of recent saddle points; the network and its scales are created at the
-        first call. The learning rate falls from "lr" to "lr_final" on a cosine over
-        the "train_steps" training steps. Returns the newest pair's loss."""
-        if self.guess is None:
-            self.init_guess()
+        """Adam steps on the current sample and on samples replayed from a device ring of
+        recent saddle points. The network is created at the first call, scaled by that
+        sample's standard deviations. The deviations of the first GUESS_SCALE_STEPS
+        samples are kept. If the first sample's fall below half their mean in any input or
+        the target, the network is created again with those raised to that floor, and
+        trains on from the replay ring. After a random start the fields grow for many
+        steps (w+ 0.014 in the first sample, 3 twenty steps later at 64^3), and scales
+        that small have left the network dead. The learning rate falls from "lr" to
+        "lr_final" on a cosine over the "train_steps" training steps. Returns the newest
+        sample's loss."""
+        if len(self.guess_scale_samples) < GUESS_SCALE_STEPS:
+            self.guess_scale_samples.append(self.guess_sample_stds())
+            if self.guess is None:
+                self.init_guess(self.guess_scale_samples[0])
+            elif len(self.guess_scale_samples) == GUESS_SCALE_STEPS:
+                first = np.asarray(self.guess_scale_samples[0])
+                floor = 0.5 * np.mean(self.guess_scale_samples, axis=0)
+                if np.any(first < floor):
+                    self.init_guess(np.maximum(first, floor))
         lr0, lr1 = self.learned["lr"], self.learned["lr_final"]
         frac = min(self.guess_trained_steps / max(self.learned["train_steps"], 1), 1.0)
         lr = lr1 + 0.5 * (lr0 - lr1) * (1.0 + np.cos(np.pi * frac))
