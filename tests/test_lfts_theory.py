@@ -117,7 +117,9 @@ def test_guess_learning_rate_falls_on_a_cosine():
             return 0.0
 
     T = 100
-    sim = types.SimpleNamespace(guess=object(), guess_trained_steps=0, guess_losses=[], saddle_device=Device(),
+    # a network that exists, its scale samples collected
+    sim = types.SimpleNamespace(guess=object(), guess_trained_steps=0, guess_losses=[],
+                                guess_scale_samples=[[1.0, 1.0]] * 8, saddle_device=Device(),
                                 learned={"lr": 1e-3, "lr_final": 1e-4, "train_steps": T, "train_iters": 16,
                                          "replay": 64})
     for _ in range(T + 3):
