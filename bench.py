@@ -155,6 +155,44 @@ def langevin_steps(sim, n, stats, learned=False):
             stats["max_error"] = max(stats.get("max_error", 0.0), float(err))
 
 
+def full_run(args, sim, precision, w):
+    """--full-run N: one run of N Langevin steps timed end to end, as a user's run would go:
+    the first --guess-train-steps steps train the guess (mixed-precision saddle points),
+    the rest use it with FP64 saddle iterations. Progress goes to stderr every 250 steps."""
+    n, n_train = args.full_run, args.guess_train_steps
+    stats_train = {"iterations": 0, "failures": 0}
+    stats_run = {"iterations": 0, "failures": 0}
+    sim.saddle_device.field_moments()
+    t0 = time.perf_counter()
+    precision(True)
+    for k in range(0, n_train, 250):
+        langevin_steps(sim, min(250, n_train - k), stats_train, learned=True)
+        print(f"full run: {k + min(250, n_train - k)} training steps, {time.perf_counter() - t0:.1f} s",
+              file=sys.stderr, flush=True)
+    precision(False)
+    t1 = time.perf_counter()
+    for k in range(0, n - n_train, 250):
+        langevin_steps(sim, min(250, n - n_train - k), stats_run, learned=True)
+        print(f"full run: {n_train + k + min(250, n - n_train - k)} steps, {time.perf_counter() - t0:.1f} s",
+              file=sys.stderr, flush=True)
+    sim.saddle_device.field_moments()
+    t2 = time.perf_counter()
+    out = {
+        "metric": f"L-FTS steps/s over one whole {n}-step run with the learned guess (training included)",
+        "value": n / (t2 - t0), "unit": "steps/s", "n_gpus": 1, "steps": n, "warmup": args.warmup,
+        "ms_per_step": 1e3 * (t2 - t0) / n, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64 (training steps: f32 propagators until err < tol, then f64)",
+        "data": "synthetic: w_A, w_B ~ N(0,1) from PCG64(seed); Langevin noise from numpy PCG64",
+        "config": {"workload": describe(args.workload, w), "train_steps": n_train, "train_s": t1 - t0,
+                   "train_saddle_iterations_per_step": stats_train["iterations"] / max(n_train, 1),
+                   "learned_steps_per_s": (n - n_train) / (t2 - t1),
+                   "learned_saddle_iterations_per_step": stats_run["iterations"] / max(n - n_train, 1),
+                   "saddle_failures": stats_train["failures"] + stats_run["failures"],
+                   "train_loss_last50": float(np.mean(sim.guess_losses[-50:])) if sim.guess_losses else None},
+    }
+    return out
+
+
 def baseline_plan(w, ds=None):
     """The propagator plan of the workload's molecules from the product's analyzer
     (host code only), in the oracle's plan form."""
@@ -795,6 +833,8 @@ def main():
     ap.add_argument("--guess-inputs", default="residual", choices=["residual", "delta", "absolute"],
                     help="residual: the CNN sees w-, the first residual of the search and the starting w+ and "
                          "predicts the correction of w+; delta: w+ += f(w-_new) - f(w-_old)")
+    ap.add_argument("--full-run", type=int, default=0, metavar="N",
+                    help="lfts64: time one whole N-step run end to end instead of the arms (training included)")
     ap.add_argument("--no-graphs", action="store_true",
                     help="replicas48: issue the saddle iterations' launches one by one instead of replaying graphs")
     ap.add_argument("--sk-steps", type=int, default=20,
@@ -859,6 +899,11 @@ def main():
         warm = {"iterations": 0, "failures": 0}
         langevin_steps(sim, args.warmup, warm)
 
+        full = full_run(args, sim, precision, w) if args.full_run else None
+    if full is not None:
+        print(json.dumps(full), flush=True)
+        return
+    with stdout_to_stderr():
         if args.guess == "learned":
             # online training on the run's own saddle points (plain saddle points, then Adam steps)
             precision(True)
