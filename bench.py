@@ -837,6 +837,9 @@ def main():
                     help="lfts64: time one whole N-step run end to end instead of the arms (training included)")
     ap.add_argument("--no-graphs", action="store_true",
                     help="replicas48: issue the saddle iterations' launches one by one instead of replaying graphs")
+    ap.add_argument("--guess-use-loss", type=float, default=0.2,
+                    help="residual guess: start the training steps' saddle points from it once its recent loss "
+                         "is below this (0: only after training)")
     ap.add_argument("--sk-steps", type=int, default=20,
                     help="steps of the untimed structure-function agreement pass (0: skip)")
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"],
@@ -881,7 +884,8 @@ def main():
     params["saddle"]["mixed_precision"] = mixed
     params["learned_guess"] = {"train_steps": args.guess_train_steps, "lr": args.guess_lr,
                                "lr_final": args.guess_lr_final, "train_iters": args.guess_train_iters,
-                               "replay": args.guess_replay, "inputs": args.guess_inputs, "seed": 0}
+                               "replay": args.guess_replay, "inputs": args.guess_inputs,
+                               "use_loss": args.guess_use_loss, "seed": 0}
     with stdout_to_stderr():
         sim = lfts.LFTS(params, random_seed=args.seed)
         tol = sim.saddle["tolerance"]

@@ -219,6 +219,8 @@ def guess_initial_params(seed, n_in=1):
 GUESS_MODES = {"absolute": 0, "delta": 1, "residual": 2}
 # training steps whose samples' mean deviations become the network's scales
 GUESS_SCALE_STEPS = 8
+# training steps whose mean loss decides whether the residual guess is used while it trains
+GUESS_USE_WINDOW = 20
 
 
 class NoiseStream:
@@ -261,13 +263,15 @@ class NoiseStream:
 class LFTS:
     """L-FTS driver; `params` has the keys of the reference examples (fts/Lamella.py).
     An optional "learned_guess" entry {"train_steps", "lr", "lr_final", "train_iters",
-    "replay", "inputs", "seed"} trains a periodic 3D CNN on the saddle points of the
+    "replay", "inputs", "use_loss", "seed"} trains a periodic 3D CNN on the saddle points of the
     first train_steps accepted Langevin steps and then starts every saddle point from
     its prediction. "inputs": "delta" (default: a network f(w-) ~ w+, used as
     w+ += f(w-_new) - f(w-_old)), "absolute" (w+ = f(w-)) or "residual" (the first
     saddle iteration evaluates the previous w+ as plain mixing would; a network g of
     w-, that iteration's residual and w+ predicts the correction of w+, which replaces
-    the first mixing step)."""
+    the first mixing step). The residual guess also starts the saddle points of the
+    training steps once its recent training loss is below "use_loss" (default 0.2; 0:
+    only after training); its training samples are the same either way."""
 
     def __init__(self, params, random_seed=None, n_replicas=1):
         """n_replicas > 1: that many independent runs of the same system advance together,
@@ -426,7 +430,7 @@ class LFTS:
         self.langevin.update({"sigma": langevin_sigma})
         self.verbose_level = params["verbose_level"]
         # learned initial guess of w+ (optional):
-        # {"train_steps", "lr", "lr_final", "train_iters", "replay", "inputs", "seed"}
+        # {"train_steps", "lr", "lr_final", "train_iters", "replay", "inputs", "use_loss", "seed"}
         self.learned = dict(params["learned_guess"]) if params.get("learned_guess") else None
         if self.learned is not None:
             assert len(self.mpt.aux_fields_real_idx) == 1 and len(self.mpt.aux_fields_imag_idx) == 1, \
@@ -507,14 +511,27 @@ class LFTS:
         self.guess.set_params(guess_initial_params(self.learned["seed"], len(in_scales)))
         return in_scales, out_scale
 
+    def guess_usable(self):
+        """Whether the residual guess may start the saddle points while it still trains: its
+        newest samples' mean loss over the last GUESS_USE_WINDOW training steps is below
+        "use_loss" (in the scaled units, where 1 is about the target's variance). Its
+        training samples do not depend on this: the first evaluation is at the step's
+        starting w+ either way, and the target is the converged w+ minus that start."""
+        use_loss = self.learned.get("use_loss", 0.2)
+        return (self.guess is not None and self.learned["inputs"] == "residual" and use_loss > 0 and
+                len(self.guess_losses) >= GUESS_USE_WINDOW and
+                float(np.mean(self.guess_losses[-GUESS_USE_WINDOW:])) < use_loss)
+
     def _guess_before_update(self, ready):
         if self.learned is None:
             return
         if self.learned["inputs"] == "delta" and ready:
             self.saddle_device.prepare_guess(self.guess)
         if self.learned["inputs"] == "residual":
-            # record the first residual while training; apply the network once it is ready
-            self.saddle_device.set_residual_guess(self.guess if ready else None, not ready)
+            # record the first residual while training; apply the network once it is ready, or
+            # while it trains once its loss is low enough
+            use = ready or self.guess_usable()
+            self.saddle_device.set_residual_guess(self.guess if use else None, not ready)
 
     def _guess_after_update(self, ready):
         if ready and self.learned["inputs"] != "residual":
