@@ -281,7 +281,9 @@ __global__ void __launch_bounds__(256) wgrad_hidden(Geo g, const __bf16* dZ, con
 // one element with v_alignbit for dz = -1 and +1: 4 vector loads for 3 MFMAs.
 // partial[chunk][t][co][ci].
 __global__ void __launch_bounds__(256) wgrad_hidden_cm(Geo g, const __bf16* dZc, const __bf16* Xc, float* partial) {
-    __shared__ float red[4][3][C * C];
+    // one tap's four wavefront sums at a time: 16 KB of LDS, so that LDS does not limit
+    // the workgroups per CU
+    __shared__ float red[4][C * C];
     const int p = blockIdx.x, chunk = blockIdx.y;
     const int dx = p / 3 - 1, dy = p % 3 - 1;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -291,78 +293,124 @@ __global__ void __launch_bounds__(256) wgrad_hidden_cm(Geo g, const __bf16* dZc,
     const __bf16* dz_c = dZc + (size_t)col * g.M;
     const __bf16* x_c = Xc + (size_t)col * g.M;
     f32x16 acc0 = {}, acc1 = {}, acc2 = {};
-    for (int v0 = vbeg; v0 < vend; v0 += 16) {
-        const int v = v0 + 8 * half;
-        u32x4 a = {0, 0, 0, 0}, lo = {0, 0, 0, 0}, cu = {0, 0, 0, 0}, hi = {0, 0, 0, 0};
-        if (v < vend) {
-            a = *reinterpret_cast<const u32x4*>(dz_c + v);
+    // four 16-voxel steps' loads are in flight before their products: with two or three
+    // wavefronts per SIMD the loop is otherwise one memory latency per step
+    for (int v00 = vbeg; v00 < vend; v00 += 64) {
+        u32x4 a[4], cu[4];
+        unsigned lo[4], hi[4];   // the dwords before and after the 8 z values: z - 2, z - 1 and z + 8, z + 9
+        bool ok[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            // branch-free: 8 voxels past the end read those at vbeg and get a zero dZ operand
+            // where the products are formed (a select here would wait for the load)
+            ok[u] = v00 + 16 * u + 8 * half < vend;
+            const int v = ok[u] ? v00 + 16 * u + 8 * half : vbeg;
+            a[u] = *reinterpret_cast<const u32x4*>(dz_c + v);
             const int z = v % g.nz, r = v / g.nz;
             int xx = r / g.ny + dx, yy = r % g.ny + dy;
             xx = xx < 0 ? xx + g.nx : (xx >= g.nx ? xx - g.nx : xx);
             yy = yy < 0 ? yy + g.ny : (yy >= g.ny ? yy - g.ny : yy);
             const __bf16* line = x_c + (size_t)(xx * g.ny + yy) * g.nz;
-            lo = *reinterpret_cast<const u32x4*>(line + (z == 0 ? g.nz - 8 : z - 8));
-            cu = *reinterpret_cast<const u32x4*>(line + z);
-            hi = *reinterpret_cast<const u32x4*>(line + (z + 8 == g.nz ? 0 : z + 8));
+            lo[u] = *reinterpret_cast<const unsigned*>(line + (z == 0 ? g.nz - 2 : z - 2));
+            cu[u] = *reinterpret_cast<const u32x4*>(line + z);
+            hi[u] = *reinterpret_cast<const unsigned*>(line + (z + 8 == g.nz ? 0 : z + 8));
         }
-        // element e of a vector sits in bits 16 (e % 2) of dword e / 2
-        const u32x4 bm = {__builtin_amdgcn_alignbit(cu.x, lo.w, 16), __builtin_amdgcn_alignbit(cu.y, cu.x, 16),
-                          __builtin_amdgcn_alignbit(cu.z, cu.y, 16), __builtin_amdgcn_alignbit(cu.w, cu.z, 16)};
-        const u32x4 bp = {__builtin_amdgcn_alignbit(cu.y, cu.x, 16), __builtin_amdgcn_alignbit(cu.z, cu.y, 16),
-                          __builtin_amdgcn_alignbit(cu.w, cu.z, 16), __builtin_amdgcn_alignbit(hi.x, cu.w, 16)};
-        const bf16x8 av = __builtin_bit_cast(bf16x8, a);
-        acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, __builtin_bit_cast(bf16x8, bm), acc0, 0, 0, 0);
-        acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, __builtin_bit_cast(bf16x8, cu), acc1, 0, 0, 0);
-        acc2 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, __builtin_bit_cast(bf16x8, bp), acc2, 0, 0, 0);
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            // element e of a vector sits in bits 16 (e % 2) of dword e / 2
+            const u32x4 c = cu[u];
+            const u32x4 bm = {__builtin_amdgcn_alignbit(c.x, lo[u], 16), __builtin_amdgcn_alignbit(c.y, c.x, 16),
+                              __builtin_amdgcn_alignbit(c.z, c.y, 16), __builtin_amdgcn_alignbit(c.w, c.z, 16)};
+            const u32x4 bp = {__builtin_amdgcn_alignbit(c.y, c.x, 16), __builtin_amdgcn_alignbit(c.z, c.y, 16),
+                              __builtin_amdgcn_alignbit(c.w, c.z, 16), __builtin_amdgcn_alignbit(hi[u], c.w, 16)};
+            const bf16x8 av = __builtin_bit_cast(bf16x8, ok[u] ? a[u] : u32x4{0, 0, 0, 0});
+            acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, __builtin_bit_cast(bf16x8, bm), acc0, 0, 0, 0);
+            acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, __builtin_bit_cast(bf16x8, c), acc1, 0, 0, 0);
+            acc2 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, __builtin_bit_cast(bf16x8, bp), acc2, 0, 0, 0);
+        }
     }
     // D[i = co][j = ci]: j = lane % 32, i = 8 (r / 4) + 4 (lane / 32) + r % 4
 #pragma unroll
-    for (int r = 0; r < 16; r++) {
-        const int i = (8 * (r >> 2) + 4 * half + (r & 3)) * C + col;
-        red[wave][0][i] = acc0[r];
-        red[wave][1][i] = acc1[r];
-        red[wave][2][i] = acc2[r];
-    }
-    __syncthreads();
-    for (int i = threadIdx.x; i < 3 * C * C; i += blockDim.x) {
-        const int k = i / (C * C), e = i % (C * C);
-        partial[((size_t)chunk * TAPS + 3 * p + k) * C * C + e] =
-            ((red[0][k][e] + red[1][k][e]) + red[2][k][e]) + red[3][k][e];
+    for (int k = 0; k < 3; k++) {
+        const f32x16& acc = k == 0 ? acc0 : (k == 1 ? acc1 : acc2);
+        if (k > 0) __syncthreads();   // the previous tap's sums are read
+#pragma unroll
+        for (int r = 0; r < 16; r++) red[wave][(8 * (r >> 2) + 4 * half + (r & 3)) * C + col] = acc[r];
+        __syncthreads();
+        for (int e = threadIdx.x; e < C * C; e += blockDim.x)
+            partial[((size_t)chunk * TAPS + 3 * p + k) * C * C + e] =
+                ((red[0][e] + red[1][e]) + red[2][e]) + red[3][e];
     }
 }
 
-// First layer: G[t][c] = sum_v dZ[v][c] x[nbr(v,t)]; last layer: G[t][c] = sum_v dy[v] X[nbr(v,t)][c].
-// Thread (group, c) of the block takes every 8th voxel of the chunk and all 27 taps; the 8 groups
-// meet in LDS in a fixed order; partial[chunk][t][c].
-__global__ void __launch_bounds__(256) wgrad_edge(Geo g, int first, const __bf16* H, const float* s,
-                                                  float* partial) {
-    __shared__ float red[8][TAPS * C];
-    const int c = threadIdx.x % C, grp = threadIdx.x / C;
+// Weight gradients of the first and last layers, G[t][c] = sum_v H[v][c] s[nbr(v, t')]:
+//   first layer, one grid row k per input channel: H = dZ, s = x_k, t' = t;
+//   last layer (FLIP): H = X, the last hidden activation, s = dy, t' = 26 - t, since
+//   sum_v dy[v] X[v + o][c] = sum_u X[u][c] dy[u - o] on the periodic grid.
+// A GEMM over voxels on v_mfma_f32_32x32x2_f32 (FP32 products and sums): per instruction, lane
+// (k, j) supplies A[c = j][k] = H[v_k][j] and B[k][t = j] = s[nbr(v_k, t')] for the voxel pair
+// (v_0, v_1), so every lane issues one coalesced H load and one gather per voxel pair; taps 27..31
+// are zero. Each wavefront takes a quarter of the chunk with two accumulators; the four
+// wavefronts' sums meet in LDS in a fixed order; partial[chunk][k][t][c].
+template <bool FLIP>
+__global__ void __launch_bounds__(256) wgrad_edge(Geo g, const __bf16* H, const float* s, float* partial) {
+    __shared__ float red[4][TAPS * C];
+    const int k = blockIdx.y;
+    s += (size_t)k * g.M;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int j = lane & 31, kk = lane >> 5;
+    const bool tap = j < TAPS;
+    const int tt = FLIP ? TAPS - 1 - j : j;
+    const int dx = tt / 9 - 1, dy = (tt / 3) % 3 - 1, dz = tt % 3 - 1;
     const int vbeg = blockIdx.x * WG_CHUNK, vend = min(vbeg + WG_CHUNK, g.M);
-    float acc[TAPS] = {};
-    if (vbeg + grp < vend) {
-        Vox cv = coords(g, vbeg + grp);
-        for (int v = vbeg + grp; v < vend; v += 8) {
-            if (first) {
-                const float h = (float)H[(size_t)v * C + c];
+    const int p0 = vbeg + wave * (WG_CHUNK / 4), p1 = min(p0 + WG_CHUNK / 4, vend);
+    f32x16 acc0 = {}, acc1 = {};
+    int v = p0 + kk;
+    Vox cv = coords(g, v);
+    // eight voxel pairs' loads are in flight before their products (with few wavefronts per SIMD
+    // the loop is otherwise one memory latency per pair). Branch-free, so that no control-flow
+    // join waits for them: a voxel past the end reads voxel p0 and gets a = 0, a padding tap
+    // reads entry 0 and gets b = 0.
+    for (int p = p0; p < p1; p += 16) {
+        float a[8], b[8];
 #pragma unroll
-                for (int t = 0; t < TAPS; t++) acc[t] += h * s[neighbour(g, cv, t)];
-            } else {
-                const float sv = s[v];
+        for (int u = 0; u < 8; u++) {
+            const bool ok = v < p1;
+            int xx = cv.x + dx, yy = cv.y + dy, zz = cv.z + dz;
+            xx = xx < 0 ? xx + g.nx : (xx >= g.nx ? xx - g.nx : xx);
+            yy = yy < 0 ? yy + g.ny : (yy >= g.ny ? yy - g.ny : yy);
+            zz = zz < 0 ? zz + g.nz : (zz >= g.nz ? zz - g.nz : zz);
+            const int nb = ok && tap ? (xx * g.ny + yy) * g.nz + zz : 0;
+            const int vh = ok ? v : p0;
+            const float av = (float)H[(size_t)vh * C + j];
+            const float bv = s[nb];
+            a[u] = ok ? av : 0.f;
+            b[u] = tap ? bv : 0.f;
+            // the next voxel pair: nz >= 3, so one carry at most
+            v += 2;
+            cv.z += 2;
+            const bool wz = cv.z >= g.nz;
+            cv.z = wz ? cv.z - g.nz : cv.z;
+            cv.y += wz ? 1 : 0;
+            const bool wy = cv.y == g.ny;
+            cv.y = wy ? 0 : cv.y;
+            cv.x += wy ? 1 : 0;
+        }
 #pragma unroll
-                for (int t = 0; t < TAPS; t++) acc[t] += sv * (float)H[(size_t)neighbour(g, cv, t) * C + c];
-            }
-            if (v + 8 < vend) cv = advance(g, cv, 8);
+        for (int u = 0; u < 8; u += 2) {
+            acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a[u], b[u], acc0, 0, 0, 0);
+            acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a[u + 1], b[u + 1], acc1, 0, 0, 0);
         }
     }
+    // D[c][t]: register r of lane (kk, j) holds row c = 8 (r / 4) + 4 kk + r % 4, column t = j
+    if (tap) {
 #pragma unroll
-    for (int t = 0; t < TAPS; t++) red[grp][t * C + c] = acc[t];
-    __syncthreads();
-    for (int i = threadIdx.x; i < TAPS * C; i += blockDim.x) {
-        float sum = 0.f;
-        for (int k = 0; k < 8; k++) sum += red[k][i];
-        partial[(size_t)blockIdx.x * TAPS * C + i] = sum;
+        for (int r = 0; r < 16; r++) red[wave][j * C + 8 * (r >> 2) + 4 * kk + (r & 3)] = acc0[r] + acc1[r];
     }
+    __syncthreads();
+    for (int i = threadIdx.x; i < TAPS * C; i += blockDim.x)
+        partial[((size_t)blockIdx.x * gridDim.y + k) * TAPS * C + i] =
+            ((red[0][i] + red[1][i]) + red[2][i]) + red[3][i];
 }
 
 // Bias gradients: channel sums of dZ (C channels) or of dy (1 channel); partial[chunk][c]
@@ -470,7 +518,7 @@ Cnn::Cnn(fts_context* ctx, int nx, int ny, int nz, int n_in, const double* in_sc
         gcm_a_.allocate(sizeof(__bf16) * (size_t)M_ * C);
         gcm_b_.allocate(sizeof(__bf16) * (size_t)M_ * C);
     }
-    partial_.allocate(sizeof(float) * (size_t)std::max(TAPS * n_chunks_ * C * C, n_chunks_ * TAPS * C));
+    partial_.allocate(sizeof(float) * (size_t)std::max(TAPS * n_chunks_ * C * C, n_chunks_ * n_in_ * TAPS * C));
     loss_partials_.allocate(sizeof(double) * 256);
     loss_.allocate(sizeof(double));
     stage_in_.allocate(sizeof(double) * (size_t)n_in * M_);
@@ -580,7 +628,8 @@ double Cnn::gradients(const double* w_minus, const double* w_plus_target, bool o
     __bf16* a = acts_.as<__bf16>();
     const size_t AS = (size_t)M_ * C;
     // last layer: weights, bias, then dZ of the last hidden activation
-    hipLaunchKernelGGL(wgrad_edge, dim3(n_chunks_), dim3(256), 0, st, g, 0, a + NHID * AS, dy_.as<float>(), part);
+    hipLaunchKernelGGL(wgrad_edge<true>, dim3(n_chunks_), dim3(256), 0, st, g, a + NHID * AS, dy_.as<float>(),
+                       part);
     hipLaunchKernelGGL(reduce_chunks, dim3(blocks(TAPS * C)), dim3(256), 0, st, part, n_chunks_, TAPS * C, TAPS * C,
                        gr + L.wl);
     hipLaunchKernelGGL(bias_grad, dim3(n_chunks_), dim3(256), 0, st, g, (const __bf16*)nullptr, dy_.as<float>(), part);
@@ -614,12 +663,9 @@ double Cnn::gradients(const double* w_minus, const double* w_plus_target, bool o
         std::swap(dz_cm, other_cm);
     }
     // first layer
-    for (int c = 0; c < n_in_; c++) {
-        hipLaunchKernelGGL(wgrad_edge, dim3(n_chunks_), dim3(256), 0, st, g, 1, dz, x_.as<float>() + (size_t)c * M_,
-                           part);
-        hipLaunchKernelGGL(reduce_chunks, dim3(blocks(TAPS * C)), dim3(256), 0, st, part, n_chunks_, TAPS * C,
-                           TAPS * C, gr + L.w0 + c * TAPS * C);
-    }
+    hipLaunchKernelGGL(wgrad_edge<false>, dim3(n_chunks_, n_in_), dim3(256), 0, st, g, dz, x_.as<float>(), part);
+    hipLaunchKernelGGL(reduce_chunks, dim3(blocks(n_in_ * TAPS * C)), dim3(256), 0, st, part, n_chunks_,
+                       n_in_ * TAPS * C, n_in_ * TAPS * C, gr + L.w0);
     hipLaunchKernelGGL(bias_grad, dim3(n_chunks_), dim3(256), 0, st, g, dz, (const float*)nullptr, part);
     hipLaunchKernelGGL(reduce_chunks, dim3(1), dim3(256), 0, st, part, n_chunks_, C, C, gr + L.b0);
     FTS_HIP_CHECK(hipGetLastError());
