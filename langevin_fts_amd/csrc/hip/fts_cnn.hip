@@ -136,25 +136,45 @@ __global__ void __launch_bounds__(256) conv_hidden(Geo g, const __bf16* X, const
     const int vi = v0 + col;   // voxel of this lane's A rows
     const Vox ci = coords(g, vi);
     f32x16 acc = {};
-    for (int t = 0; t < TAPS; t++) {
-        const __bf16* xa = X + (size_t)neighbour(g, ci, t) * C + half * 8;
-        const __bf16* wb = W + ((size_t)t * C + col) * C + half * 8;
+    // the three z taps of one (dx, dy) issue their 12 loads before their 6 products
+#pragma unroll 1
+    for (int t0 = 0; t0 < TAPS; t0 += 3) {
+        bf16x8 a[3][2], b[3][2];
 #pragma unroll
-        for (int ch = 0; ch < 2; ch++) {
-            const bf16x8 a = *reinterpret_cast<const bf16x8*>(xa + ch * 16);
-            const bf16x8 b = *reinterpret_cast<const bf16x8*>(wb + ch * 16);
-            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+        for (int u = 0; u < 3; u++) {
+            const __bf16* xa = X + (size_t)neighbour(g, ci, t0 + u) * C + half * 8;
+            const __bf16* wb = W + ((size_t)(t0 + u) * C + col) * C + half * 8;
+#pragma unroll
+            for (int ch = 0; ch < 2; ch++) {
+                a[u][ch] = *reinterpret_cast<const bf16x8*>(xa + ch * 16);
+                b[u][ch] = *reinterpret_cast<const bf16x8*>(wb + ch * 16);
+            }
         }
+#pragma unroll
+        for (int u = 0; u < 3; u++)
+#pragma unroll
+            for (int ch = 0; ch < 2; ch++) acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[u][ch], b[u][ch], acc, 0, 0, 0);
     }
     // D[i][j]: j = lane % 32 (output channel), i = 8 (r / 4) + 4 (lane / 32) + r % 4 (voxel)
     const float bj = bias ? bias[col] : 0.f;
+    // the ReLU mask of the backward pass, all 16 entries loaded before they are used
+    bool keep[16];
+#pragma unroll
+    for (int r = 0; r < 16; r++) keep[r] = true;
+    if (mask) {
+        __bf16 mk[16];
+#pragma unroll
+        for (int r = 0; r < 16; r++) mk[r] = mask[(size_t)(v0 + 8 * (r >> 2) + 4 * half + (r & 3)) * C + col];
+#pragma unroll
+        for (int r = 0; r < 16; r++) keep[r] = (float)mk[r] > 0.f;
+    }
     __bf16 h[16];
 #pragma unroll
     for (int r = 0; r < 16; r++) {
         const int v = v0 + 8 * (r >> 2) + 4 * half + (r & 3);
         float val = acc[r] + bj;
         if (relu) val = fmaxf(val, 0.f);
-        if (mask && !((float)mask[(size_t)v * C + col] > 0.f)) val = 0.f;
+        if (!keep[r]) val = 0.f;
         h[r] = (__bf16)val;
         Y[(size_t)v * C + col] = h[r];
     }
