@@ -194,6 +194,28 @@ void launch_xpass_wl_nl(const Grid& g, int nl, const XBatch<T>& b, const T* bx, 
     throw Error("no wave-local pencil kernel of " + std::to_string(nl) + " columns");
 }
 
+template <typename T, int N, int NL>
+int launch_stress_wl(const Grid& g, const StressBatch<T>& b, const StressGeom& sg, const T* bx, const T* by,
+                     const T* bz, double* partials) {
+    using G = WPencil<N, NL>;
+    const dim3 grid(N * G::TILES, b.n), block(G::BD);
+    hipLaunchKernelGGL((stress_pencil_wl<T, N, NL>), grid, block, G::lds_elems * sizeof(typename vec2<T>::type),
+                       g.stream(), b, sg, g.tw<T>(0), bx, by, bz, partials);
+    return N * G::TILES;
+}
+
+template <typename T, int N>
+int launch_stress_wl_nl(const Grid& g, int nl, const StressBatch<T>& b, const StressGeom& sg, const T* bx,
+                        const T* by, const T* bz, double* partials) {
+    switch (nl) {
+        case 4: return launch_stress_wl<T, N, 4>(g, b, sg, bx, by, bz, partials);
+        case 8: return launch_stress_wl<T, N, 8>(g, b, sg, bx, by, bz, partials);
+        case 16: return launch_stress_wl<T, N, 16>(g, b, sg, bx, by, bz, partials);
+        default: break;
+    }
+    throw Error("no wave-local stress kernel of " + std::to_string(nl) + " columns");
+}
+
 template <typename T>
 constexpr int prec_index() {
     return std::is_same<T, float>::value ? 1 : 0;
@@ -221,6 +243,9 @@ void allow_lds_wl() {
     allow_lds(&xpass_kernel_wl<T, N, 4>);
     allow_lds(&xpass_kernel_wl<T, N, 8>);
     allow_lds(&xpass_kernel_wl<T, N, 16>);
+    allow_lds(&stress_pencil_wl<T, N, 4>);
+    allow_lds(&stress_pencil_wl<T, N, 8>);
+    allow_lds(&stress_pencil_wl<T, N, 16>);
 }
 
 template <int N>
@@ -746,6 +771,28 @@ void Grid::launch_xpass(const XBatch<T>& b, const T* bx, const T* by, const T* b
 }
 
 template <typename T>
+int Grid::launch_stress_pencil(const StressBatch<T>& b, const StressGeom& sg, const T* bx, const T* by, const T* bz,
+                               double* partials) const {
+    require_supported<T>(*this);
+    if (!wave_local) throw Error("the fused stress pencil pass needs a wave-local grid");
+    // tiles of one 128-byte line per spectrum row (rows start on a line): narrower tiles
+    // share lines with their neighbours, which run on other XCDs with their own L2, and
+    // every line is then fetched twice
+    const int nl = 128 / (int)sizeof(typename vec2<T>::type);
+    int blocks = 0;
+    switch (ct_n) {
+        case 32: blocks = launch_stress_wl_nl<T, 32>(*this, nl, b, sg, bx, by, bz, partials); break;
+        case 48: blocks = launch_stress_wl_nl<T, 48>(*this, nl, b, sg, bx, by, bz, partials); break;
+        case 64: blocks = launch_stress_wl_nl<T, 64>(*this, nl, b, sg, bx, by, bz, partials); break;
+        case 96: blocks = launch_stress_wl_nl<T, 96>(*this, nl, b, sg, bx, by, bz, partials); break;
+        case 128: blocks = launch_stress_wl_nl<T, 128>(*this, nl, b, sg, bx, by, bz, partials); break;
+        default: throw Error("the fused stress pencil pass needs an N^3 grid of a specialised size");
+    }
+    FTS_HIP_CHECK(hipGetLastError());
+    return blocks;
+}
+
+template <typename T>
 void Grid::launch_dots(const DotBatch<T>& b, double* out) const {
     if (b.n == 0) return;
     hipLaunchKernelGGL((dot_stage1<T>), dim3(RED_BLOCKS, b.n), dim3(RED_THREADS), 0, stream(), b, M, dV,
@@ -910,6 +957,10 @@ template void Grid::launch_slab<double>(int, const SlabBatch<double>&) const;
 template void Grid::launch_slab<float>(int, const SlabBatch<float>&) const;
 template void Grid::launch_xpass<double>(const XBatch<double>&, const double*, const double*, const double*) const;
 template void Grid::launch_xpass<float>(const XBatch<float>&, const float*, const float*, const float*) const;
+template int Grid::launch_stress_pencil<double>(const StressBatch<double>&, const StressGeom&, const double*,
+                                                const double*, const double*, double*) const;
+template int Grid::launch_stress_pencil<float>(const StressBatch<float>&, const StressGeom&, const float*,
+                                               const float*, const float*, double*) const;
 template void Grid::launch_dots<double>(const DotBatch<double>&, double*) const;
 template void Grid::launch_dots<float>(const DotBatch<float>&, double*) const;
 template void Grid::fft_forward<double>(const double*, double2*, int);

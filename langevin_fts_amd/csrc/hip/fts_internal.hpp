@@ -76,6 +76,10 @@ static_assert(sizeof(SlabBatch<double>) <= 4000, "slab batch exceeds the kernel-
 static_assert(sizeof(XBatch<double>) <= 4000, "pencil batch exceeds the kernel-argument budget");
 static_assert(sizeof(DotBatch<double>) <= 4000, "dot batch exceeds the kernel-argument budget");
 
+template <typename T>
+struct StressBatch;
+struct StressGeom;
+
 class Grid {
 public:
     Grid(fts_context* ctx, int dim, const int* nx, const double* lx, const double* mask);
@@ -144,6 +148,14 @@ public:
     void launch_slab(int mode, const SlabBatch<T>& b) const;
     template <typename T>
     void launch_xpass(const XBatch<T>& b, const T* bx, const T* by, const T* bz) const;
+    // The stress of b.n terms from the slab pass's output spectra, with their x transforms
+    // fused into the reduction (wave-local grids only): 3 partial sums per workgroup into
+    // partials, term-major; returns the workgroups per term.
+    template <typename T>
+    int launch_stress_pencil(const StressBatch<T>& b, const StressGeom& sg, const T* bx, const T* by, const T* bz,
+                             double* partials) const;
+    // an upper bound of the workgroups per term of launch_stress_pencil (its tiles are 8 or 16 wide)
+    int stress_pencil_blocks_max() const { return ct_n * ((ct_n / 2 + 1 + 7) / 8); }
     // first stage into partials, second stage into out (device, b.n doubles)
     template <typename T>
     void launch_dots(const DotBatch<T>& b, double* out) const;

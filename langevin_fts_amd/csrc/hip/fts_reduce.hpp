@@ -292,6 +292,109 @@ static __global__ void __launch_bounds__(RED_BLOCKS) stress_stage2(const double*
     }
 }
 
+// The stress above with the x transforms of both spectra fused into it (wave-local N^3
+// grids): workgroup (pencil j, kz tile; term p) takes the slab pass's output columns of A and
+// of B, transforms each along x in turn and reduces its elements' terms of stress_stage1
+// without writing either spectrum. Each spectrum is read once after the slab pass, not
+// written by a pencil pass and read again. partials[(p gridDim.x + block) 3 + d].
+template <typename T, int N, int NLW>
+__global__ void __launch_bounds__(64) stress_pencil_wl(StressBatch<T> batch, StressGeom sg,
+                                                       const typename vec2<T>::type* twx, const T* bx, const T* by,
+                                                       const T* bz, double* partials) {
+    using T2 = typename vec2<T>::type;
+    using G = WPencil<N, NLW>;
+    using F = typename G::F;
+    constexpr int NL = G::NL, NZC = G::NZC;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T2* S = reinterpret_cast<T2*>(smem_raw);   // line l, element i at S[i NL + l]
+    T2* Wx = S + N * NL;
+    const int lane = threadIdx.x;
+    const StressItem<T>& it = batch.it[blockIdx.y];
+    const int j = blockIdx.x / G::TILES;
+    const int k0 = (blockIdx.x - j * G::TILES) * NL;
+    const int nl = min(NL, NZC - k0);
+    constexpr int SP = wspec_pitch(N);
+    constexpr size_t ROW = (size_t)N * SP;
+    for (int m = lane; m < N; m += 64) Wx[m] = twx[m];
+    using M0 = WStageMap<N, false, 0, NL, true>;
+    using MF = WStageMap<N, false, F::n - 1, NL, true>;
+    constexpr int R0 = F::r(0), RF = F::r(F::n - 1);
+    auto load = [&](const T2* c, T2 (&v)[M0::NT][R0]) {
+#pragma unroll
+        for (int t = 0; t < M0::NT; t++) {
+            const int g = lane + 64 * t;
+            if (M0::ok(g, nl)) {
+                const int l = M0::line(g), i0 = M0::idx(g);
+#pragma unroll
+                for (int r = 0; r < R0; r++) v[t][r] = c[(i0 + r * M0::B) * ROW + l];
+            }
+        }
+    };
+    // A's transformed columns stay in registers, at the positions where B's come out
+    T2 fa[MF::NT][RF];
+    {
+        T2 v[M0::NT][R0];
+        load(it.a + (size_t)j * SP + k0, v);
+        wave_sync();
+        wl_stages<-1, T, T2, N, false, NL, true, 1, NL, false, 0>(v, S, Wx, lane, nl, [&](auto& o) {
+#pragma unroll
+            for (int t = 0; t < MF::NT; t++)
+#pragma unroll
+                for (int r = 0; r < RF; r++) fa[t][r] = o[t][r];
+        });
+    }
+    double s0 = 0, s1 = 0, s2 = 0;
+    {
+        T2 v[M0::NT][R0];
+        load(it.b + (size_t)j * SP + k0, v);
+        wave_sync();
+        wl_stages<-1, T, T2, N, false, NL, true, 1, NL, false, 0>(v, S, Wx, lane, nl, [&](auto& o) {
+            const int jt = j > N / 2 ? N - j : j;
+#pragma unroll
+            for (int t = 0; t < MF::NT; t++) {
+                const int g = lane + 64 * t;
+                if (MF::ok(g, nl)) {
+                    const int k = k0 + MF::line(g), i0 = MF::idx(g);
+                    const double wgt = (k != 0 && 2 * k != N) ? 2.0 : 1.0;
+#pragma unroll
+                    for (int r = 0; r < RF; r++) {
+                        const int i = i0 + r * MF::B;
+                        const int it0 = i > N / 2 ? N - i : i;
+                        const T2 A = fa[t][r], B = o[t][r];
+                        double coeff = it.bsq * ((double)A.x * (double)B.x + (double)A.y * (double)B.y);
+                        if (it.table >= 0)
+                            coeff *= (double)bx[(size_t)it.table * N + i] * (double)by[(size_t)it.table * N + j] *
+                                     (double)bz[(size_t)it.table * NZC + k];
+                        s0 += coeff * (it0 * it0 * sg.f0 * wgt);
+                        s1 += coeff * (jt * jt * sg.f1 * wgt);
+                        s2 += coeff * (k * k * sg.f2 * wgt);
+                    }
+                }
+            }
+        });
+    }
+    s0 = wave_sum(s0);
+    s1 = wave_sum(s1);
+    s2 = wave_sum(s2);
+    if (lane == 0) {
+        double* p = partials + ((size_t)blockIdx.y * gridDim.x + blockIdx.x) * 3;
+        p[0] = s0;
+        p[1] = s1;
+        p[2] = s2;
+    }
+}
+
+// second stage of stress_pencil_wl: out[3 p + d] = the sum of term p's n partials, in a fixed order
+static __global__ void __launch_bounds__(256) stress_stage2_n(const double* partials, int n, double* out) {
+    __shared__ double ws[16];
+    for (int d = 0; d < 3; d++) {
+        double v = 0.0;
+        for (int i = threadIdx.x; i < n; i += blockDim.x) v += partials[((size_t)blockIdx.x * n + i) * 3 + d];
+        v = block_sum(v, ws);
+        if (threadIdx.x == 0) out[blockIdx.x * 3 + d] = v;
+    }
+}
+
 static_assert(sizeof(PhiBatch<double>) <= 4000, "phi batch exceeds the kernel-argument budget");
 static_assert(sizeof(StressBatch<double>) <= 4000, "stress batch exceeds the kernel-argument budget");
 static_assert(sizeof(SumBatch) <= 4000, "sum batch exceeds the kernel-argument budget");

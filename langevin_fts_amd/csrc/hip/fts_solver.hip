@@ -145,6 +145,7 @@ private:
     std::vector<std::vector<std::array<int, 3>>> slices_;
 
     DeviceBuffer qstore_, hstore_, w64_, w32_, ew_, qinit_, ws_, phib_, phis_, Qd_, stress_out_;
+    DeviceBuffer stress_part_;   // partial sums of the fused stress pencil pass (wave-local grids)
     T* wT_ = nullptr;
     std::vector<size_t> qoff_;
     std::map<std::pair<int, int>, size_t> hoff_;
@@ -794,6 +795,9 @@ void Solver<T>::build_stress_terms() {
         }
     }
     stress_out_.allocate(sizeof(double) * 3 * std::max<size_t>(1, sterms_.size()));
+    if (g_->wave_local && !real_ && !sterms_.empty())
+        stress_part_.allocate(sizeof(double) * 3 * (size_t)std::min(n_ws_ / 2, MAX_BATCH) *
+                              g_->stress_pencil_blocks_max());
 }
 
 template <typename T>
@@ -1049,15 +1053,26 @@ void Solver<T>::compute_stress() {
             xi.push_back({slot(2 * k + 1), -1});
             sb.it[k] = StressItem<T>{slot(2 * k), slot(2 * k + 1), t.table, t.bsq};
         }
+        const bool fused = stress_part_.get() != nullptr;
         for (size_t i = 0; i < fw.size(); i += MAX_BATCH) {
             SlabBatch<T> b{};
             b.n = (int)std::min<size_t>(MAX_BATCH, fw.size() - i);
             for (int k = 0; k < b.n; k++) b.it[k] = fw[i + k];
             g_->launch_slab<T>(SLAB_FWD, b);
+            if (fused) continue;
             XBatch<T> x{};
             x.n = b.n;
             for (int k = 0; k < b.n; k++) x.it[k] = xi[i + k];
             g_->launch_xpass<T>(x, nullptr, nullptr, nullptr);
+        }
+        if (fused) {
+            // wave-local grids: x transforms and reduction in one pass, no spectrum written
+            const int nb = g_->launch_stress_pencil<T>(sb, sg, bt_.x<T>(), bt_.y<T>(), bt_.z<T>(),
+                                                       stress_part_.as<double>());
+            hipLaunchKernelGGL(stress_stage2_n, dim3(nt), dim3(256), 0, g_->stream(), stress_part_.as<double>(), nb,
+                               stress_out_.as<double>() + t0 * 3);
+            FTS_HIP_CHECK(hipGetLastError());
+            continue;
         }
         hipLaunchKernelGGL((stress_stage1<T>), dim3(RED_BLOCKS, nt), dim3(RED_THREADS), 0, g_->stream(), sb, sg,
                            bt_.x<T>(), bt_.y<T>(), bt_.z<T>(), g_->partials.as<double>());
