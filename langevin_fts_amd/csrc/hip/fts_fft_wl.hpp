@@ -175,16 +175,19 @@ constexpr int wslab_pitch(int N, int NLY) {
     return (N == 48 && NLY == 5) ? P + 8 : P;
 }
 
-// Row pitch (complex elements) of the half spectra that the wave-local passes exchange
-// through HBM: nz/2+1 rounded up to a multiple of 8, so that every spectrum row, and
-// with it every 8-column pencil tile, starts on a 128-byte line in FP64 (64 bytes in
-// FP32). The pad columns are never read or written. FTS_SPEC_PAD=0 (a tuning define)
-// keeps nz/2+1.
+// Row pitch (complex elements T2) of the half spectra that the wave-local passes exchange
+// through HBM: nz/2+1 rounded up to a whole number of 128-byte lines (8 columns in FP64,
+// 16 in FP32), so that every spectrum row, and with it every pencil tile of that many
+// columns, starts on a line. A tile that straddles two lines, or shares one with a
+// neighbouring tile on another XCD (its own L2), fetches lines twice. The pad columns are
+// never read or written. FTS_SPEC_PAD=0 (a tuning define) keeps nz/2+1.
 #ifndef FTS_SPEC_PAD
 #define FTS_SPEC_PAD 1
 #endif
+template <typename T2>
 constexpr int wspec_pitch(int N) {
-    return FTS_SPEC_PAD ? (N / 2 + 1 + 7) / 8 * 8 : N / 2 + 1;
+    constexpr int L = 128 / (int)sizeof(T2);
+    return FTS_SPEC_PAD ? (N / 2 + 1 + L - 1) / L * L : N / 2 + 1;
 }
 
 template <int N, int BD_ = wslab_default_bd(N)>
@@ -196,8 +199,6 @@ struct WSlab {
     static constexpr int NLY = (NZC + NW - 1) / NW;   // y columns per wavefront
     static constexpr int NLZ = (NP + NW - 1) / NW;    // packed z rows per wavefront
     static constexpr int P = wslab_pitch(N, NLY);     // plane row pitch; packed z line p spans rows 2p, 2p+1
-    static constexpr int SP = wspec_pitch(N);         // spectrum row pitch in HBM
-    static constexpr int NSPEC = N * SP;              // one x-slab of the spectrum
     static constexpr size_t lds_elems = (size_t)N * P + 2 * N;
     static_assert(BD % 64 == 0 && BD <= 1024, "whole wavefronts, at most 1024 threads");
     static_assert(NW * NLY >= NZC && NW * NLZ >= NP, "the wavefronts cover every y column and packed z row");
@@ -227,7 +228,8 @@ __global__ void __launch_bounds__(BD) slab_kernel_wl(SlabBatch<T> batch, const t
     const int wave = threadIdx.x >> 6;
     const SlabItem<T>& it = batch.it[blockIdx.y];
     const size_t roff = (size_t)blockIdx.x * N * N;
-    const size_t coff = (size_t)blockIdx.x * G::NSPEC;
+    constexpr int SP = wspec_pitch<T2>(N);            // spectrum row pitch in HBM
+    const size_t coff = (size_t)blockIdx.x * N * SP;  // one x-slab of the spectrum
     const T inv_m = (T)inv_m_d;
     const bool has_w = it.w != nullptr;
     const int col0 = wave * G::NLY;
@@ -284,7 +286,7 @@ __global__ void __launch_bounds__(BD) slab_kernel_wl(SlabBatch<T> batch, const t
                 if (MA::ok(g, nc)) {
                     const int c = cb + MA::line(g), j = MA::idx(g);
 #pragma unroll
-                    for (int r = 0; r < G::YI::r(0); r++) v[t][r] = spec[(j + r * MA::B) * G::SP + c];
+                    for (int r = 0; r < G::YI::r(0); r++) v[t][r] = spec[(j + r * MA::B) * SP + c];
                 }
             }
             if (h == 0) __syncthreads();   // twiddles stored; the plane's previous users are done
@@ -474,7 +476,7 @@ __global__ void __launch_bounds__(BD) slab_kernel_wl(SlabBatch<T> batch, const t
                         const int c = cb + MO::line(g), j = MO::idx(g);
 #pragma unroll
                         for (int r = 0; r < G::YF::r(G::YF::n - 1); r++)
-                            put2(&spec[(j + r * MO::B) * G::SP + c], o[t][r]);
+                            put2(&spec[(j + r * MO::B) * SP + c], o[t][r]);
                     }
                 }
             });
@@ -581,7 +583,7 @@ __global__ void __launch_bounds__(64) xpass_kernel_wl(XBatch<T> batch, const typ
     const int j = blockIdx.x / G::TILES;
     const int k0 = (blockIdx.x - j * G::TILES) * NL;
     const int nl = min(NL, NZC - k0);
-    constexpr int SP = wspec_pitch(N);
+    constexpr int SP = wspec_pitch<T2>(N);
     constexpr size_t ROW = (size_t)N * SP;
     T2* c = it.c + (size_t)j * SP + k0;
     const int table = it.table;

@@ -352,7 +352,7 @@ Grid::Grid(fts_context* c, int d, const int* n, const double* l, const double* m
     }
     nzc = tnx[2] / 2 + 1;
     Mc = tnx[0] * tnx[1] * nzc;
-    spitch = nzc;
+    spitch = spitch32 = nzc;
     Ms = (size_t)Mc;
     // ComputationBox.cpp:47-98: dv = prod dx (times mask), volume = prod lx
     dV = 1.0;
@@ -445,8 +445,9 @@ void Grid::setup_fft() {
     wave_local = ct_n != 0 && !std::getenv("FTS_CT_KERNELS");
     if (ct_n) set_lds_attributes_ct();
     if (wave_local) {
-        spitch = wspec_pitch(ct_n);
-        Ms = (size_t)tnx[0] * tnx[1] * spitch;
+        spitch = wspec_pitch<vec2<double>::type>(ct_n);
+        spitch32 = wspec_pitch<vec2<float>::type>(ct_n);
+        Ms = (size_t)tnx[0] * tnx[1] * std::max(spitch, spitch32);
         tune_fft_schedule<double>();
         tune_fft_schedule<float>();
     }
@@ -508,7 +509,7 @@ template <typename T, int N>
 TuneChoice tune_wave_local(const Grid& g) {
     using T2 = typename vec2<T>::type;
     constexpr int NZC = N / 2 + 1, ITEMS = 2;
-    const size_t M = (size_t)N * N * N, Mc = (size_t)N * N * wspec_pitch(N);
+    const size_t M = (size_t)N * N * N, Mc = (size_t)N * N * wspec_pitch<T2>(N);
     DeviceBuffer spec, q, w, tab;
     // two spectra per item (c0, c1: the Richardson step and the two-spectrum forward
     // transform of continuous chains) and one real slice (qin and qout)
@@ -843,7 +844,8 @@ void Grid::fft_forward(const T* in, typename vec2<T>::type* out, int batch) {
     require_fft();
     using T2 = typename vec2<T>::type;
     // padded rows: transform into a workspace of the padded layout, then copy the rows out
-    const bool padded = spitch != nzc;
+    const int pitch = spec_pitch<T>();
+    const bool padded = pitch != nzc;
     DeviceBuffer tmp;
     if (padded) tmp.allocate(sizeof(T2) * Ms * std::min(batch, MAX_BATCH));
     for (int b0 = 0; b0 < batch; b0 += MAX_BATCH) {
@@ -863,7 +865,7 @@ void Grid::fft_forward(const T* in, typename vec2<T>::type* out, int batch) {
         launch_xpass<T>(xb, nullptr, nullptr, nullptr);
         for (int b = 0; padded && b < nb; b++)
             FTS_HIP_CHECK(hipMemcpy2DAsync(out + (size_t)(b0 + b) * Mc, sizeof(T2) * nzc, tmp.as<T2>() + (size_t)b * Ms,
-                                           sizeof(T2) * spitch, sizeof(T2) * nzc, (size_t)tnx[0] * tnx[1],
+                                           sizeof(T2) * pitch, sizeof(T2) * nzc, (size_t)tnx[0] * tnx[1],
                                            hipMemcpyDeviceToDevice, stream()));
     }
     if (padded) FTS_HIP_CHECK(hipStreamSynchronize(stream()));
@@ -880,10 +882,12 @@ void Grid::fft_backward_ws(const typename vec2<T>::type* in, T* out, int batch, 
     const int chunk = std::min(ws_fields, MAX_BATCH);
     for (int b0 = 0; b0 < batch; b0 += chunk) {
         const int nb = std::min(chunk, batch - b0);
-        // the caller's rows of nzc values into the workspace's rows of spitch
-        FTS_HIP_CHECK(hipMemcpy2DAsync(ws, sizeof(T2) * spitch, in + (size_t)b0 * Mc, sizeof(T2) * nzc,
-                                       sizeof(T2) * nzc, (size_t)tnx[0] * tnx[1] * nb, hipMemcpyDeviceToDevice,
-                                       stream()));
+        // the caller's rows of nzc values into the workspace's rows of this precision's
+        // pitch, one spectrum per slot of Ms elements (Ms fits the wider pitch of the two)
+        for (int b = 0; b < nb; b++)
+            FTS_HIP_CHECK(hipMemcpy2DAsync(ws + (size_t)b * Ms, sizeof(T2) * spec_pitch<T>(),
+                                           in + (size_t)(b0 + b) * Mc, sizeof(T2) * nzc, sizeof(T2) * nzc,
+                                           (size_t)tnx[0] * tnx[1], hipMemcpyDeviceToDevice, stream()));
         SlabBatch<T> sb{};
         XBatch<T> xb{};
         sb.n = xb.n = nb;

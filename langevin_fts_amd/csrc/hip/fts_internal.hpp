@@ -95,8 +95,9 @@ public:
     int M, nzc, Mc;
     // row pitch and per-field size (complex elements) of the half spectra in device
     // buffers: nzc and Mc, or the padded pitch of the wave-local kernels
-    int spitch = 0;
-    size_t Ms = 0;
+    int spitch = 0;     // FP64 spectrum row pitch (complex elements)
+    int spitch32 = 0;   // FP32 spectrum row pitch
+    size_t Ms = 0;      // spectrum slot (complex elements), for either precision
     double volume;
     double dV;
     std::vector<double> mask_h;
@@ -132,6 +133,10 @@ public:
     }
 
     hipStream_t stream() const { return ctx->stream; }
+    template <typename T>
+    int spec_pitch() const {
+        return sizeof(T) == sizeof(float) ? spitch32 : spitch;
+    }
     template <typename T>
     const T* mask() const;
     template <typename T>

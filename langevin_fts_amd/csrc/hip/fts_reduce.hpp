@@ -313,7 +313,7 @@ __global__ void __launch_bounds__(64) stress_pencil_wl(StressBatch<T> batch, Str
     const int j = blockIdx.x / G::TILES;
     const int k0 = (blockIdx.x - j * G::TILES) * NL;
     const int nl = min(NL, NZC - k0);
-    constexpr int SP = wspec_pitch(N);
+    constexpr int SP = wspec_pitch<T2>(N);
     constexpr size_t ROW = (size_t)N * SP;
     for (int m = lane; m < N; m += 64) Wx[m] = twx[m];
     using M0 = WStageMap<N, false, 0, NL, true>;

@@ -1028,7 +1028,7 @@ void Solver<T>::compute_stress() {
     sg.n1 = g_->tnx[1];
     sg.nz = g_->tnx[2];
     sg.nzc = g_->nzc;
-    sg.sp = g_->spitch;
+    sg.sp = g_->spec_pitch<T>();
     const double PI = 3.14159265358979323846;
     sg.f0 = std::pow(2 * PI / (g_->tnx[0] * g_->tdx[0]), 2);
     sg.f1 = std::pow(2 * PI / (g_->tnx[1] * g_->tdx[1]), 2);
