@@ -2,6 +2,7 @@
 #pragma once
 
 #include <array>
+#include <cstdlib>
 #include <memory>
 #include <vector>
 
@@ -441,8 +442,9 @@ private:
     // the three inputs of the residual guess (and its target): the w+ the first residual
     // was evaluated at is the backup's (from_backup) or the current one
     void residual_inputs(double* in, double* target, bool from_backup);
+    void predict_residual_correction(Cnn& cnn);   // the network's correction of w+ into guess_new_
     void apply_residual_guess(Cnn& cnn);
-    DeviceBuffer replay_, guess_old_, guess_new_, guess_in_, res_first_;
+    DeviceBuffer replay_, guess_old_, guess_new_, guess_in_, res_first_, res_w0_;
     Cnn* res_cnn_ = nullptr;
     bool res_record_ = false, has_first_res_ = false;
     bool has_backup_ = false;

@@ -915,6 +915,7 @@ void SaddlePoint::find(double tolerance, int max_iter, int verbose, double* log,
                 continue;
             }
             active++;
+            bool guess_after_mix = false;
             if (it == 1 && NR_ == 1 && (res_record_ || res_cnn_)) {
                 // the residual of the first evaluation: kept for training, and with a
                 // trained network its correction replaces this iteration's mixing
@@ -922,9 +923,19 @@ void SaddlePoint::find(double tolerance, int max_iter, int verbose, double* log,
                 if (res_first_.bytes() == 0) res_first_.allocate(sizeof(double) * M_);
                 FTS_HIP_CHECK(hipMemcpyAsync(res_first_.get(), hr0, sizeof(double) * M_, hipMemcpyDeviceToDevice, st));
                 has_first_res_ = true;
-                if (res_cnn_) {
+                if (res_cnn_ && !ams_[r]->will_insert(e)) {
                     apply_residual_guess(*res_cnn_);
                     continue;
+                }
+                if (res_cnn_) {
+                    // the first evaluation also enters the mixing history, so that the
+                    // next mixing step has the secant pair (guess - w0, its residual
+                    // change); the mixing's own new fields are then replaced by the guess
+                    predict_residual_correction(*res_cnn_);
+                    if (res_w0_.bytes() == 0) res_w0_.allocate(sizeof(double) * M_);
+                    FTS_HIP_CHECK(hipMemcpyAsync(res_w0_.get(), w_aux_.as<double>() + (size_t)imag_idx_[0] * M_,
+                                                 sizeof(double) * M_, hipMemcpyDeviceToDevice, st));
+                    guess_after_mix = true;
                 }
             }
             if (batched) {
@@ -960,6 +971,13 @@ void SaddlePoint::find(double tolerance, int max_iter, int verbose, double* log,
                     sc.dst[c] = wa(r) + (size_t)imag_idx_[c] * M_;
                 }
                 hipLaunchKernelGGL(scatter_kernel, dim3(grid_blocks(M_)), dim3(256), 0, st, sc, M_);
+                FTS_HIP_CHECK(hipGetLastError());
+            }
+            if (guess_after_mix) {
+                double* wi = w_aux_.as<double>() + (size_t)imag_idx_[0] * M_;
+                FTS_HIP_CHECK(hipMemcpyAsync(wi, res_w0_.get(), sizeof(double) * M_, hipMemcpyDeviceToDevice, st));
+                hipLaunchKernelGGL(add_in_place_kernel, dim3(grid_blocks(M_)), dim3(256), 0, st, wi,
+                                   guess_new_.as<const double>(), M_);
                 FTS_HIP_CHECK(hipGetLastError());
             }
         }
@@ -1078,11 +1096,15 @@ void SaddlePoint::set_residual_guess(Cnn* cnn, bool record) {
     res_record_ = record;
 }
 
-void SaddlePoint::apply_residual_guess(Cnn& cnn) {
+void SaddlePoint::predict_residual_correction(Cnn& cnn) {
     if (guess_in_.bytes() == 0) guess_in_.allocate(sizeof(double) * 3 * M_);
     if (guess_new_.bytes() == 0) guess_new_.allocate(sizeof(double) * M_);
     residual_inputs(guess_in_.as<double>(), nullptr, false);
     cnn.predict(guess_in_.as<const double>(), guess_new_.as<double>(), true);
+}
+
+void SaddlePoint::apply_residual_guess(Cnn& cnn) {
+    predict_residual_correction(cnn);
     hipLaunchKernelGGL(add_in_place_kernel, dim3(grid_blocks(M_)), dim3(256), 0, ctx_->stream,
                        w_aux_.as<double>() + (size_t)imag_idx_[0] * M_, guess_new_.as<const double>(), M_);
     FTS_HIP_CHECK(hipGetLastError());

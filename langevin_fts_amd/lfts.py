@@ -269,7 +269,7 @@ class LFTS:
     w+ += f(w-_new) - f(w-_old)), "absolute" (w+ = f(w-)) or "residual" (the first
     saddle iteration evaluates the previous w+ as plain mixing would; a network g of
     w-, that iteration's residual and w+ predicts the correction of w+, which replaces
-    the first mixing step). The residual guess also starts the saddle points of the
+    the first mixing step; that evaluation stays in the mixing history). The residual guess also starts the saddle points of the
     training steps once its recent training loss is below "use_loss" (default 0.2; 0:
     only after training); its training samples are the same either way."""
 
