@@ -17,7 +17,9 @@
 //                    gradients: MFMA GEMMs over voxels, split over voxel chunks and
 //                    reduced in a fixed order, with both operands read from
 //                    channel-major copies (written by the producing kernels while
-//                    training) when nz % 8 == 0
+//                    training) when nz % 8 == 0; first- and last-layer weight
+//                    gradients: GEMMs over voxels on v_mfma_f32_32x32x2_f32 (FP32
+//                    operands), one launch for all input channels
 //   optimizer        Adam on FP32 master weights, then bf16 copies for the kernels
 //
 // Tap t = 9 (dx + 1) + 3 (dy + 1) + (dz + 1); the flipped tap is 26 - t.
