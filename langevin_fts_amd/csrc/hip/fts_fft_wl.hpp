@@ -155,6 +155,12 @@ constexpr int wslab_default_bd(int N) {
     return N <= 32 ? 256 : (N <= 48 ? 384 : (N <= 64 ? 320 : 1024));
 }
 constexpr int WPENCIL_DEFAULT_NL = 8;
+// default kz columns per pencil wavefront for an N^3 grid and complex element size: 4 in
+// FP64 from 96^3 (more wavefronts; the XCD tile order keeps a line's tiles on one L2),
+// measured 4-6% faster per step than 8 there and slower at 64^3
+constexpr int wpencil_default_nl(int N, int elem_bytes) {
+    return elem_bytes == 16 && N >= 96 ? 4 : WPENCIL_DEFAULT_NL;
+}
 
 // Plane row pitch (complex elements) of the slab pass. A y column exchange puts the
 // NLY columns of a wavefront on consecutive lanes and the next butterfly on the next
@@ -567,6 +573,32 @@ struct WPencil {
     using I = WPlan<N, true>;   // first radix = last radix of F
 };
 
+// Pencil tiles narrower than a 128-byte line share that line of a spectrum row (rows start
+// on a line) with SH - 1 neighbours. Workgroups go to the 8 XCDs round-robin by linear id,
+// and each XCD has its own L2, so neighbouring tiles would each fetch the line. The grid
+// then pads a row's tiles to a whole number of lines, and xcd_tile orders the tiles so that
+// the SH tiles of one line get ids 8 apart: one XCD, one fetch.
+template <typename T2, int NL>
+constexpr int wpencil_share() {
+    return 128 / (NL * (int)sizeof(T2)) > 1 ? 128 / (NL * (int)sizeof(T2)) : 1;
+}
+template <typename T2, int N, int NL>
+constexpr int wpencil_tiles() {
+    constexpr int SH = wpencil_share<T2, NL>();
+    return ((N / 2 + 1 + NL - 1) / NL + SH - 1) / SH * SH;
+}
+// tile of block b: b = 8 SH g + 8 s + k (k < 8, s < SH) takes tile SH (8 g + k) + s; a
+// bijection when the grid's x extent is a multiple of 8 SH (N x a multiple of SH, N % 8 == 0)
+template <int SH>
+__device__ __forceinline__ int xcd_tile(int b) {
+    if constexpr (SH == 1) {
+        return b;
+    } else {
+        const int g = b / (8 * SH), r = b - g * 8 * SH;
+        return SH * (8 * g + (r & 7)) + (r >> 3);
+    }
+}
+
 template <typename T, int N, int NLW>
 __global__ void __launch_bounds__(64) xpass_kernel_wl(XBatch<T> batch, const typename vec2<T>::type* twx,
                                                       const T* bx, const T* by, const T* bz) {
@@ -575,13 +607,17 @@ __global__ void __launch_bounds__(64) xpass_kernel_wl(XBatch<T> batch, const typ
     using F = typename G::F;
     using I = typename G::I;
     constexpr int NL = G::NL, NZC = G::NZC;
+    constexpr int TP = wpencil_tiles<T2, N, NL>();
+    static_assert(N % 8 == 0, "the XCD tile order needs a grid extent of a multiple of 8 lines' tiles");
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T2* S = reinterpret_cast<T2*>(smem_raw);   // line l, element i at S[i NL + l]
     T2* Wx = S + N * NL;
     const int lane = threadIdx.x;
     const XItem<T>& it = batch.it[blockIdx.y];
-    const int j = blockIdx.x / G::TILES;
-    const int k0 = (blockIdx.x - j * G::TILES) * NL;
+    const int tb = xcd_tile<wpencil_share<T2, NL>()>(blockIdx.x);
+    const int j = tb / TP;
+    const int k0 = (tb - j * TP) * NL;
+    if (k0 >= NZC) return;   // a pad tile: the whole wavefront leaves
     const int nl = min(NL, NZC - k0);
     constexpr int SP = wspec_pitch<T2>(N);
     constexpr size_t ROW = (size_t)N * SP;

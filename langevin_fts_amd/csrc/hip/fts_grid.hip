@@ -178,7 +178,7 @@ void launch_slab_wl_bd(const Grid& g, int bd, int mode, const SlabBatch<T>& b) {
 template <typename T, int N, int NL>
 void launch_xpass_wl(const Grid& g, const XBatch<T>& b, const T* bx, const T* by, const T* bz) {
     using G = WPencil<N, NL>;
-    const dim3 grid(N * G::TILES, b.n), block(G::BD);
+    const dim3 grid(N * wpencil_tiles<typename vec2<T>::type, N, NL>(), b.n), block(G::BD);
     hipLaunchKernelGGL((xpass_kernel_wl<T, N, NL>), grid, block, G::lds_elems * sizeof(typename vec2<T>::type),
                        g.stream(), b, g.tw<T>(0), bx, by, bz);
 }
@@ -301,7 +301,8 @@ template <typename T, int N>
 void launch_xpass_ct(const Grid& g, const XBatch<T>& b, const T* bx, const T* by, const T* bz) {
     if (g.wave_local) {
         const int nl = g.pencil_nl[prec_index<T>()];
-        launch_xpass_wl_nl<T, N>(g, nl ? nl : WPENCIL_DEFAULT_NL, b, bx, by, bz);
+        launch_xpass_wl_nl<T, N>(g, nl ? nl : wpencil_default_nl(N, (int)sizeof(typename vec2<T>::type)), b, bx,
+                                 by, bz);
         return;
     }
     using G = PencilShape<N>;
@@ -541,7 +542,7 @@ TuneChoice tune_wave_local(const Grid& g) {
         xb.it[i].table = 0;
     }
     const T* bx = tab.as<T>();
-    TuneChoice c{wslab_default_bd(N), WPENCIL_DEFAULT_NL, "", {0, 0, 0}};
+    TuneChoice c{wslab_default_bd(N), wpencil_default_nl(N, (int)sizeof(T2)), "", {0, 0, 0}};
     // whole steps (slab pass then pencil pass, as the propagators alternate them)
     std::string note = "us per step of 2 propagators, slab threads/pencil columns";
     float t_def = 0.f, t_best = 1e30f;
@@ -603,7 +604,7 @@ void Grid::tune_fft_schedule() {
     const int p = prec_index<T>();
     slab_bd[p] = wslab_default_bd(ct_n);
     for (int m = 0; m < 3; m++) slab_bd_mode[p][m] = 0;
-    pencil_nl[p] = WPENCIL_DEFAULT_NL;
+    pencil_nl[p] = wpencil_default_nl(ct_n, (int)sizeof(typename vec2<T>::type));
     // Tuning picks launch shapes by wall-clock timing, so the choice can differ between
     // processes. At 32^3 and 64^3 every candidate gives bit-identical results; at 48^3,
     // 96^3 and 128^3 they differ by rounding, and a run could then follow another
@@ -634,7 +635,7 @@ void Grid::tune_fft_schedule() {
             }
         } catch (const Error& e) {
             (void)hipGetLastError();
-            c = TuneChoice{wslab_default_bd(ct_n), WPENCIL_DEFAULT_NL,
+            c = TuneChoice{wslab_default_bd(ct_n), wpencil_default_nl(ct_n, (int)sizeof(typename vec2<T>::type)),
                            std::string("defaults (tuning failed: ") + e.what() + ")", {0, 0, 0}};
         }
         hit = cache.emplace(key, c).first;

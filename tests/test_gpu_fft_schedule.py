@@ -106,17 +106,18 @@ def test_schedule_errors(capi, ctx):
 
 
 def test_tuning_off_keeps_defaults(capi, ctx):
-    """FTS_FFT_TUNE=0 at grid creation: the default shapes, no timing"""
+    """FTS_FFT_TUNE=0 at grid creation: the default shapes, no timing. The pencil pass
+    defaults to 8 kz columns per wavefront, and to 4 in FP64 from 96^3."""
     import os
 
     defaults = {32: 256, 48: 384, 64: 320, 96: 1024, 128: 1024}
     os.environ["FTS_FFT_TUNE"] = "0"
     try:
-        for n in (32, 64):
+        for n in (32, 64, 96, 128):
             g = capi.Grid(ctx, [n] * 3, [4.0] * 3)
             for precision in (capi.FTS_FP64, capi.FTS_FP32):
                 bd, nl, note = g.fft_schedule(precision)
-                assert (bd, nl) == (defaults[n], 8)
+                assert (bd, nl) == (defaults[n], 4 if precision == capi.FTS_FP64 and n >= 96 else 8)
                 assert note == "defaults"
     finally:
         os.environ.pop("FTS_FFT_TUNE", None)
