@@ -129,34 +129,11 @@ __global__ void __launch_bounds__(256) conv_first(Geo g, int n_in, const float* 
 
 // Y[v][co] = act(bias[co] + sum_t sum_ci W[t][co][ci] X[nbr(v,t)][ci]) (x [X_in[v][co] > 0] when
 // mask is given: the ReLU derivative of a backward pass). M % 32 == 0.
-__global__ void __launch_bounds__(256) conv_hidden(Geo g, const __bf16* X, const __bf16* W, const float* bias,
-                                                   const __bf16* mask, int relu, __bf16* Y, __bf16* Ycm) {
-    const int lane = threadIdx.x & 63;
-    const int v0 = (blockIdx.x * 4 + (threadIdx.x >> 6)) * 32;
-    if (v0 >= g.M) return;
-    const int half = lane >> 5, col = lane & 31;
-    const int vi = v0 + col;   // voxel of this lane's A rows
-    const Vox ci = coords(g, vi);
-    f32x16 acc = {};
-    // the three z taps of one (dx, dy) issue their 12 loads before their 6 products
-#pragma unroll 1
-    for (int t0 = 0; t0 < TAPS; t0 += 3) {
-        bf16x8 a[3][2], b[3][2];
-#pragma unroll
-        for (int u = 0; u < 3; u++) {
-            const __bf16* xa = X + (size_t)neighbour(g, ci, t0 + u) * C + half * 8;
-            const __bf16* wb = W + ((size_t)(t0 + u) * C + col) * C + half * 8;
-#pragma unroll
-            for (int ch = 0; ch < 2; ch++) {
-                a[u][ch] = *reinterpret_cast<const bf16x8*>(xa + ch * 16);
-                b[u][ch] = *reinterpret_cast<const bf16x8*>(wb + ch * 16);
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < 3; u++)
-#pragma unroll
-            for (int ch = 0; ch < 2; ch++) acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[u][ch], b[u][ch], acc, 0, 0, 0);
-    }
+// Bias, ReLU and ReLU mask of the 32 voxels v0.. x 32 output channels of one wavefront's
+// accumulator, stored channels-last and, with Ycm, channel-major.
+__device__ __forceinline__ void conv_hidden_store(const Geo& g, const f32x16& acc, int v0, int half, int col,
+                                                  const float* bias, const __bf16* mask, int relu, __bf16* Y,
+                                                  __bf16* Ycm) {
     // D[i][j]: j = lane % 32 (output channel), i = 8 (r / 4) + 4 (lane / 32) + r % 4 (voxel)
     const float bj = bias ? bias[col] : 0.f;
     // the ReLU mask of the backward pass, all 16 entries loaded before they are used
@@ -189,6 +166,129 @@ __global__ void __launch_bounds__(256) conv_hidden(Geo g, const __bf16* X, const
             *reinterpret_cast<bf16x4*>(Ycm + (size_t)col * g.M + v0 + 8 * q + 4 * half) = pk;
         }
     }
+}
+
+__global__ void __launch_bounds__(256) conv_hidden(Geo g, const __bf16* X, const __bf16* W, const float* bias,
+                                                   const __bf16* mask, int relu, __bf16* Y, __bf16* Ycm) {
+    const int lane = threadIdx.x & 63;
+    const int v0 = (blockIdx.x * 4 + (threadIdx.x >> 6)) * 32;
+    if (v0 >= g.M) return;
+    const int half = lane >> 5, col = lane & 31;
+    const int vi = v0 + col;   // voxel of this lane's A rows
+    const Vox ci = coords(g, vi);
+    f32x16 acc = {};
+    // the three z taps of one (dx, dy) issue their 12 loads before their 6 products
+#pragma unroll 1
+    for (int t0 = 0; t0 < TAPS; t0 += 3) {
+        bf16x8 a[3][2], b[3][2];
+#pragma unroll
+        for (int u = 0; u < 3; u++) {
+            const __bf16* xa = X + (size_t)neighbour(g, ci, t0 + u) * C + half * 8;
+            const __bf16* wb = W + ((size_t)(t0 + u) * C + col) * C + half * 8;
+#pragma unroll
+            for (int ch = 0; ch < 2; ch++) {
+                a[u][ch] = *reinterpret_cast<const bf16x8*>(xa + ch * 16);
+                b[u][ch] = *reinterpret_cast<const bf16x8*>(wb + ch * 16);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 3; u++)
+#pragma unroll
+            for (int ch = 0; ch < 2; ch++) acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[u][ch], b[u][ch], acc, 0, 0, 0);
+    }
+    conv_hidden_store(g, acc, v0, half, col, bias, mask, relu, Y, Ycm);
+}
+
+// The same layer with its operands staged in LDS (nz % 32 == 0, nz <= 128, ny % TILE_Y == 0):
+// a workgroup owns TILE_Y y rows of one x plane, every z, as nz/32 x TILE_Y wavefronts of 32
+// voxels. It runs the taps in three phases, one per dx: each phase copies the TILE_Y + 2
+// periodic lines of that plane around the rows and the 9 taps' weights into LDS, then every
+// wavefront takes its A and B operands there. Activation rows are read about 4.5 times instead
+// of 27 and the weights once per workgroup instead of once per wavefront. A 64-byte voxel row
+// (or weight row) is four 16-byte chunks, stored at chunk k ^ ((i / 4) % 4) for row i, so that
+// 16 lanes reading one chunk of 16 consecutive rows hit 64 distinct banks.
+constexpr int TILE_Y = 4;
+constexpr int TILE_LINES = TILE_Y + 2;          // lines of one dx plane
+constexpr int TILE_W_CHUNKS = 9 * C * 4;        // 16-byte chunks of one phase's weights
+__host__ __device__ constexpr size_t conv_tile_lds(int nz) {
+    return ((size_t)TILE_LINES * nz * 4 + TILE_W_CHUNKS) * 16;
+}
+
+__global__ void __launch_bounds__(1024) conv_hidden_tiled(Geo g, const __bf16* X, const __bf16* W,
+                                                          const float* bias, const __bf16* mask, int relu, __bf16* Y,
+                                                          __bf16* Ycm) {
+    extern __shared__ __align__(16) unsigned char conv_lds[];
+    const int nz = g.nz;
+    u32x4* LA = reinterpret_cast<u32x4*>(conv_lds);   // [line][z][4 chunks]
+    u32x4* LW = LA + TILE_LINES * nz * 4;              // [tap][co][4 chunks]
+    const int rows = g.ny / TILE_Y;
+    const int x = blockIdx.x / rows, y0 = (blockIdx.x - x * rows) * TILE_Y;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int half = lane >> 5, col = lane & 31;
+    const int wpr = nz >> 5;   // wavefronts per y row
+    const int r = wave / wpr, z0 = (wave - r * wpr) * 32;
+    const int z = z0 + col;    // voxel of this lane's A rows
+    // the halo lines split evenly over the 8 nz threads: 3 chunks each; the weights need up to 5
+    constexpr int PA = TILE_LINES * 4 / 8;
+    static_assert(TILE_LINES * 4 % 8 == 0, "the halo lines split evenly over the threads");
+    constexpr int PW = (TILE_W_CHUNKS + 255) / 256;
+    f32x16 acc = {};
+#pragma unroll 1
+    for (int dxi = 0; dxi < 3; dxi++) {
+        if (dxi > 0) __syncthreads();   // every wavefront is done with the previous phase
+        u32x4 av[PA], wv[PW];
+        int ad[PA];
+#pragma unroll
+        for (int k = 0; k < PA; k++) {
+            const int c = threadIdx.x + k * blockDim.x;
+            const int q = c & 3, vz = c >> 2;
+            const int zz = vz % nz, ry = vz / nz;
+            int xx = x + dxi - 1, yy = y0 + ry - 1;
+            xx = xx < 0 ? xx + g.nx : (xx >= g.nx ? xx - g.nx : xx);
+            yy = yy < 0 ? yy + g.ny : (yy >= g.ny ? yy - g.ny : yy);
+            av[k] = *reinterpret_cast<const u32x4*>(X + ((size_t)(xx * g.ny + yy) * nz + zz) * C + q * 8);
+            ad[k] = (ry * nz + zz) * 4 + (q ^ ((zz >> 2) & 3));
+        }
+        const u32x4* wsrc = reinterpret_cast<const u32x4*>(W) + (size_t)dxi * TILE_W_CHUNKS;
+#pragma unroll
+        for (int k = 0; k < PW; k++) {
+            const int c = threadIdx.x + k * blockDim.x;
+            if (c < TILE_W_CHUNKS) wv[k] = wsrc[c];
+        }
+#pragma unroll
+        for (int k = 0; k < PA; k++) LA[ad[k]] = av[k];
+#pragma unroll
+        for (int k = 0; k < PW; k++) {
+            const int c = threadIdx.x + k * blockDim.x;
+            const int row = c >> 2;   // tap * C + co
+            if (c < TILE_W_CHUNKS) LW[(row << 2) + ((c & 3) ^ ((row >> 2) & 3))] = wv[k];
+        }
+        __syncthreads();
+        // the three z taps of one dy read their operands before their 6 products
+#pragma unroll 1
+        for (int dyi = 0; dyi < 3; dyi++) {
+            const int line = r + dyi;   // y row r + dy + 1 of the plane's lines
+            bf16x8 a[3][2], b[3][2];
+#pragma unroll
+            for (int u = 0; u < 3; u++) {
+                int zz = z + u - 1;
+                zz = zz < 0 ? zz + nz : (zz >= nz ? zz - nz : zz);
+                const int wrow = (3 * dyi + u) * C + col;
+#pragma unroll
+                for (int ch = 0; ch < 2; ch++) {
+                    const int k = half + 2 * ch;
+                    a[u][ch] = __builtin_bit_cast(bf16x8, LA[(line * nz + zz) * 4 + (k ^ ((zz >> 2) & 3))]);
+                    b[u][ch] = __builtin_bit_cast(bf16x8, LW[(wrow << 2) + (k ^ ((wrow >> 2) & 3))]);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 3; u++)
+#pragma unroll
+                for (int ch = 0; ch < 2; ch++)
+                    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[u][ch], b[u][ch], acc, 0, 0, 0);
+        }
+    }
+    conv_hidden_store(g, acc, ((x * g.ny + y0 + r) * nz) + z0, half, col, bias, mask, relu, Y, Ycm);
 }
 
 __global__ void __launch_bounds__(256) conv_last(Geo g, const __bf16* X, const float* W, const float* b, float* y) {
@@ -534,6 +634,10 @@ Cnn::Cnn(fts_context* ctx, int nx, int ny, int nz, int n_in, const double* in_sc
     grad_a_.allocate(sizeof(__bf16) * (size_t)M_ * C);
     grad_b_.allocate(sizeof(__bf16) * (size_t)M_ * C);
     cm_ = nz % 8 == 0;
+    tiled_ = nz % 32 == 0 && nz <= 128 && ny % TILE_Y == 0;
+    if (tiled_)
+        FTS_HIP_CHECK(hipFuncSetAttribute((const void*)&conv_hidden_tiled, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                          (int)conv_tile_lds(128)));
     n_hchunks_ = (M_ + HID_CHUNK - 1) / HID_CHUNK;
     if (cm_) {
         acts_cm_.allocate(sizeof(__bf16) * (size_t)NHID * M_ * C);
@@ -570,6 +674,25 @@ void Cnn::get_grads(float* host) {
     FTS_HIP_CHECK(hipStreamSynchronize(ctx_->stream));
 }
 
+void Cnn::conv_hidden_launch(const void* X, const void* W, const float* bias, const void* mask, int relu, void* Y,
+                             void* Ycm) {
+    const Geo g{nx_, ny_, nz_, M_};
+    const __bf16* x = static_cast<const __bf16*>(X);
+    const __bf16* w = static_cast<const __bf16*>(W);
+    const __bf16* m = static_cast<const __bf16*>(mask);
+    __bf16* y = static_cast<__bf16*>(Y);
+    __bf16* ycm = static_cast<__bf16*>(Ycm);
+    if (tiled_) {
+        // nz / 32 wavefronts per y row, TILE_Y rows per workgroup (256 to 1024 threads)
+        const size_t lds = conv_tile_lds(nz_);
+        hipLaunchKernelGGL(conv_hidden_tiled, dim3(nx_ * (ny_ / TILE_Y)), dim3(8 * nz_), lds, ctx_->stream, g, x, w,
+                           bias, m, relu, y, ycm);
+    } else {
+        hipLaunchKernelGGL(conv_hidden, dim3(blocks(M_ / 32, 4)), dim3(256), 0, ctx_->stream, g, x, w, bias, m, relu,
+                           y, ycm);
+    }
+}
+
 void Cnn::refresh_weights() {
     hipLaunchKernelGGL(hidden_weights, dim3(blocks(NHID * TAPS * C * C)), dim3(256), 0, ctx_->stream,
                        params_.as<float>() + Layout(n_in_).wh, wf_.as<__bf16>(), wb_.as<__bf16>());
@@ -602,10 +725,8 @@ void Cnn::forward(bool train) {
     hipLaunchKernelGGL(conv_first, dim3(blocks(M_)), dim3(256), 0, st, g, n_in_, x_.as<float>(), p + L.w0, p + L.b0,
                        a, acm);
     for (int l = 0; l < NHID; l++)
-        hipLaunchKernelGGL(conv_hidden, dim3(blocks(M_ / 32, 4)), dim3(256), 0, st, g, a + l * AS,
-                           wf_.as<__bf16>() + (size_t)l * TAPS * C * C, p + L.wh + l * HSZ + TAPS * C * C,
-                           (const __bf16*)nullptr, 1, a + (l + 1) * AS,
-                           acm && l + 1 < NHID ? acm + (l + 1) * AS : (__bf16*)nullptr);
+        conv_hidden_launch(a + l * AS, wf_.as<__bf16>() + (size_t)l * TAPS * C * C, p + L.wh + l * HSZ + TAPS * C * C,
+                           nullptr, 1, a + (l + 1) * AS, acm && l + 1 < NHID ? acm + (l + 1) * AS : nullptr);
     hipLaunchKernelGGL(conv_last, dim3(blocks(M_)), dim3(256), 0, st, g, a + NHID * AS, p + L.wl, p + L.bl,
                        y_.as<float>());
     FTS_HIP_CHECK(hipGetLastError());
@@ -678,9 +799,8 @@ double Cnn::gradients(const double* w_minus, const double* w_plus_target, bool o
         hipLaunchKernelGGL(reduce_chunks, dim3(1), dim3(256), 0, st, part, n_chunks_, C, C,
                            gr + L.wh + l * HSZ + TAPS * C * C);
         // the next lower layer's dz, with its channel-major copy unless it is the first layer's
-        hipLaunchKernelGGL(conv_hidden, dim3(blocks(M_ / 32, 4)), dim3(256), 0, st, g, dz,
-                           wb_.as<__bf16>() + (size_t)l * TAPS * C * C, (const float*)nullptr, a + l * AS, 0, other,
-                           l > 0 ? other_cm : (__bf16*)nullptr);
+        conv_hidden_launch(dz, wb_.as<__bf16>() + (size_t)l * TAPS * C * C, nullptr, a + l * AS, 0, other,
+                           l > 0 ? other_cm : nullptr);
         std::swap(dz, other);
         std::swap(dz_cm, other_cm);
     }

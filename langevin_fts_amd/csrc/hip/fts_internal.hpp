@@ -496,6 +496,9 @@ private:
     // hidden weight gradients from channel-major operands ([C][M] copies of the layer
     // inputs and of dZ) when nz % 8 == 0, over chunks of n_hchunks_
     bool cm_ = false;
+    bool tiled_ = false;   // hidden layers through the LDS-tiled kernel (nz % 32 == 0, nz <= 128, ny % 4 == 0)
+    void conv_hidden_launch(const void* X, const void* W, const float* bias, const void* mask, int relu, void* Y,
+                            void* Ycm);
     int n_hchunks_ = 0;
     DeviceBuffer acts_cm_, gcm_a_, gcm_b_;
     std::vector<double> in_scales_;

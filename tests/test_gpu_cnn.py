@@ -34,7 +34,9 @@ def rel_l2(a, b):
     return float(np.linalg.norm(np.asarray(a) - b) / max(np.linalg.norm(b), 1e-30))
 
 
-@pytest.mark.parametrize("nx", [[16, 16, 16], [8, 12, 16]])
+# nz % 32 == 0, nz <= 128 and ny % 4 == 0 run the hidden layers through the LDS-tiled
+# kernel (256, 512 and 1024 threads per workgroup below); the others through the plain one
+@pytest.mark.parametrize("nx", [[16, 16, 16], [8, 12, 16], [8, 8, 32], [4, 8, 64], [4, 4, 128], [6, 12, 96]])
 def test_forward_matches_oracle(lf, nx):
     say(f"cnn forward {nx}")
     p = params(1)
@@ -74,9 +76,10 @@ def test_three_input_forward_and_scales(lf):
 
 # nz % 8 == 0: hidden weight gradients from channel-major copies (32^3: 8 voxel chunks);
 # nz = 12 or 6: the channels-last kernel; 4 x 4 x 6 = 96 voxels leaves half of the edge
-# kernels' second 64-voxel group past the grid's end
+# kernels' second 64-voxel group past the grid's end; 32^3 and 4 x 8 x 64 take the
+# LDS-tiled hidden layers forward and backward (with the ReLU mask)
 @pytest.mark.parametrize("nx,n_in", [([16, 16, 16], 1), ([8, 12, 16], 1), ([8, 12, 16], 3), ([32, 32, 32], 1),
-                                     ([8, 8, 12], 1), ([4, 4, 6], 3)])
+                                     ([8, 8, 12], 1), ([4, 4, 6], 3), ([4, 8, 64], 3)])
 def test_gradients_match_oracle(lf, nx, n_in):
     say(f"cnn gradients {nx} inputs {n_in}")
     p = params(5, n_in)
