@@ -102,8 +102,8 @@ int fts_grid_fft_backward_noalloc(fts_grid* g, int precision, const void* d_in, 
  * so the defaults stay there unless the environment sets FTS_FFT_TUNE=1 (a timed
  * choice could differ between processes and with it a chaotic run's trajectory).
  * FTS_FFT_TUNE=0 keeps the defaults everywhere. The default pencil_columns is 8, and
- * 4 in FP64 from 96^3; the pencil candidates give bit-identical results at every size.
- * One process keeps one choice per (N, precision). slab_threads is the half step's (discrete
+ * 4 in FP64 from 96^3. One process keeps one choice per (N, precision). slab_threads is
+ * the half step's (discrete
  * chains); the forward transform with two spectra out and the Richardson step of
  * continuous chains are timed alone and may take another candidate, which the note
  * names. get writes the choice and a note with the measured candidates
