@@ -402,11 +402,17 @@ __global__ void __launch_bounds__(256) wgrad_hidden(Geo g, const __bf16* dZ, con
 // 8-voxel vectors at z - 8, z and z + 8 of the neighbour line (periodic in z), shifted by
 // one element with v_alignbit for dz = -1 and +1: 4 vector loads for 3 MFMAs.
 // partial[chunk][t][co][ci].
-__global__ void __launch_bounds__(256) wgrad_hidden_cm(Geo g, const __bf16* dZc, const __bf16* Xc, float* partial) {
+__global__ void __launch_bounds__(256) wgrad_hidden_cm(Geo g, int n_chunks, const __bf16* dZc, const __bf16* Xc,
+                                                       float* partial) {
     // one tap's four wavefront sums at a time: 16 KB of LDS, so that LDS does not limit
     // the workgroups per CU
     __shared__ float red[4][C * C];
-    const int p = blockIdx.x, chunk = blockIdx.y;
+    // the 9 (dx, dy) blocks of a voxel chunk read the same dZ and overlapping X lines: they get
+    // linear ids 8 apart (b = 8 (9 q + p) + k for chunk 8 q + k), so that they run on one XCD of
+    // the 8, which hand out workgroups round-robin, and share its L2
+    const int k8 = blockIdx.x & 7, m = blockIdx.x >> 3;
+    const int q = m / 9, p = m - 9 * q, chunk = 8 * q + k8;
+    if (chunk >= n_chunks) return;   // pad of the last group of 8 chunks
     const int dx = p / 3 - 1, dy = p % 3 - 1;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int half = lane >> 5, col = lane & 31;
@@ -786,8 +792,8 @@ double Cnn::gradients(const double* w_minus, const double* w_plus_target, bool o
     for (int l = NHID - 1; l >= 0; l--) {
         // layer l maps a[l] -> a[l + 1]; dz = dL/d(pre-activation of a[l + 1])
         if (cm_) {
-            hipLaunchKernelGGL(wgrad_hidden_cm, dim3(TAPS / 3, n_hchunks_), dim3(256), 0, st, g, dz_cm,
-                               acts_cm_.as<const __bf16>() + l * AS, part);
+            hipLaunchKernelGGL(wgrad_hidden_cm, dim3(TAPS / 3 * 8 * ((n_hchunks_ + 7) / 8)), dim3(256), 0, st, g,
+                               n_hchunks_, dz_cm, acts_cm_.as<const __bf16>() + l * AS, part);
             hipLaunchKernelGGL(reduce_chunks, dim3(blocks(TAPS * C * C)), dim3(256), 0, st, part, n_hchunks_,
                                TAPS * C * C, TAPS * C * C, gr + L.wh + l * HSZ);
         } else {
