@@ -94,6 +94,14 @@ __device__ __forceinline__ int neighbour(const Geo& g, Vox c, int t) {
     return (xx * g.ny + yy) * g.nz + zz;
 }
 
+// Channel-major copies for the hidden weight gradients, blocked by 8 voxels: channel c of
+// voxel v at (v / 8) 8 C + 8 c + v % 8, so that the 8 voxels of a group are 16 contiguous bytes
+// per channel and the 32 channels of a group one contiguous 512-byte block. (A plain
+// [C][M] layout puts the channels M apart: a wavefront's load then touched 32 pages.)
+__device__ __forceinline__ size_t cm_at(int c, int v) {
+    return (size_t)(v >> 3) * (8 * C) + 8 * c + (v & 7);
+}
+
 // ---------------------------------------------------------------- forward
 // x: [n_in][M], W: [n_in][27][C]
 __global__ void __launch_bounds__(256) conv_first(Geo g, int n_in, const float* x, const float* W, const float* b,
@@ -122,7 +130,7 @@ __global__ void __launch_bounds__(256) conv_first(Geo g, int n_in, const float* 
         o[q] = h;
         if (out_cm) {
 #pragma unroll
-            for (int e = 0; e < 8; e++) out_cm[(size_t)(q * 8 + e) * g.M + v] = h[e];
+            for (int e = 0; e < 8; e++) out_cm[cm_at(q * 8 + e, v)] = h[e];
         }
     }
 }
@@ -163,7 +171,7 @@ __device__ __forceinline__ void conv_hidden_store(const Geo& g, const f32x16& ac
         for (int q = 0; q < 4; q++) {
             typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
             const bf16x4 pk = {h[4 * q], h[4 * q + 1], h[4 * q + 2], h[4 * q + 3]};
-            *reinterpret_cast<bf16x4*>(Ycm + (size_t)col * g.M + v0 + 8 * q + 4 * half) = pk;
+            *reinterpret_cast<bf16x4*>(Ycm + cm_at(col, v0 + 8 * q + 4 * half)) = pk;
         }
     }
 }
@@ -355,7 +363,7 @@ __global__ void __launch_bounds__(256) back_last(Geo g, const float* dy, const f
         o[q] = r;
         if (dXcm) {
 #pragma unroll
-            for (int e = 0; e < 8; e++) dXcm[(size_t)(q * 8 + e) * g.M + u] = r[e];
+            for (int e = 0; e < 8; e++) dXcm[cm_at(q * 8 + e, u)] = r[e];
         }
     }
 }
@@ -394,7 +402,7 @@ __global__ void __launch_bounds__(256) wgrad_hidden(Geo g, const __bf16* dZ, con
     for (int i = threadIdx.x; i < C * C; i += blockDim.x) out[i] = ((red[0][i] + red[1][i]) + red[2][i]) + red[3][i];
 }
 
-// The same gradient from channel-major copies dZc[C][M], Xc[C][M] (nz % 8 == 0). Block
+// The same gradient from channel-major copies dZc, Xc (blocked by 8 voxels, cm_at; nz % 8 == 0). Block
 // (p, chunk) owns the taps t = 3 p + dz + 1 (dz = -1, 0, 1) of the line offset
 // p = 3 (dx + 1) + (dy + 1); its 4 wavefronts split the chunk's voxels. A wavefront step
 // takes 16 voxels: lane (c, h) holds dZc[c][v .. v + 7], v = v0 + 8 h (the A operand, one
@@ -418,8 +426,8 @@ __global__ void __launch_bounds__(256) wgrad_hidden_cm(Geo g, int n_chunks, cons
     const int half = lane >> 5, col = lane & 31;
     const int vbeg = chunk * HID_CHUNK + wave * (HID_CHUNK / 4);
     const int vend = min(vbeg + HID_CHUNK / 4, g.M);
-    const __bf16* dz_c = dZc + (size_t)col * g.M;
-    const __bf16* x_c = Xc + (size_t)col * g.M;
+    const __bf16* dz_c = dZc + 8 * col;   // + (v / 8) 8 C for the group of voxel v
+    const __bf16* x_c = Xc + 8 * col;
     f32x16 acc0 = {}, acc1 = {}, acc2 = {};
     // four 16-voxel steps' loads are in flight before their products: with two or three
     // wavefronts per SIMD the loop is otherwise one memory latency per step
@@ -433,15 +441,18 @@ __global__ void __launch_bounds__(256) wgrad_hidden_cm(Geo g, int n_chunks, cons
             // where the products are formed (a select here would wait for the load)
             ok[u] = v00 + 16 * u + 8 * half < vend;
             const int v = ok[u] ? v00 + 16 * u + 8 * half : vbeg;
-            a[u] = *reinterpret_cast<const u32x4*>(dz_c + v);
+            a[u] = *reinterpret_cast<const u32x4*>(dz_c + (size_t)(v >> 3) * (8 * C));
             const int z = v % g.nz, r = v / g.nz;
             int xx = r / g.ny + dx, yy = r % g.ny + dy;
             xx = xx < 0 ? xx + g.nx : (xx >= g.nx ? xx - g.nx : xx);
             yy = yy < 0 ? yy + g.ny : (yy >= g.ny ? yy - g.ny : yy);
-            const __bf16* line = x_c + (size_t)(xx * g.ny + yy) * g.nz;
-            lo[u] = *reinterpret_cast<const unsigned*>(line + (z == 0 ? g.nz - 2 : z - 2));
-            cu[u] = *reinterpret_cast<const u32x4*>(line + z);
-            hi[u] = *reinterpret_cast<const unsigned*>(line + (z + 8 == g.nz ? 0 : z + 8));
+            // the line's 8-voxel groups, periodic in z: z - 2 and z - 1 end the previous group,
+            // z + 8 and z + 9 start the next
+            const __bf16* line = x_c + (size_t)((xx * g.ny + yy) * g.nz >> 3) * (8 * C);
+            const int gz = z >> 3, ng = g.nz >> 3;
+            lo[u] = *reinterpret_cast<const unsigned*>(line + (size_t)(gz == 0 ? ng - 1 : gz - 1) * (8 * C) + 6);
+            cu[u] = *reinterpret_cast<const u32x4*>(line + (size_t)gz * (8 * C));
+            hi[u] = *reinterpret_cast<const unsigned*>(line + (size_t)(gz + 1 == ng ? 0 : gz + 1) * (8 * C));
         }
 #pragma unroll
         for (int u = 0; u < 4; u++) {
