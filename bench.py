@@ -826,7 +826,7 @@ def main():
     ap.add_argument("--guess", default="learned", choices=["learned", "plain"],
                     help="learned: after the plain arm, train the CNN guess and time the steps that use it")
     ap.add_argument("--guess-train-steps", type=int, default=1000, help="Langevin steps the CNN trains on")
-    ap.add_argument("--guess-train-iters", type=int, default=16, help="Adam iterations per training step")
+    ap.add_argument("--guess-train-iters", type=int, default=48, help="Adam iterations per training step")
     ap.add_argument("--guess-lr", type=float, default=5e-4, help="initial Adam learning rate of the guess")
     ap.add_argument("--guess-lr-final", type=float, default=5e-5, help="learning rate at the end of training")
     ap.add_argument("--guess-replay", type=int, default=64, help="stored training samples replayed in training")

@@ -439,7 +439,7 @@ class LFTS:
             self.learned.setdefault("train_steps", 1000)
             self.learned.setdefault("lr", 5e-4)
             self.learned.setdefault("lr_final", self.learned["lr"] / 10)
-            self.learned.setdefault("train_iters", 16)
+            self.learned.setdefault("train_iters", 48)
             self.learned.setdefault("replay", 64)
             if "inputs" not in self.learned:
                 # the boolean form: "delta": True / False
