@@ -269,9 +269,10 @@ class LFTS:
     w+ += f(w-_new) - f(w-_old)), "absolute" (w+ = f(w-)) or "residual" (the first
     saddle iteration evaluates the previous w+ as plain mixing would; a network g of
     w-, that iteration's residual and w+ predicts the correction of w+, which replaces
-    the first mixing step; that evaluation stays in the mixing history). The residual guess also starts the saddle points of the
-    training steps once its recent training loss is below "use_loss" (default 0.2; 0:
-    only after training); its training samples are the same either way."""
+    the first mixing step; that evaluation stays in the mixing history). The residual
+    guess also starts the saddle points of the training steps once its recent training
+    loss is below "use_loss" (default 0.2; 0: only after training); its training samples
+    are the same either way. "train_iters": Adam steps per training step (48)."""
 
     def __init__(self, params, random_seed=None, n_replicas=1):
         """n_replicas > 1: that many independent runs of the same system advance together,
